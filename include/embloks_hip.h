@@ -1,0 +1,200 @@
+/*
+ * embloks_hip.h - C ABI of the MI355X (gfx950) physics backend for embloks scenes.
+ *
+ * This library replaces the reference's GPU backend `GpuSim`
+ * (crates/embloks-gpu/src/sim.rs:180-850) behind a plain C interface: upload a
+ * scene, step it, read back fields and electrons. All structs are POD; host
+ * arrays are owned by the caller and copied in or out; device memory is owned
+ * by the library. Every call returns 0 on success or a negative EMB_E* code,
+ * with a message in emb_last_error() (thread-local).
+ *
+ * Array conventions follow the reference: per-cell arrays are x-major,
+ * index = c * rows + r (crates/embloks-core/src/fields.rs:21-22). Yee arrays are
+ * shaped like `Maxwell`'s (crates/embloks-core/src/maxwell.rs:76-79):
+ *   Ex (nx + 1, ny), Ey (nx, ny + 1), Bz (nx + 1, ny + 1), potential (nx, ny),
+ * with nx = cols + 2 * margin, ny = rows + 2 * margin, index = i * width_y + j.
+ *
+ * Handles are thread-compatible, not thread-safe: one host thread drives one
+ * simulation at a time (the reference's GpuSim is used the same way,
+ * crates/embloks-web/src/app.rs:168-175).
+ */
+#ifndef EMBLOKS_HIP_H
+#define EMBLOKS_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    EMB_OK = 0,
+    EMB_E_INVALID = -1,   /* bad argument or scene */
+    EMB_E_DEVICE = -2,    /* a HIP call failed */
+    EMB_E_NOMEM = -3,     /* device memory or the admission budget is exhausted */
+    EMB_E_STATE = -4,     /* call not valid in the handle's state */
+    EMB_E_NUMERIC = -5,   /* a device-side check failed (see emb_last_error) */
+};
+
+/* Physics subset of `Settings` (crates/embloks-core/src/settings.rs:164-236).
+ * emb_settings_default fills the reference's defaults. */
+typedef struct EmbSettings {
+    double dt;
+    double cell_charge;
+    int32_t carriers_per_cell;
+    int32_t battery_density;
+    double carrier_charge_to_mass;
+    double drag_per_resistance;
+    double battery_resistance;
+    double battery_trim_time;
+    double battery_gain_low;
+    double battery_gain_high;
+    double current_smoothing_time;
+    double wall_restitution;
+    double max_step;
+    int32_t max_substeps;
+    int32_t smoothing_passes;
+    double coulomb_k;
+    double speed_of_light;
+    int32_t absorbing_margin;
+    int32_t reserved0;
+    double absorbing_strength;
+    double electron_softening;
+    double electron_cutoff;
+    double display_smoothing_time;
+} EmbSettings;
+
+/* A scene: the reference's `World` (crates/embloks-core/src/world.rs:24-38) as
+ * flat arrays. Electrons carry the World's uniform charge and mass
+ * (world.rs:59-66). `gains` lists the battery gain per stack in find_stacks
+ * order (battery.rs:100-147); NaN marks a stack without a gain entry (gain 1).
+ * `time` is the scene's simulated time (Simulation::time). */
+typedef struct EmbScene {
+    int32_t cols, rows;
+    const uint8_t *kind;        /* 0 vacuum, 1 conductor, 2 battery (materials.rs:8-12) */
+    const double *resistance;
+    const double *density;
+    const double *emf;
+    const uint8_t *orientation; /* 0 right, 1 down, 2 left, 3 up (materials.rs:30-35) */
+    const double *frequency;
+    int64_t n_electrons;
+    const double *pos;          /* [n_electrons][2] */
+    const double *vel;          /* [n_electrons][2] */
+    int64_t n_fixed;
+    const double *fixed_pos;    /* [n_fixed][2] */
+    const double *fixed_charge; /* [n_fixed] */
+    int32_t n_gains;
+    int32_t reserved0;
+    const double *gains;        /* [n_gains], NaN = no entry */
+    double time;
+} EmbScene;
+
+typedef struct EmbDims {
+    int32_t cols, rows, margin, nx, ny;
+    int32_t n_stacks;
+    int64_t n_electrons;
+    int32_t fdtd_substeps;
+    int32_t last_substeps;      /* electron substeps of the most recent tick */
+    double time;
+} EmbDims;
+
+typedef struct EmbDevice EmbDevice;
+typedef struct EmbSim EmbSim;
+
+const char *emb_last_error(void);
+void emb_settings_default(EmbSettings *out);
+
+/* Device lifecycle; the device handle is shared by every simulation on it, as
+ * `Kernels` is in the reference (crates/embloks-gpu/src/sim.rs:164-178). */
+int emb_device_open(int ordinal, EmbDevice **out);
+int emb_device_close(EmbDevice *dev);
+
+/* GpuSim::new / with_kernels + upload (sim.rs:214-370): builds the tables,
+ * uploads the scene and brings the fields to the static field of its charges
+ * (Simulation::new, crates/embloks-core/src/simulation.rs:25-30). */
+int emb_sim_create(EmbDevice *dev, const EmbSettings *settings, const EmbScene *scene, EmbSim **out);
+/* GpuSim::upload (sim.rs:395-529): replace electrons, materials, gains and
+ * static charges of an existing simulation of the same grid; the fields carry
+ * on, corrected for Gauss's law with the new charges (Simulation::sync). */
+int emb_sim_upload(EmbSim *sim, const EmbScene *scene, int reset_display);
+/* GpuSim::encode_ticks + submit (sim.rs:607-661, 784-791): enqueue `ticks`
+ * physics ticks (Simulation::step, simulation.rs:63-72) on the sim's stream.
+ * Asynchronous; emb_sim_sync waits. */
+int emb_sim_step(EmbSim *sim, int ticks);
+int emb_sim_sync(EmbSim *sim);
+int emb_sim_dims(EmbSim *sim, EmbDims *out);
+/* read_state (sim.rs:719-768): electrons in upload order and the gains per
+ * stack (NaN where the stack has no gain entry). */
+int emb_sim_read_state(EmbSim *sim, double *pos, double *vel, double *gains);
+/* read_yee (sim.rs:818-823). */
+int emb_sim_read_yee(EmbSim *sim, double *ex, double *ey, double *bz);
+/* read_potential (sim.rs:826-830): the potential of the charges (nx, ny). */
+int emb_sim_read_potential(EmbSim *sim, double *potential);
+/* read_current (sim.rs:843-849): physics::current_density, (cols, rows, 2). */
+int emb_sim_read_current(EmbSim *sim, double *current);
+/* The charge deposit as the int64 fixed-point grid the kernels hold: electron
+ * charge in units of 2^-40 of one electron (cols * rows, unsmoothed, from the
+ * electrons' current positions). No counterpart in the reference, which
+ * deposits in floating point (fields.rs:43-63); exported so that the deposit
+ * can be checked bit for bit. */
+int emb_sim_read_charge_fixed(EmbSim *sim, int64_t *rho);
+/* FP64 device reductions over the current state: max |div E - 4 pi k rho|,
+ * total deposited electron charge (exact, in electron charges), field energy
+ * sum(Ex^2) + sum(Ey^2) + c^2 sum(Bz^2). No counterpart in the reference. */
+int emb_sim_diagnostics(EmbSim *sim, double *gauss_residual, double *electron_charge, double *field_energy);
+int emb_sim_destroy(EmbSim *sim);
+
+/* --- Continuous-batching serving loop (no counterpart in the reference) ----------
+ * Scene requests of different sizes are submitted with a step count; the server
+ * admits them while their device footprint fits `hbm_budget_bytes`, keeps their
+ * state resident, and advances every admitted scene together, one batched
+ * kernel per phase of the tick, replayed as a captured HIP graph. `batched` = 0
+ * runs the same scenes one scene per launch instead (the plain comparison). */
+typedef struct EmbServerConfig {
+    int64_t hbm_budget_bytes;
+    int32_t max_slots;          /* scenes resident at once */
+    int32_t batched;            /* 1: batched kernels + graphs, 0: one scene per launch */
+    int32_t ticks_per_round;    /* ticks advanced between admission rounds */
+    int32_t use_graph;
+} EmbServerConfig;
+
+typedef struct EmbServerStats {
+    int64_t completed;
+    int64_t cell_steps;         /* nx * ny * ticks summed over completed work */
+    double elapsed_s;
+    double cell_steps_per_s;
+    double step_latency_p50_ms;
+    double step_latency_p99_ms;
+    int64_t step_samples;
+    int32_t peak_resident;
+    int32_t reserved0;
+} EmbServerStats;
+
+typedef struct EmbServer EmbServer;
+
+int emb_server_create(EmbDevice *dev, const EmbSettings *settings, const EmbServerConfig *config, EmbServer **out);
+/* Queue a scene for `steps` ticks; returns its id in *id. Requests whose
+ * footprint alone exceeds the budget are refused with EMB_E_NOMEM. */
+int emb_server_submit(EmbServer *srv, const EmbScene *scene, int64_t steps, int64_t *id);
+/* Cancel a queued or running request. */
+int emb_server_cancel(EmbServer *srv, int64_t id);
+/* Run one admission round: admit what fits, advance every resident scene by up
+ * to ticks_per_round ticks, retire finished ones. *active gets the number of
+ * scenes resident or queued afterwards. */
+int emb_server_poll(EmbServer *srv, int32_t *active);
+/* 1 if request `id` finished (its result can be read), 0 if not yet, <0 error. */
+int emb_server_done(EmbServer *srv, int64_t id);
+/* Result of a finished request: electrons and fields as emb_sim_read_state /
+ * emb_sim_read_yee would give them. Any pointer may be NULL. */
+int emb_server_result(EmbServer *srv, int64_t id, double *pos, double *vel, double *ex, double *ey, double *bz);
+int emb_server_stats(EmbServer *srv, EmbServerStats *out);
+int emb_server_destroy(EmbServer *srv);
+
+/* Footprint in bytes that a scene of this shape would occupy on the device. */
+int64_t emb_scene_footprint(const EmbSettings *settings, int32_t cols, int32_t rows, int64_t n_electrons);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif
