@@ -1,0 +1,191 @@
+"""Benchmark: Yee cell-steps/s of the per-tick physics on one MI355X.
+
+The timed workload is a 512 x 512 scene (544 x 544 Yee nodes with the absorbing
+margin): 64 random filled conductor rectangles (sides 4-64, resistance
+log-uniform 0.1-10), 16 battery cells (8 DC, 8 AC), 16 carriers per conductor
+cell (about 1M electrons), seeded. A step is one full physics tick (battery
+trim, electron push with P3M close-range forces, sort, deposit + smoothing,
+FDTD, DST Poisson divergence cleaning); a cell-step is one Yee node advanced by
+one tick. Inputs are resident in HBM before the timed region.
+
+Also reported on the same line: the roofline of the dominant kernel (HIP
+events per kernel phase on the simulation's stream), the FP64 oracle's rate on
+this host (a bounded sample, one thread), and a scaled serving stream of mixed
+64^2 - 1024^2 scenes run batched (persistent batched kernels + HIP graph) and
+one scene per launch.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-serve]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+HBM_PEAK_GBS = 8000.0
+FP32_MFMA_PEAK_TFLOPS = 157.3
+
+
+def algorithmic(phase: str, nx: int, ny: int, cols: int, rows: int, n: int):
+    """(amount, unit, bound) per launch of one kernel phase of one tick."""
+    nodes, cells = nx * ny, cols * rows
+    if phase == "dst_gemm":
+        # Two right-hand sides, four dense products each: 8 nx ny (nx + ny) flops.
+        return 8.0 * nx * ny * (nx + ny), "flop", "mfma"
+    if phase == "push":
+        return 32.0 * n, "byte", "hbm"      # x, y, vx, vy in and out (SoA FP32)
+    if phase == "fdtd":
+        return 32.0 * nodes, "byte", "hbm"  # Ex, Ey, Bz, Jx, Jy in; Ex, Ey, Bz out
+    if phase == "deposit_smooth":
+        return 16.0 * n + 12.0 * nodes + 16.0 * cells, "byte", "hbm"
+    if phase in ("count", "scatter"):
+        return 16.0 * n, "byte", "hbm"
+    if phase == "canon":
+        return 36.0 * n + 8.0 * cells, "byte", "hbm"
+    if phase == "scan":
+        return 12.0 * cells, "byte", "hbm"
+    if phase == "correct":
+        return 16.0 * nodes, "byte", "hbm"
+    return 0.0, "byte", "hbm"
+
+
+def run_single(args):
+    from embloks_amd import world as W
+    from embloks_amd.gpusim import GpuSim
+
+    world = W.rectangles_circuit(512, carriers_per_cell=16, seed=2)
+    sim = GpuSim(world)
+    sim.run_ticks(args.warmup)
+    t0 = time.perf_counter()
+    sim.step(args.steps)
+    sim.sync()
+    elapsed = time.perf_counter() - t0
+    phases = sim.kernel_times(max(10, min(args.steps, 100)))
+    residual, charge, energy = sim.diagnostics()
+    info = dict(nx=sim.nx, ny=sim.ny, cols=sim.cols, rows=sim.rows, n=sim.n, substeps=sim.dims().last_substeps,
+                gauss_residual=residual, electron_charge=charge)
+    return world, elapsed, phases, info
+
+
+def roofline(phases: dict, info: dict):
+    dom = max(phases, key=phases.get)
+    amount, unit, bound = algorithmic(dom, info["nx"], info["ny"], info["cols"], info["rows"], info["n"])
+    sec = phases[dom] / 1e3
+    if unit == "flop":
+        achieved = amount / sec / 1e12
+        peak, u = FP32_MFMA_PEAK_TFLOPS, "TFLOP/s"
+    else:
+        achieved = amount / sec / 1e9
+        peak, u = HBM_PEAK_GBS, "GB/s"
+    return dom, {"bound": bound, "achieved": round(achieved, 2), "peak": peak, "unit": u,
+                 "frac": round(achieved / peak, 4), "traffic": None, "kernel": dom,
+                 "avg_ms": round(phases[dom], 4)}
+
+
+def cpu_baseline(world, budget_s: float = 20.0):
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from oracle import orc
+
+    orc.set_threads(1)
+    sim = orc.Sim(world)
+    sim.step(1)
+    ticks, t0 = 0, time.perf_counter()
+    while ticks < 2 or (time.perf_counter() - t0 < budget_s * 0.5 and ticks < 10):
+        sim.step(1)
+        ticks += 1
+    sec = (time.perf_counter() - t0) / ticks
+    m = world.settings["absorbing_margin"]
+    nodes = (world.cols + 2 * m) * (world.rows + 2 * m)
+    return {"value": round(nodes / sec, 1), "unit": "cell-steps/s", "cores": 1, "kind": "port",
+            "sample": f"FP64 oracle, same 512x512 scene, {ticks} ticks after 1 warm-up, {sec * 1e3:.0f} ms/tick"}
+
+
+def run_serving(n_requests: int, steps_scale: float, batched: bool):
+    from embloks_amd import abi
+    from embloks_amd import world as W
+    from embloks_amd.gpusim import Server
+
+    stream = W.serving_stream(n_requests=n_requests, seed=5,
+                              min_steps=max(1, int(1000 * steps_scale)), max_steps=max(2, int(20000 * steps_scale)))
+    srv = Server(abi.settings(carriers_per_cell=1), hbm_budget_bytes=64 << 30, max_slots=256, batched=batched,
+                 ticks_per_round=16, use_graph=batched)
+    t0 = time.perf_counter()
+    pending = list(stream)
+    active = 0
+    while pending or active:
+        now = time.perf_counter() - t0
+        while pending and pending[0][0] <= now:
+            _, steps, w = pending.pop(0)
+            srv.submit(w, steps)
+        active = srv.poll()
+        if not active and pending:
+            time.sleep(max(0.0, min(0.001, pending[0][0] - (time.perf_counter() - t0))))
+    st = srv.stats()
+    srv.destroy()
+    return st
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=48)
+    ap.add_argument("--serve-scale", type=float, default=0.02, help="fraction of each request's 1000-20000 steps")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU design is not built.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    world, elapsed, phases, info = run_single(args)
+    nodes = info["nx"] * info["ny"]
+    value = nodes * args.steps / elapsed
+    dom, roof = roofline(phases, info)
+    out = {
+        "metric": "Yee cell-steps/s + p99 step latency, batched vs plain, <=1e-5 rel err vs FP64",
+        "value": round(value, 1),
+        "unit": "cell-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded random-rectangle circuit)",
+        "config": {"workload": "512x512 scene, 64 conductor rectangles, 16 batteries, 16 carriers/cell",
+                   "yee_nodes": nodes, "visible_cells": info["cols"] * info["rows"], "electrons": info["n"],
+                   "parallelism": "single GPU"},
+        "phases_ms": {k: round(v, 4) for k, v in phases.items()},
+        "roofline": roof,
+        "gauss_residual_fp64": info["gauss_residual"],
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(world)
+    if not args.no_serve:
+        serve = {}
+        for mode in ("batched", "plain"):
+            st = run_serving(args.serve_requests, args.serve_scale, mode == "batched")
+            serve[mode] = {"cell_steps_per_s": round(st["cell_steps_per_s"], 1),
+                           "p50_ms": round(st["step_latency_p50_ms"], 4), "p99_ms": round(st["step_latency_p99_ms"], 4),
+                           "completed": st["completed"], "peak_resident": st["peak_resident"],
+                           "elapsed_s": round(st["elapsed_s"], 3)}
+        b, p = serve["batched"]["cell_steps_per_s"], serve["plain"]["cell_steps_per_s"]
+        serve["speedup"] = round(b / p, 2) if p > 0 else None
+        serve["stream"] = (f"{args.serve_requests} requests, Poisson arrivals mean 2 ms, sides log-uniform 64-1024, "
+                           f"steps uniform {max(1, int(1000 * args.serve_scale))}-{max(2, int(20000 * args.serve_scale))}")
+        out["serving"] = serve
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -86,6 +86,8 @@ class EmbServerConfig(C.Structure):
         ("batched", C.c_int32),
         ("ticks_per_round", C.c_int32),
         ("use_graph", C.c_int32),
+        ("keep_results", C.c_int32),
+        ("reserved0", C.c_int32),
     ]
 
 

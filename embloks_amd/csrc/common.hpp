@@ -1,0 +1,160 @@
+// common.hpp - device data model shared by the kernels and the host runtime.
+//
+// One `DevScene` describes one resident scene: its dimensions, physics
+// constants (FP32) and pointers into its device arena. Kernels receive an
+// array of them plus a per-kernel block prefix (blocks of scene s are
+// prefix[s] .. prefix[s+1]), so the same kernel advances one scene (the plain
+// path) or every resident scene in one launch (the batched path).
+//
+// Layouts follow the reference (crates/embloks-core/src/maxwell.rs:12-15,
+// fields.rs:21-22): x-major arrays, index = i * height + j.
+//   Ex (nx+1, ny)   Ey (nx, ny+1)   Bz (nx+1, ny+1)   node arrays (nx, ny)
+//   cell arrays (cols, rows), cell = c * rows + r.
+#pragma once
+
+#include <stdint.h>
+
+namespace emb {
+
+// Fixed-point scales of the deterministic deposit: electron charge in units of
+// 2^-40 of one electron (CIC weights carry 20 fractional bits per axis), q*v in
+// units of 2^-28 (charge velocity of one electron).
+constexpr int W_BITS = 20;
+constexpr double RHO_SCALE = 1099511627776.0;   // 2^40
+constexpr double J_SCALE = 268435456.0;         // 2^28
+constexpr float J_FROM_W = 1.0f / 4096.0f;      // 2^28 / 2^40
+
+constexpr int MAX_STACKS_PER_BLOCK = 64;
+constexpr int SCAN_TILE = 1024;   // cells per block of the cell-list scan
+constexpr int TICK_TABLE = 64;    // ticks of host-computed AC phases / EMF targets
+
+// Cell flags (CellInfo.flags).
+constexpr uint32_t F_CONDUCTIVE = 1u;
+constexpr uint32_t F_WIRE = 2u;
+constexpr uint32_t F_BATTERY = 4u;
+constexpr int F_ORI_SHIFT = 3;      // 2 bits of orientation
+constexpr int F_BAT_SHIFT = 8;      // battery-cell index + 1 in the upper bits
+
+struct CellInfo {
+    float damping;      // drag rate R * density * cell_charge * q/m (physics.rs:17-21)
+    uint32_t flags;
+};
+
+struct BatteryCell {
+    float emf;          // |EMF| of the cell, along its orientation (battery.rs:210-228)
+    float frequency;    // 0 = DC
+    int32_t stack;      // index of its stack, -1 if none
+    int32_t ac_slot;    // column in the per-tick phase table, -1 for DC
+};
+
+struct StackInfo {
+    int32_t low_off, low_n, high_off, high_n;   // terminal-block cell lists
+    int32_t cur_off, cur_n;                     // cells whose current is averaged
+    float dir_x, dir_y;
+    float amplitude, internal_resistance;
+    int32_t both_wired;
+    int32_t pad;
+};
+
+// Per-scene device counters and reductions.
+struct SceneCounters {
+    uint32_t tick;            // ticks completed (parity selects ping-pong buffers)
+    uint32_t vmax_bits[2];    // max |v| (float bits) of the electrons, by tick parity
+    uint32_t scan_ticket;     // last-block-done counter of the cell-list scan
+    int32_t substeps;         // electron substeps of the current tick
+    int32_t rest_error;       // set when the serial substep path rejected a scene
+    uint32_t pad[2];
+};
+
+struct DevScene {
+    // --- shape
+    int32_t cols, rows, nx, ny, margin;
+    int32_t n;                // electrons
+    int32_t n_stacks;
+    int32_t fdtd_substeps;
+    int32_t max_substeps;
+    int32_t reach;            // ceil(cutoff)
+    int32_t square_n;
+    int32_t close_on;
+    int32_t pad0[2];
+    // --- constants (FP32)
+    float dt, max_step, restitution;
+    float qm;                 // q / m
+    float inv_mass;
+    float q;                  // electron charge
+    float k_pair;             // k q q (close_range.rs:366)
+    float k_ion_q;            // k * cell_charge * q (close_range.rs:348)
+    float k_q;                // k * q (charges_potential, close_range.rs:463)
+    float cutoff2, g_scale, half, square_samples;
+    float c2h;                // h * c^2 of one FDTD substep (maxwell.rs:257)
+    float half_h;             // 0.5 * h (maxwell.rs:254)
+    float jscale;             // 0.5 * 4 pi k h (maxwell.rs:240-248)
+    float four_pi_k;
+    float trim_rate, gain_low, gain_high;
+    float rho_from_fixed;     // q_e / 2^40 / 4^(2 passes): int64 charge -> FP32
+    float j_from_fixed;       // q_e / 2^28 / 4^(2 passes)
+    float i_from_fixed;       // q_e / 2^28 (cell-binned current)
+    float pad_f;
+    int32_t passes;
+    int32_t pad1;
+    // --- fields (FP32)
+    float *ex, *ey, *bz;
+    float *rho, *jx, *jy;     // smoothed sources on nodes
+    float *w1, *w2, *phi;     // Poisson work: 2 channels each, channel stride nodes
+    const float *damp_ex_x, *damp_ex_y, *damp_ey_x, *damp_ey_y, *damp_bz_x, *damp_bz_y;
+    const float *sine_x, *sine_y;     // DST-I matrices (nx^2, ny^2)
+    const float *inv_lap;             // norm / (lambda_x + lambda_y), (nx, ny) (maxwell.rs:60-67)
+    // --- electrons (SoA, upload order)
+    float *x, *y, *vx, *vy;
+    // --- cell list
+    float *xs, *ys, *vxs, *vys;   // electrons in sorted order (snapshot of the substep start)
+    int32_t *order;           // electron index per sorted slot
+    int32_t *ecell, *erank;   // per electron: cell and rank within it
+    int32_t *count;           // per cell (zero between sorts)
+    int32_t *lstart;          // per cell: start within its scan tile
+    int32_t *toff;            // per scan tile: offset of the tile
+    int32_t *cstart;          // per cell + 1: absolute start in the sorted order
+    // --- deposit
+    long long *cur;           // [2 * cells]: cell-binned sum of v, units 2^-28 (x q_e)
+    const long long *static_rho;   // ions + fixed charges per cell, units 2^-40 electron charges
+    long long *dep_dump;      // optional [3 * cells]: unsmoothed electron deposit (diagnostics)
+    // --- materials, tables, batteries
+    const CellInfo *info;
+    const float *density;     // per cell (ion density)
+    const float *ion_pot;     // per cell: close-range potential of all ions
+    const float *g_table;     // PAIR_SAMPLES + 2
+    const float *square;      // square_n^2 * 3
+    const BatteryCell *bats;
+    const StackInfo *stacks;
+    const int32_t *stack_cells;
+    float *gains;             // per stack
+    int32_t *has_gain;        // per stack
+    const float *phase_tab;   // [TICK_TABLE][n_ac]: frac(f t) per AC cell and tick
+    const float *target_tab;  // [TICK_TABLE][n_stacks]: EMF target per tick
+    int32_t n_ac;
+    int32_t pad2;
+    SceneCounters *ctr;
+};
+
+// Kernel classes with their own block prefix over the batch.
+enum KernelClass {
+    KC_ELECTRONS = 0,   // 256 electrons per block
+    KC_SCAN,            // SCAN_TILE cells per block
+    KC_CELLS,           // 256 cells per block
+    KC_SMOOTH,          // 32 x 32 tiles of the smoothing region
+    KC_FDTD,            // 32 x 64 tiles of the (nx+1) x (ny+1) corner space
+    KC_GEMM,            // 2 channels x 64 x 64 output tiles of nx x ny
+    KC_NODES,           // 256 corners of (nx+1) x (ny+1) per block
+    KC_STACKS,          // one block per battery stack (at least one per scene)
+    KC_SCENE,           // one block per scene
+    KC_COUNT
+};
+
+struct Batch {
+    const DevScene *scenes;     // device array
+    const int32_t *prefix;      // device: KC_COUNT * (n_scenes + 1)
+    int32_t n_scenes;
+    int32_t total[KC_COUNT];    // blocks per class over the batch
+};
+
+}  // namespace emb

@@ -1,0 +1,880 @@
+// kernels.hip - the per-tick physics kernels for gfx950.
+//
+// Every kernel walks its kernel class's blocks over the batch (grid-stride), so
+// one launch serves one scene or many. Per tick (Simulation::step,
+// crates/embloks-core/src/simulation.rs:63-72, in the order of
+// GpuSim::encode_ticks, crates/embloks-gpu/src/sim.rs:607-661):
+//
+//   k_trim      battery gain feedback (battery.rs:260-281)
+//   k_push      electron substep 0: gather, impressed field, P3M close force,
+//               implicit drag, step clamp, confinement (physics.rs:124-162)
+//   k_rest      electron substeps 1.. when the tick needs more than one
+//   k_count, k_scan, k_scatter, k_canon
+//               counting sort of the electrons by cell (close_range.rs:258-286)
+//   k_deposit_smooth
+//               integer CIC deposit of rho and J plus the 1-2-1 smoothing
+//               (fields.rs:18-63, maxwell.rs:280-316), cell-binned current
+//               (physics.rs:34-43) and the speed maximum for the next tick
+//   k_fdtd      leapfrog + absorbing margin (maxwell.rs:235-270), then the
+//               right-hand sides of the Gauss solve
+//   dst GEMMs   (gemm.hip) and k_correct: E -= grad(phi) (maxwell.rs:176-204)
+
+#include "device.hpp"
+
+namespace emb {
+
+// ---------------------------------------------------------------------------------------
+// Counting sort by clamped cell (close_range.rs:258-286; particles.wgsl:23-85).
+// Ranks within a cell come from atomics; k_canon then orders each cell's slots by
+// electron index, so the sorted order (and every sum over it) is deterministic.
+
+__global__ __launch_bounds__(256) void k_count(Batch b) {
+    for (int blk = blockIdx.x; blk < b.total[KC_ELECTRONS]; blk += gridDim.x) {
+        const int s = scene_of(b, KC_ELECTRONS, blk);
+        const DevScene &S = b.scenes[s];
+        const int i = block_in_scene(b, KC_ELECTRONS, s, blk) * 256 + (int)threadIdx.x;
+        if (i < S.n) {
+            const int cell = cell_of(S, S.x[i], S.y[i]);
+            S.erank[i] = atomicAdd(&S.count[cell], 1);
+            S.ecell[i] = cell;
+        }
+    }
+}
+
+// Exclusive scan of 256 values, one per thread; returns the exclusive prefix.
+__device__ __forceinline__ int block_scan_256(int v, int *wsum, int &total) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int inc = v;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int t = __shfl_up(inc, o, 64);
+        if (lane >= o) inc += t;
+    }
+    if (lane == 63) wsum[w] = inc;
+    __syncthreads();
+    int off = 0;
+    total = 0;
+    for (int k = 0; k < 4; k++) {
+        const int ws = wsum[k];
+        if (k < w) off += ws;
+        total += ws;
+    }
+    __syncthreads();
+    return off + inc - v;
+}
+
+__global__ __launch_bounds__(256) void k_scan(Batch b) {
+    __shared__ int wsum[4];
+    __shared__ int last;
+    for (int blk = blockIdx.x; blk < b.total[KC_SCAN]; blk += gridDim.x) {
+        const int s = scene_of(b, KC_SCAN, blk);
+        const DevScene &S = b.scenes[s];
+        const int tile = block_in_scene(b, KC_SCAN, s, blk);
+        const int cells = S.cols * S.rows;
+        const int ntiles = (cells + SCAN_TILE - 1) / SCAN_TILE;
+        const int base = tile * SCAN_TILE + (int)threadIdx.x * 4;
+        int v[4], sum = 0;
+        for (int q = 0; q < 4; q++) {
+            v[q] = base + q < cells ? S.count[base + q] : 0;
+            sum += v[q];
+        }
+        int total;
+        int run = block_scan_256(sum, wsum, total);
+        for (int q = 0; q < 4; q++) {
+            if (base + q < cells) {
+                S.lstart[base + q] = run;
+                S.count[base + q] = 0;
+            }
+            run += v[q];
+        }
+        if (threadIdx.x == 0) S.toff[tile] = total;
+        __threadfence();
+        __syncthreads();
+        if (threadIdx.x == 0) last = atomicAdd(&S.ctr->scan_ticket, 1u) == (unsigned)(ntiles - 1);
+        __syncthreads();
+        if (last) {
+            // The last tile of this scene turns the tile totals into offsets.
+            __threadfence();
+            int carry = 0;
+            for (int c0 = 0; c0 < ntiles; c0 += 256) {
+                const int k = c0 + (int)threadIdx.x;
+                const int t = k < ntiles ? __hip_atomic_load(&S.toff[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
+                int chunk_total;
+                const int ex = block_scan_256(t, wsum, chunk_total);
+                if (k < ntiles) S.toff[k] = carry + ex;
+                carry += chunk_total;
+            }
+            if (threadIdx.x == 0) S.ctr->scan_ticket = 0;
+        }
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(256) void k_scatter(Batch b) {
+    for (int blk = blockIdx.x; blk < b.total[KC_ELECTRONS]; blk += gridDim.x) {
+        const int s = scene_of(b, KC_ELECTRONS, blk);
+        const DevScene &S = b.scenes[s];
+        const int i = block_in_scene(b, KC_ELECTRONS, s, blk) * 256 + (int)threadIdx.x;
+        if (i < S.n) S.order[scan_start(S, S.ecell[i]) + S.erank[i]] = i;
+    }
+}
+
+// Per cell: absolute start, electron-index order within the cell, and the sorted
+// snapshot of positions and velocities.
+__device__ __forceinline__ void canon_cell(const DevScene &S, int c, int s0, int s1) {
+    int *order = S.order;
+    for (int a = s0 + 1; a < s1; a++) {
+        const int key = order[a];
+        int p = a - 1;
+        while (p >= s0 && order[p] > key) {
+            order[p + 1] = order[p];
+            p--;
+        }
+        order[p + 1] = key;
+    }
+    for (int a = s0; a < s1; a++) {
+        const int e = order[a];
+        S.xs[a] = S.x[e];
+        S.ys[a] = S.y[e];
+        S.vxs[a] = S.vx[e];
+        S.vys[a] = S.vy[e];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_canon(Batch b) {
+    for (int blk = blockIdx.x; blk < b.total[KC_CELLS]; blk += gridDim.x) {
+        const int s = scene_of(b, KC_CELLS, blk);
+        const DevScene &S = b.scenes[s];
+        const int c = block_in_scene(b, KC_CELLS, s, blk) * 256 + (int)threadIdx.x;
+        const int cells = S.cols * S.rows;
+        if (c < cells) {
+            const int s0 = scan_start(S, c), s1 = scan_start(S, c + 1);
+            S.cstart[c] = s0;
+            if (c == cells - 1) S.cstart[cells] = S.n;
+            canon_cell(S, c, s0, s1);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Close-range tables (close_range.rs:244-254, common.wgsl:88-100).
+
+// Channel k of the square table at fractional indices, `a`/`b` from truncation
+// as in CloseRange::force (close_range.rs:337-347).
+__device__ __forceinline__ float square_trunc(const DevScene &S, int k, float u, float v) {
+    const int n = S.square_n;
+    const int a = clampi((int)u, 0, n - 2), b = clampi((int)v, 0, n - 2);
+    const float fu = u - (float)a, fv = v - (float)b;
+    const float *t = S.square + ((size_t)a * n + b) * 3 + k;
+    return t[0] * (1.0f - fu) * (1.0f - fv) + t[(size_t)n * 3] * fu * (1.0f - fv) + t[3] * (1.0f - fu) * fv +
+           t[(size_t)n * 3 + 3] * fu * fv;
+}
+
+// The same with floor, as table_bilinear (close_range.rs:245-254).
+__device__ __forceinline__ float square_floor(const DevScene &S, int k, float u, float v) {
+    const int n = S.square_n;
+    const int a = clampi((int)floorf(u), 0, n - 2), b = clampi((int)floorf(v), 0, n - 2);
+    const float fu = u - (float)a, fv = v - (float)b;
+    const float *t = S.square + ((size_t)a * n + b) * 3 + k;
+    return t[0] * (1.0f - fu) * (1.0f - fv) + t[(size_t)n * 3] * fu * (1.0f - fv) + t[3] * (1.0f - fu) * fv +
+           t[(size_t)n * 3 + 3] * fu * fv;
+}
+
+// ---------------------------------------------------------------------------------------
+// Battery trim (battery.rs:260-281; fields.wgsl:300-343). One 64-thread block per
+// stack. The potential a terminal reads is the node potential plus the
+// close-range potential of the ions and electrons near it (Simulation::potential,
+// simulation.rs:48-55; CloseRange::potential, close_range.rs:415-464).
+
+__global__ __launch_bounds__(64) void k_trim(Batch b) {
+    for (int blk = blockIdx.x; blk < b.total[KC_STACKS]; blk += gridDim.x) {
+        const int s = scene_of(b, KC_STACKS, blk);
+        const DevScene &S = b.scenes[s];
+        const int st = block_in_scene(b, KC_STACKS, s, blk);
+        const uint32_t t = S.ctr->tick;
+        if (st == 0 && threadIdx.x == 0) S.ctr->vmax_bits[(t + 1) & 1] = 0;
+        if (st >= S.n_stacks) continue;
+        const StackInfo I = S.stacks[st];
+        if (!I.both_wired || I.amplitude == 0.0f) {
+            if (threadIdx.x == 0) {
+                S.has_gain[st] = 0;
+                S.gains[st] = 1.0f;
+            }
+            continue;
+        }
+        const int R = S.close_on ? S.reach : -1;
+        const int side = 2 * R + 1;
+        const int per = 1 + (S.close_on ? side * side : 0);
+        const int lists = I.low_n + I.high_n;
+        float acc_lo = 0.0f, acc_hi = 0.0f;
+        for (int it = (int)threadIdx.x; it < lists * per; it += 64) {
+            const int li = it / per, sub = it - li * per;
+            const bool high = li >= I.low_n;
+            const int cell = S.stack_cells[high ? I.high_off + (li - I.low_n) : I.low_off + li];
+            const int c = cell / S.rows, r = cell - c * S.rows;
+            float v = 0.0f;
+            if (sub == 0) {
+                v = S.phi[(size_t)(c + S.margin) * S.ny + (r + S.margin)];
+                if (S.close_on) v += S.ion_pot[cell];
+            } else {
+                const int cc = c + (sub - 1) / side - R, rr = r + (sub - 1) % side - R;
+                if (cc >= 0 && rr >= 0 && cc < S.cols && rr < S.rows) {
+                    const int nb = cc * S.rows + rr;
+                    const float cx = (float)c + 0.5f, cy = (float)r + 0.5f;
+                    for (int k = S.cstart[nb]; k < S.cstart[nb + 1]; k++) {
+                        const float u = (S.xs[k] - cx + S.half) * S.square_samples;
+                        const float w = (S.ys[k] - cy + S.half) * S.square_samples;
+                        v += square_floor(S, 0, u, w);
+                    }
+                    v *= S.k_q;
+                }
+            }
+            if (high) acc_hi += v;
+            else acc_lo += v;
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+            acc_lo += __shfl_xor(acc_lo, o, 64);
+            acc_hi += __shfl_xor(acc_hi, o, 64);
+        }
+        if (threadIdx.x == 0) {
+            const float v_low = acc_lo / (float)I.low_n, v_high = acc_hi / (float)I.high_n;
+            const int cells = S.cols * S.rows;
+            float isum = 0.0f;
+            for (int k = 0; k < I.cur_n; k++) {
+                const int cell = S.stack_cells[I.cur_off + k];
+                const float ix = (float)S.cur[cell] * S.i_from_fixed;
+                const float iy = (float)S.cur[cells + cell] * S.i_from_fixed;
+                isum += ix * I.dir_x + iy * I.dir_y;
+            }
+            const float current = isum / (float)I.cur_n;
+            const float target = S.target_tab[(size_t)(t % TICK_TABLE) * S.n_stacks + st];
+            const float delivered = v_high - v_low + current * I.internal_resistance;
+            float gain = S.has_gain[st] ? S.gains[st] : 1.0f;
+            gain += S.trim_rate * (target - delivered) * target / (I.amplitude * I.amplitude);
+            S.gains[st] = fminf(fmaxf(gain, S.gain_low), S.gain_high);
+            S.has_gain[st] = 1;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Electron push (physics.rs:124-162; particles.wgsl:193-227).
+
+// Bilinear interpolation of a (w, h) array at (cell coordinate + offset + margin),
+// with the index clamped like Maxwell::bilinear (maxwell.rs:381-390). The integer
+// and fractional parts are split before adding the margin, so no bits are lost.
+__device__ __forceinline__ float gather(const float *a, int w, int h, float gx, float gy, int m) {
+    const float fgx = floorf(gx), fgy = floorf(gy);
+    const int i = clampi((int)fgx + m, 0, w - 2), j = clampi((int)fgy + m, 0, h - 2);
+    const float fx = gx - (float)(i - m), fy = gy - (float)(j - m);
+    const float *p = a + (size_t)i * h + j;
+    return p[0] * (1.0f - fx) * (1.0f - fy) + p[h] * fx * (1.0f - fy) + p[1] * (1.0f - fx) * fy + p[h + 1] * fx * fy;
+}
+
+__device__ __forceinline__ bool conductive(const DevScene &S, int c, int r) {
+    return (S.info[c * S.rows + r].flags & F_CONDUCTIVE) != 0u;
+}
+
+// Lowest and highest FP32 positions inside cell `start` at least 1e-6 from its
+// faces (physics.rs:10, 113), or one ulp where FP32 cannot resolve 1e-6.
+__device__ __forceinline__ float wall_low(int start) {
+    const float s = (float)start, v = s + 1e-6f;
+    return v > s ? v : nextafterf(s, INFINITY);
+}
+__device__ __forceinline__ float wall_high(int start) {
+    const float s = (float)(start + 1), v = s - 1e-6f;
+    return v < s ? v : nextafterf(s, -INFINITY);
+}
+
+// physics.rs:83-115 for one axis.
+__device__ __forceinline__ void resolve(const DevScene &S, int axis, int start, int other, float &p, float &v) {
+    const int end = (int)floorf(p);
+    if (end == start) return;
+    const bool forward = end > start;
+    const int face = forward ? start : start - 1;
+    const int n = axis == 0 ? S.cols : S.rows, n_other = axis == 0 ? S.rows : S.cols;
+    bool open = face >= 0 && face < n - 1 && other >= 0 && other < n_other;
+    if (open) open = axis == 0 ? (conductive(S, face, other) && conductive(S, face + 1, other))
+                               : (conductive(S, other, face) && conductive(S, other, face + 1));
+    if (open) return;
+    const float wall = (float)(forward ? start + 1 : start);
+    const float bounced = wall - S.restitution * (p - wall);
+    p = fminf(fmaxf(bounced, wall_low(start)), wall_high(start));
+    v *= -S.restitution;
+}
+
+__device__ __forceinline__ int substeps_of(const DevScene &S, uint32_t t) {
+    const float vmax = __uint_as_float(S.ctr->vmax_bits[t & 1]);
+    const float want = ceilf(vmax * S.dt / S.max_step);
+    int sub = want >= 1.0f ? (int)fminf(want, (float)S.max_substeps) : 1;
+    return sub < 1 ? 1 : sub;
+}
+
+__device__ void push_one(const DevScene &S, int slot, int k, float h, uint32_t t) {
+    const int id = S.order[slot];
+    const float px = S.xs[slot], py = S.ys[slot];
+    const float vx = S.vxs[slot], vy = S.vys[slot];
+    const int hc = clampi((int)floorf(px), 0, S.cols - 1), hr = clampi((int)floorf(py), 0, S.rows - 1);
+    const int home = hc * S.rows + hr;
+    const CellInfo info = S.info[home];
+    const int m = S.margin;
+    // Maxwell::gather (maxwell.rs:336-350): each component from its staggered grid.
+    float ex = gather(S.ex, S.nx + 1, S.ny, px, py - 0.5f, m);
+    float ey = gather(S.ey, S.nx, S.ny + 1, px - 0.5f, py, m);
+    const float bz = gather(S.bz, S.nx + 1, S.ny + 1, px, py, m);
+    // battery::impressed_field_grid (battery.rs:210-228).
+    if (info.flags & F_BATTERY) {
+        const BatteryCell bat = S.bats[(info.flags >> F_BAT_SHIFT) - 1];
+        float value = bat.emf;
+        if (bat.stack >= 0 && S.has_gain[bat.stack]) value *= S.gains[bat.stack];
+        if (bat.frequency > 0.0f) {
+            const float phase = S.phase_tab[(size_t)(t % TICK_TABLE) * S.n_ac + bat.ac_slot] + bat.frequency * ((float)k * h);
+            value *= sinf(6.283185307179586f * phase);
+        }
+        const int o = (info.flags >> F_ORI_SHIFT) & 3;
+        ex += o == 0 ? value : (o == 2 ? -value : 0.0f);
+        ey += o == 1 ? value : (o == 3 ? -value : 0.0f);
+    }
+    // physics.rs:49-51, 139-149
+    float ax = S.qm * (ex + vy * bz), ay = S.qm * (ey - vx * bz);
+    if (S.close_on) {
+        // CloseRange::force (close_range.rs:314-376), every pair seen from both sides.
+        float fx = 0.0f, fy = 0.0f;
+        const int R = S.reach;
+        const int c_lo = max(hc - R, 0), c_hi = min(hc + R, S.cols - 1);
+        const int r_lo = max(hr - R, 0), r_hi = min(hr + R, S.rows - 1);
+        for (int cc = c_lo; cc <= c_hi; cc++) {
+            for (int rr = r_lo; rr <= r_hi; rr++) {
+                const int cell = cc * S.rows + rr;
+                const float dens = S.density[cell];
+                if (dens > 0.0f) {
+                    const float u = (px - (float)cc - 0.5f + S.half) * S.square_samples;
+                    const float w = (py - (float)rr - 0.5f + S.half) * S.square_samples;
+                    const float coef = S.k_ion_q * dens;
+                    fx += coef * square_trunc(S, 1, u, w);
+                    fy += coef * square_trunc(S, 2, u, w);
+                }
+                const int e = S.cstart[cell + 1];
+                for (int j = S.cstart[cell]; j < e; j++) {
+                    if (j == slot) continue;
+                    const float dx = px - S.xs[j], dy = py - S.ys[j];
+                    const float d2 = dx * dx + dy * dy;
+                    if (d2 >= S.cutoff2) continue;
+                    const float tt = d2 * S.g_scale;
+                    const int si = (int)tt;
+                    const float g0 = S.g_table[si];
+                    const float f = S.k_pair * (g0 + (S.g_table[si + 1] - g0) * (tt - (float)si));
+                    fx += f * dx;
+                    fy += f * dy;
+                }
+            }
+        }
+        ax += fx * S.inv_mass;
+        ay += fy * S.inv_mass;
+    }
+    // kick (physics.rs:57-60), then at most max_step (physics.rs:151-155).
+    const float scale = 1.0f / (1.0f + info.damping * h);
+    float nvx = (vx + ax * h) * scale, nvy = (vy + ay * h) * scale;
+    float mx = nvx * h, my = nvy * h;
+    const float len = sqrtf(mx * mx + my * my);
+    const float limit = fminf(S.max_step / fmaxf(len, 1e-30f), 1.0f);
+    mx *= limit;
+    my *= limit;
+    // confine (physics.rs:66-79): x first, then y from the resulting column.
+    float tx = px + mx, ty = py + my;
+    const int ocx = (int)floorf(px), ocy = (int)floorf(py);
+    resolve(S, 0, ocx, ocy, tx, nvx);
+    resolve(S, 1, ocy, (int)floorf(tx), ty, nvy);
+    S.x[id] = tx;
+    S.y[id] = ty;
+    S.vx[id] = nvx;
+    S.vy[id] = nvy;
+}
+
+__global__ __launch_bounds__(256) void k_push(Batch b, int k) {
+    for (int blk = blockIdx.x; blk < b.total[KC_ELECTRONS]; blk += gridDim.x) {
+        const int s = scene_of(b, KC_ELECTRONS, blk);
+        const DevScene &S = b.scenes[s];
+        const uint32_t t = S.ctr->tick;
+        const int sub = substeps_of(S, t);
+        if (k >= sub) continue;
+        const float h = S.dt / (float)sub;
+        const int slot = block_in_scene(b, KC_ELECTRONS, s, blk) * 256 + (int)threadIdx.x;
+        if (slot == 0 && k == 0) S.ctr->substeps = sub;
+        if (slot < S.n) push_one(S, slot, k, h, t);
+    }
+}
+
+// Substeps 1.. of a tick that needs several (fast electrons): one 1024-thread
+// block per scene re-sorts and pushes serially. Rare (electrons drift far below
+// max_step / h), so it trades speed for a single launch with no grid barrier.
+__global__ __launch_bounds__(1024) void k_rest(Batch b) {
+    __shared__ int wpart[1024];
+    for (int blk = blockIdx.x; blk < b.total[KC_SCENE]; blk += gridDim.x) {
+        const int s = scene_of(b, KC_SCENE, blk);
+        const DevScene &S = b.scenes[s];
+        const uint32_t t = S.ctr->tick;
+        const int sub = substeps_of(S, t);
+        if (sub <= 1) continue;
+        const float h = S.dt / (float)sub;
+        const int cells = S.cols * S.rows;
+        const int tid = threadIdx.x;
+        for (int k = 1; k < sub; k++) {
+            __threadfence();
+            __syncthreads();
+            for (int i = tid; i < S.n; i += 1024) {
+                const int cell = cell_of(S, S.x[i], S.y[i]);
+                S.erank[i] = atomicAdd(&S.count[cell], 1);
+                S.ecell[i] = cell;
+            }
+            __threadfence();
+            __syncthreads();
+            const int chunk = (cells + 1023) / 1024;
+            const int c0 = min(tid * chunk, cells), c1 = min(c0 + chunk, cells);
+            int sum = 0;
+            for (int c = c0; c < c1; c++) sum += S.count[c];
+            wpart[tid] = sum;
+            __syncthreads();
+            if (tid == 0) {
+                int acc = 0;
+                for (int q = 0; q < 1024; q++) {
+                    const int v = wpart[q];
+                    wpart[q] = acc;
+                    acc += v;
+                }
+            }
+            __syncthreads();
+            int run = wpart[tid];
+            for (int c = c0; c < c1; c++) {
+                S.cstart[c] = run;
+                run += S.count[c];
+                S.count[c] = 0;
+            }
+            if (tid == 0) S.cstart[cells] = S.n;
+            __threadfence();
+            __syncthreads();
+            for (int i = tid; i < S.n; i += 1024) S.order[S.cstart[S.ecell[i]] + S.erank[i]] = i;
+            __threadfence();
+            __syncthreads();
+            for (int c = tid; c < cells; c += 1024) canon_cell(S, c, S.cstart[c], S.cstart[c + 1]);
+            __threadfence();
+            __syncthreads();
+            for (int slot = tid; slot < S.n; slot += 1024) push_one(S, slot, k, h, t);
+        }
+        __threadfence();
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Deposit + smoothing (fields.rs:18-63, 91-112; maxwell.rs:280-316): per 32 x 32
+// tile of the smoothing region (visible cells extended by `passes`), electrons of
+// the tile's window scatter integer CIC weights into LDS int64 accumulators;
+// the binomial filter (passes x [1 2 1] per axis) then runs in LDS and the tile
+// writes rho, Jx, Jy on its nodes. Integer arithmetic makes the sum independent
+// of order: the result is bit-deterministic and charge is conserved exactly.
+
+constexpr int ST = 32;          // output tile
+constexpr int MAX_PASSES = 4;
+constexpr int SW = ST + 2 * MAX_PASSES;   // LDS row pitch: window of the tile plus a halo of `passes`
+
+__global__ __launch_bounds__(256) void k_deposit_smooth(Batch b, int bias) {
+    __shared__ long long dep[3][SW * SW];
+    __shared__ long long tmp[ST * SW];
+    __shared__ float wmax[4];
+    for (int blk = blockIdx.x; blk < b.total[KC_SMOOTH]; blk += gridDim.x) {
+        const int s = scene_of(b, KC_SMOOTH, blk);
+        const DevScene &S = b.scenes[s];
+        const int P = S.passes;
+        const int W = ST + 2 * P;
+        const int local = block_in_scene(b, KC_SMOOTH, s, blk);
+        const int region_c = S.cols + 2 * P, region_r = S.rows + 2 * P;
+        const int tiles_r = (region_r + ST - 1) / ST;
+        const int R0x = (local / tiles_r) * ST, R0y = (local % tiles_r) * ST;
+        // Window cell coordinates: [wc0, wc0 + W) x [wr0, wr0 + W).
+        const int wc0 = R0x - 2 * P, wr0 = R0y - 2 * P;
+        const int cells = S.cols * S.rows;
+        const uint32_t t = S.ctr->tick + bias;
+        for (int q = threadIdx.x; q < W * W; q += 256) {
+            const int i = q / W, j = q - (q / W) * W;
+            const int c = wc0 + i, r = wr0 + j;
+            const bool on = c >= 0 && r >= 0 && c < S.cols && r < S.rows;
+            dep[0][i * SW + j] = on ? S.static_rho[c * S.rows + r] : 0;
+            dep[1][i * SW + j] = 0;
+            dep[2][i * SW + j] = 0;
+        }
+        __syncthreads();
+        // Electrons whose CIC corners reach the window live in cells [w0 - 1, w0 + W].
+        const int sc0 = max(wc0 - 1, 0), sc1 = min(wc0 + W, S.cols - 1);
+        const int sr0 = max(wr0 - 1, 0), sr1 = min(wr0 + W, S.rows - 1);
+        for (int c = sc0; c <= sc1 && sr0 <= sr1; c++) {
+            const int e0 = S.cstart[c * S.rows + sr0], e1 = S.cstart[c * S.rows + sr1 + 1];
+            for (int k = e0 + (int)threadIdx.x; k < e1; k += 256) {
+                const float x = S.xs[k], y = S.ys[k];
+                const float u = x - 0.5f, v = y - 0.5f;
+                const float cu = floorf(u), cv = floorf(v);
+                const int fu = (int)rintf((u - cu) * 1048576.0f), fv = (int)rintf((v - cv) * 1048576.0f);
+                const int wx[2] = {(1 << W_BITS) - fu, fu}, wy[2] = {(1 << W_BITS) - fv, fv};
+                const int ca = clampi((int)cu, 0, S.cols - 1), cb = clampi((int)cu + 1, 0, S.cols - 1);
+                const int ra = clampi((int)cv, 0, S.rows - 1), rb = clampi((int)cv + 1, 0, S.rows - 1);
+                const float vx = S.vxs[k], vy = S.vys[k];
+                for (int a = 0; a < 2; a++) {
+                    const int li = (a ? cb : ca) - wc0;
+                    if (li < 0 || li >= W || wx[a] == 0) continue;
+                    for (int bb = 0; bb < 2; bb++) {
+                        const int lj = (bb ? rb : ra) - wr0;
+                        if (lj < 0 || lj >= W || wy[bb] == 0) continue;
+                        const long long w = (long long)wx[a] * (long long)wy[bb];
+                        const float wf = __fmul_rn((float)wx[a], (float)wy[bb]);
+                        const long long jx = __float2ll_rn(__fmul_rn(__fmul_rn(vx, wf), J_FROM_W));
+                        const long long jy = __float2ll_rn(__fmul_rn(__fmul_rn(vy, wf), J_FROM_W));
+                        const int at = li * SW + lj;
+                        atomicAdd((unsigned long long *)&dep[0][at], as_ull(-w));
+                        atomicAdd((unsigned long long *)&dep[1][at], as_ull(jx));
+                        atomicAdd((unsigned long long *)&dep[2][at], as_ull(jy));
+                    }
+                }
+            }
+        }
+        // Cells owned by this tile: current (physics.rs:34-43) and speed maximum.
+        float vmax = 0.0f;
+        for (int q = threadIdx.x; q < ST * ST; q += 256) {
+            const int c = R0x - P + q / ST, r = R0y - P + (q % ST);
+            if (c < 0 || r < 0 || c >= S.cols || r >= S.rows) continue;
+            const int cell = c * S.rows + r;
+            long long ix = 0, iy = 0;
+            for (int k = S.cstart[cell]; k < S.cstart[cell + 1]; k++) {
+                const float vx = S.vxs[k], vy = S.vys[k];
+                ix += __float2ll_rn(__fmul_rn(vx, (float)J_SCALE));
+                iy += __float2ll_rn(__fmul_rn(vy, (float)J_SCALE));
+                vmax = fmaxf(vmax, sqrtf(vx * vx + vy * vy));
+            }
+            S.cur[cell] = ix;
+            S.cur[cells + cell] = iy;
+        }
+        vmax = wave_max(vmax);
+        if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = vmax;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const float m = fmaxf(fmaxf(wmax[0], wmax[1]), fmaxf(wmax[2], wmax[3]));
+            atomicMax(&S.ctr->vmax_bits[(t + 1) & 1], __float_as_uint(m));
+        }
+        if (S.dep_dump) {
+            for (int q = threadIdx.x; q < ST * ST; q += 256) {
+                const int c = R0x - P + q / ST, r = R0y - P + (q % ST);
+                if (c < 0 || r < 0 || c >= S.cols || r >= S.rows) continue;
+                const int at = (q / ST + P) * SW + (q % ST + P);
+                const int cell = c * S.rows + r;
+                S.dep_dump[cell] = dep[0][at] - S.static_rho[cell];
+                S.dep_dump[cells + cell] = dep[1][at];
+                S.dep_dump[2 * cells + cell] = dep[2][at];
+            }
+        }
+        // Binomial smoothing: passes of [1 2 1] = the order-2P binomial per axis.
+        long long bin[2 * MAX_PASSES + 1];
+        bin[0] = 1;
+        for (int q = 1; q <= 2 * P; q++) {
+            bin[q] = 0;
+            for (int z = q; z > 0; z--) bin[z] += bin[z - 1];
+        }
+#pragma unroll
+        for (int ch = 0; ch < 3; ch++) {
+            const float scale = ch == 0 ? S.rho_from_fixed : S.j_from_fixed;
+            float *out = ch == 0 ? S.rho : (ch == 1 ? S.jx : S.jy);
+            __syncthreads();
+            for (int q = threadIdx.x; q < ST * W; q += 256) {
+                const int i = q / W, j = q - (q / W) * W;
+                long long acc = 0;
+                for (int d = 0; d <= 2 * P; d++) acc += bin[d] * dep[ch][(i + d) * SW + j];
+                tmp[i * SW + j] = acc;
+            }
+            __syncthreads();
+            for (int q = threadIdx.x; q < ST * ST; q += 256) {
+                const int i = q / ST, j = q % ST;
+                const int a = R0x + i, rr = R0y + j;
+                if (a >= region_c || rr >= region_r) continue;
+                long long acc = 0;
+                for (int d = 0; d <= 2 * P; d++) acc += bin[d] * tmp[i * SW + j + d];
+                const size_t node = (size_t)(a - P + S.margin) * S.ny + (rr - P + S.margin);
+                out[node] = (float)acc * scale;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// FDTD leapfrog with the absorbing margin, fused (maxwell.rs:235-270;
+// fields.wgsl:106-186): per 32 x 64 tile of the corner space, Bz' = Bz - h/2 curl E
+// on the tile plus halo, E' from curl Bz' and the edge currents, Bz'' = Bz' - h/2
+// curl E', then the damping factors. With `rhs` set (the last FDTD substep), the
+// tile also writes the right-hand sides of the Gauss solve: -4 pi k rho and div E
+// (maxwell.rs:170-173, 200-204; fields.wgsl:192-203).
+
+constexpr int TI = 32, TJ = 64;
+
+__global__ __launch_bounds__(256) void k_fdtd(Batch b, int rhs) {
+    __shared__ float EX[(TI + 3) * (TJ + 3)];
+    __shared__ float EY[(TI + 3) * (TJ + 3)];
+    __shared__ float BZ[(TI + 2) * (TJ + 2)];
+    __shared__ float JX[(TI + 2) * (TJ + 2)];
+    __shared__ float JY[(TI + 2) * (TJ + 2)];
+    constexpr int WE = TJ + 3, WB = TJ + 2;
+    for (int blk = blockIdx.x; blk < b.total[KC_FDTD]; blk += gridDim.x) {
+        const int s = scene_of(b, KC_FDTD, blk);
+        const DevScene &S = b.scenes[s];
+        const int nx = S.nx, ny = S.ny;
+        const int local = block_in_scene(b, KC_FDTD, s, blk);
+        const int tiles_j = (ny + 1 + TJ - 1) / TJ;
+        const int i0 = (local / tiles_j) * TI, j0 = (local % tiles_j) * TJ;
+        // Loads: E at [i0-2, i0+TI] x [j0-2, j0+TJ], Bz at [i0-1, i0+TI] x [j0-1, j0+TJ],
+        // node J at [i0-2, i0+TI) x [j0-2, j0+TJ).
+        for (int q = threadIdx.x; q < (TI + 3) * WE; q += 256) {
+            const int a = q / WE, c = q - (q / WE) * WE;
+            const int i = i0 - 2 + a, j = j0 - 2 + c;
+            EX[q] = (i >= 0 && i <= nx && j >= 0 && j < ny) ? S.ex[(size_t)i * ny + j] : 0.0f;
+            EY[q] = (i >= 0 && i < nx && j >= 0 && j <= ny) ? S.ey[(size_t)i * (ny + 1) + j] : 0.0f;
+        }
+        for (int q = threadIdx.x; q < (TI + 2) * WB; q += 256) {
+            const int a = q / WB, c = q - (q / WB) * WB;
+            const int i = i0 - 1 + a, j = j0 - 1 + c;
+            BZ[q] = (i >= 0 && i <= nx && j >= 0 && j <= ny) ? S.bz[(size_t)i * (ny + 1) + j] : 0.0f;
+            const int in = i - 1, jn = j - 1;
+            const bool node = in >= 0 && in < nx && jn >= 0 && jn < ny;
+            JX[q] = node ? S.jx[(size_t)in * ny + jn] : 0.0f;
+            JY[q] = node ? S.jy[(size_t)in * ny + jn] : 0.0f;
+        }
+        __syncthreads();
+        // Bz' on [i0-1, i0+TI] x [j0-1, j0+TJ]: curl E (maxwell.rs:207-227).
+        for (int q = threadIdx.x; q < (TI + 2) * WB; q += 256) {
+            const int a = q / WB, c = q - (q / WB) * WB;
+            const int i = i0 - 1 + a, j = j0 - 1 + c;
+            if (i < 0 || i > nx || j < 0 || j > ny) continue;
+            const float curl = ((EY[(a + 1) * WE + c + 1] - EY[a * WE + c + 1]) - EX[(a + 1) * WE + c + 1]) + EX[(a + 1) * WE + c];
+            BZ[q] = BZ[q] + (-S.half_h) * curl;
+        }
+        __syncthreads();
+        // E' on [i0-1, i0+TI) x [j0-1, j0+TJ) (maxwell.rs:255-264).
+        for (int q = threadIdx.x; q < (TI + 1) * (TJ + 1); q += 256) {
+            const int a = q / (TJ + 1), c = q - (q / (TJ + 1)) * (TJ + 1);
+            const int i = i0 - 1 + a, j = j0 - 1 + c;
+            const int e = (a + 1) * WE + c + 1;
+            if (i >= 0 && i <= nx && j >= 0 && j < ny) {
+                const float jxe = (i == 0 || i == nx) ? 0.0f : S.jscale * (JX[a * WB + c + 1] + JX[(a + 1) * WB + c + 1]);
+                EX[e] += S.c2h * (BZ[a * WB + c + 1] - BZ[a * WB + c]) - jxe;
+            }
+            if (i >= 0 && i < nx && j >= 0 && j <= ny) {
+                const float jye = (j == 0 || j == ny) ? 0.0f : S.jscale * (JY[(a + 1) * WB + c] + JY[(a + 1) * WB + c + 1]);
+                EY[e] -= S.c2h * (BZ[(a + 1) * WB + c] - BZ[a * WB + c]) + jye;
+            }
+        }
+        __syncthreads();
+        // Bz'' and damping on the owned corners [i0, i0+TI) x [j0, j0+TJ).
+        for (int q = threadIdx.x; q < TI * TJ; q += 256) {
+            const int a = q / TJ + 1, c = q % TJ + 1;   // Bz index of corner (i0 + a - 1, j0 + c - 1)
+            const int i = i0 + a - 1, j = j0 + c - 1;
+            if (i > nx || j > ny) continue;
+            const float curl = ((EY[(a + 1) * WE + c + 1] - EY[a * WE + c + 1]) - EX[(a + 1) * WE + c + 1]) + EX[(a + 1) * WE + c];
+            const float v = BZ[a * WB + c] + (-S.half_h) * curl;
+            S.bz[(size_t)i * (ny + 1) + j] = v * fminf(S.damp_bz_x[i], S.damp_bz_y[j]);
+        }
+        __syncthreads();
+        // Damp E' in place over its whole range, then write the owned edges.
+        for (int q = threadIdx.x; q < (TI + 1) * (TJ + 1); q += 256) {
+            const int a = q / (TJ + 1), c = q - (q / (TJ + 1)) * (TJ + 1);
+            const int i = i0 - 1 + a, j = j0 - 1 + c;
+            const int e = (a + 1) * WE + c + 1;
+            if (i >= 0 && i <= nx && j >= 0 && j < ny) EX[e] *= fminf(S.damp_ex_x[i], S.damp_ex_y[j]);
+            if (i >= 0 && i < nx && j >= 0 && j <= ny) EY[e] *= fminf(S.damp_ey_x[i], S.damp_ey_y[j]);
+        }
+        __syncthreads();
+        for (int q = threadIdx.x; q < TI * TJ; q += 256) {
+            const int a = q / TJ, c = q % TJ;
+            const int i = i0 + a, j = j0 + c;
+            const int e = (a + 2) * WE + c + 2;
+            if (i <= nx && j < ny) S.ex[(size_t)i * ny + j] = EX[e];
+            if (i < nx && j <= ny) S.ey[(size_t)i * (ny + 1) + j] = EY[e];
+            if (rhs) {
+                // Node (i - 1, j - 1): its four edges lie in [i0-1, i0+TI) x [j0-1, j0+TJ).
+                const int in = i - 1, jn = j - 1;
+                if (in >= 0 && jn >= 0 && in < nx && jn < ny) {
+                    const int en = (a + 1) * WE + c + 1;
+                    const float div = (EX[en + WE] - EX[en]) + (EY[en + 1] - EY[en]);
+                    const size_t node = (size_t)in * ny + jn;
+                    const size_t nodes = (size_t)nx * ny;
+                    S.w1[node] = -S.four_pi_k * S.rho[node];
+                    S.w1[nodes + node] = div;
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Right-hand sides without an FDTD step (scene creation and upload).
+__global__ __launch_bounds__(256) void k_rhs(Batch b) {
+    for (int blk = blockIdx.x; blk < b.total[KC_NODES]; blk += gridDim.x) {
+        const int s = scene_of(b, KC_NODES, blk);
+        const DevScene &S = b.scenes[s];
+        const int q = block_in_scene(b, KC_NODES, s, blk) * 256 + (int)threadIdx.x;
+        const int nx = S.nx, ny = S.ny;
+        if (q >= nx * ny) continue;
+        const int i = q / ny, j = q - (q / ny) * ny;
+        const float div = (S.ex[(size_t)(i + 1) * ny + j] - S.ex[(size_t)i * ny + j]) +
+                          (S.ey[(size_t)i * (ny + 1) + j + 1] - S.ey[(size_t)i * (ny + 1) + j]);
+        S.w1[q] = -S.four_pi_k * S.rho[q];
+        S.w1[(size_t)nx * ny + q] = div;
+    }
+}
+
+// E -= grad(phi_potential + phi_correction), phi zero beyond the grid
+// (maxwell.rs:176-193, 200-204; fields.wgsl:206-237). With `tick` set, the
+// scene's tick counter advances here, after everything that reads it.
+__global__ __launch_bounds__(256) void k_correct(Batch b, int tick) {
+    for (int blk = blockIdx.x; blk < b.total[KC_NODES]; blk += gridDim.x) {
+        const int s = scene_of(b, KC_NODES, blk);
+        const DevScene &S = b.scenes[s];
+        const int local = block_in_scene(b, KC_NODES, s, blk);
+        const int q = local * 256 + (int)threadIdx.x;
+        const int nx = S.nx, ny = S.ny;
+        const size_t nodes = (size_t)nx * ny;
+        if (tick && local == 0 && threadIdx.x == 0) S.ctr->tick += 1;
+        if (q >= (nx + 1) * (ny + 1)) continue;
+        const int i = q / (ny + 1), j = q - (q / (ny + 1)) * (ny + 1);
+        auto p = [&](int a, int c) { return S.phi[nodes + (size_t)a * ny + c] + S.phi[(size_t)a * ny + c]; };
+        if (j < ny) {
+            const float high = i < nx ? p(i, j) : 0.0f, low = i > 0 ? p(i - 1, j) : 0.0f;
+            S.ex[(size_t)i * ny + j] -= high - low;
+        }
+        if (i < nx) {
+            const float high = j < ny ? p(i, j) : 0.0f, low = j > 0 ? p(i, j - 1) : 0.0f;
+            S.ey[(size_t)i * (ny + 1) + j] -= high - low;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// FP64 checks: the Gauss-law residual max |div E - 4 pi k rho| and the field
+// energy sum Ex^2 + Ey^2 + c^2 Bz^2, per block into `out` (host sums the blocks).
+__global__ __launch_bounds__(256) void k_diagnostics(const DevScene *scene, double c2, double *out, int blocks) {
+    const DevScene &S = *scene;
+    __shared__ double sres[256], sen[256];
+    const int nx = S.nx, ny = S.ny;
+    const long long corners = (long long)(nx + 1) * (ny + 1);
+    double res = 0.0, en = 0.0;
+    for (long long q = blockIdx.x * 256ll + threadIdx.x; q < corners; q += (long long)blocks * 256) {
+        const int i = (int)(q / (ny + 1)), j = (int)(q % (ny + 1));
+        if (i < nx && j < ny) {
+            const double div = ((double)S.ex[(size_t)(i + 1) * ny + j] - (double)S.ex[(size_t)i * ny + j]) +
+                               ((double)S.ey[(size_t)i * (ny + 1) + j + 1] - (double)S.ey[(size_t)i * (ny + 1) + j]);
+            const double want = (double)S.four_pi_k * (double)S.rho[(size_t)i * ny + j];
+            res = fmax(res, fabs(div - want));
+        }
+        const double bz = S.bz[q];
+        en += c2 * bz * bz;
+        if (j < ny) {
+            const double e = S.ex[(size_t)i * ny + j];
+            en += e * e;
+        }
+        if (i < nx) {
+            const double e = S.ey[(size_t)i * (ny + 1) + j];
+            en += e * e;
+        }
+    }
+    sres[threadIdx.x] = res;
+    sen[threadIdx.x] = en;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) {
+            sres[threadIdx.x] = fmax(sres[threadIdx.x], sres[threadIdx.x + o]);
+            sen[threadIdx.x] += sen[threadIdx.x + o];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[2 * blockIdx.x] = sres[0];
+        out[2 * blockIdx.x + 1] = sen[0];
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Kernel sequences (host side).
+
+hipError_t launch_dst_gemms(const Batch &b, hipStream_t stream);
+
+static inline int grid_of(int total) { return total < 8192 ? total : 8192; }
+
+#define EMB_LAUNCH(kernel, kc, threads, ...)                                         \
+    do {                                                                             \
+        if (b.total[kc] > 0) kernel<<<grid_of(b.total[kc]), threads, 0, stream>>>(__VA_ARGS__); \
+    } while (0)
+
+// Counting sort of the electrons by cell: counts, scan, scatter, canonical order.
+hipError_t sort_kernels(const Batch &b, hipStream_t stream) {
+    EMB_LAUNCH(k_count, KC_ELECTRONS, 256, b);
+    EMB_LAUNCH(k_scan, KC_SCAN, 256, b);
+    EMB_LAUNCH(k_scatter, KC_ELECTRONS, 256, b);
+    EMB_LAUNCH(k_canon, KC_CELLS, 256, b);
+    return hipGetLastError();
+}
+
+// Sources and Gauss's law for the present electrons, with no FDTD step
+// (Maxwell::for_world and GpuSim::upload: maxwell.rs:88-94, sim.rs:518-528).
+hipError_t setup_kernels(const Batch &b, hipStream_t stream) {
+    hipError_t e = sort_kernels(b, stream);
+    if (e != hipSuccess) return e;
+    EMB_LAUNCH(k_deposit_smooth, KC_SMOOTH, 256, b, -1);
+    EMB_LAUNCH(k_rhs, KC_NODES, 256, b);
+    e = launch_dst_gemms(b, stream);
+    if (e != hipSuccess) return e;
+    EMB_LAUNCH(k_correct, KC_NODES, 256, b, 0);
+    return hipGetLastError();
+}
+
+// One tick (Simulation::step, simulation.rs:63-72). With `marks`, an event is
+// recorded before each phase (TICK_PHASES of them) and one after the last.
+#define EMB_MARK(p) \
+    do {            \
+        if (marks) (void)hipEventRecord(marks[p], stream); \
+    } while (0)
+hipError_t tick_kernels(const Batch &b, int fdtd_substeps, hipStream_t stream, hipEvent_t *marks) {
+    EMB_MARK(0);
+    EMB_LAUNCH(k_trim, KC_STACKS, 64, b);
+    EMB_MARK(1);
+    EMB_LAUNCH(k_push, KC_ELECTRONS, 256, b, 0);
+    EMB_MARK(2);
+    if (b.total[KC_ELECTRONS] > 0) EMB_LAUNCH(k_rest, KC_SCENE, 1024, b);
+    EMB_MARK(3);
+    EMB_LAUNCH(k_count, KC_ELECTRONS, 256, b);
+    EMB_MARK(4);
+    EMB_LAUNCH(k_scan, KC_SCAN, 256, b);
+    EMB_MARK(5);
+    EMB_LAUNCH(k_scatter, KC_ELECTRONS, 256, b);
+    EMB_MARK(6);
+    EMB_LAUNCH(k_canon, KC_CELLS, 256, b);
+    EMB_MARK(7);
+    EMB_LAUNCH(k_deposit_smooth, KC_SMOOTH, 256, b, 0);
+    EMB_MARK(8);
+    for (int f = 0; f < fdtd_substeps; f++) EMB_LAUNCH(k_fdtd, KC_FDTD, 256, b, f == fdtd_substeps - 1 ? 1 : 0);
+    EMB_MARK(9);
+    hipError_t e = launch_dst_gemms(b, stream);
+    if (e != hipSuccess) return e;
+    EMB_MARK(10);
+    EMB_LAUNCH(k_correct, KC_NODES, 256, b, 1);
+    EMB_MARK(11);
+    return hipGetLastError();
+}
+
+extern const char *const TICK_PHASE_NAMES[] = {"trim", "push", "rest", "count", "scan", "scatter", "canon",
+                                        "deposit_smooth", "fdtd", "dst_gemm", "correct"};
+
+hipError_t deposit_kernel(const Batch &b, hipStream_t stream) {
+    EMB_LAUNCH(k_deposit_smooth, KC_SMOOTH, 256, b, -1);
+    return hipGetLastError();
+}
+
+hipError_t diagnostics_kernel(const DevScene *scene, double c2, double *out, int blocks, hipStream_t stream) {
+    k_diagnostics<<<blocks, 256, 0, stream>>>(scene, c2, out, blocks);
+    return hipGetLastError();
+}
+
+}  // namespace emb

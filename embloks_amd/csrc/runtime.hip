@@ -1,0 +1,709 @@
+// runtime.hip - device handle, resident scenes and the single-scene C ABI
+// (the GpuSim replacement, crates/embloks-gpu/src/sim.rs:180-850).
+
+#include "runtime.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace emb {
+
+static thread_local std::string g_error;
+
+void set_error(const std::string &msg) { g_error = msg; }
+int fail(int code, const std::string &msg) {
+    g_error = msg;
+    return code;
+}
+
+namespace {
+
+constexpr double PI = 3.14159265358979323846;
+
+struct Bump {
+    char *base;
+    size_t off = 0;
+    template <class T> T *take(size_t n) {
+        off = (off + 255) & ~(size_t)255;
+        T *p = base ? reinterpret_cast<T *>(base + off) : nullptr;
+        off += std::max<size_t>(n, 1) * sizeof(T);
+        return p;
+    }
+};
+
+struct Counts {
+    int cols, rows, nx, ny;
+    int64_t n;
+    int n_stacks, n_stack_cells, n_bats, n_ac;
+};
+
+// Every per-scene array in one arena (sizes follow the reference's regions,
+// sim.rs:230-297, 449-466).
+void layout(Bump &a, const Counts &c, DevScene &d, DevScene **d_scene, int32_t **d_prefix, SceneCounters **ctr,
+            float **tab) {
+    const size_t nx = c.nx, ny = c.ny, nodes = nx * ny, cells = (size_t)c.cols * c.rows, n = (size_t)c.n;
+    d.ex = a.take<float>((nx + 1) * ny);
+    d.ey = a.take<float>(nx * (ny + 1));
+    d.bz = a.take<float>((nx + 1) * (ny + 1));
+    d.rho = a.take<float>(nodes);
+    d.jx = a.take<float>(nodes);
+    d.jy = a.take<float>(nodes);
+    d.w1 = a.take<float>(2 * nodes);
+    d.w2 = a.take<float>(2 * nodes);
+    d.phi = a.take<float>(2 * nodes);
+    d.damp_ex_x = a.take<float>(nx + 1);
+    d.damp_ex_y = a.take<float>(ny);
+    d.damp_ey_x = a.take<float>(nx);
+    d.damp_ey_y = a.take<float>(ny + 1);
+    d.damp_bz_x = a.take<float>(nx + 1);
+    d.damp_bz_y = a.take<float>(ny + 1);
+    d.inv_lap = a.take<float>(nodes);
+    d.x = a.take<float>(n);
+    d.y = a.take<float>(n);
+    d.vx = a.take<float>(n);
+    d.vy = a.take<float>(n);
+    d.xs = a.take<float>(n);
+    d.ys = a.take<float>(n);
+    d.vxs = a.take<float>(n);
+    d.vys = a.take<float>(n);
+    d.order = a.take<int32_t>(n);
+    d.ecell = a.take<int32_t>(n);
+    d.erank = a.take<int32_t>(n);
+    d.count = a.take<int32_t>(cells);
+    d.lstart = a.take<int32_t>(cells);
+    d.toff = a.take<int32_t>((cells + SCAN_TILE - 1) / SCAN_TILE);
+    d.cstart = a.take<int32_t>(cells + 1);
+    d.cur = a.take<long long>(2 * cells);
+    d.static_rho = a.take<long long>(cells);
+    d.info = a.take<CellInfo>(cells);
+    d.density = a.take<float>(cells);
+    d.ion_pot = a.take<float>(cells);
+    d.bats = a.take<BatteryCell>(c.n_bats);
+    d.stacks = a.take<StackInfo>(c.n_stacks);
+    d.stack_cells = a.take<int32_t>(c.n_stack_cells);
+    d.gains = a.take<float>(c.n_stacks);
+    d.has_gain = a.take<int32_t>(c.n_stacks);
+    *tab = a.take<float>((size_t)TICK_TABLE * (c.n_ac + c.n_stacks));
+    d.phase_tab = *tab;
+    d.target_tab = *tab ? *tab + (size_t)TICK_TABLE * c.n_ac : nullptr;
+    *ctr = a.take<SceneCounters>(1);
+    d.ctr = *ctr;
+    *d_scene = a.take<DevScene>(1);
+    *d_prefix = a.take<int32_t>(2 * KC_COUNT);
+}
+
+template <class T> hipError_t put(T *dst, const std::vector<T> &src, hipStream_t st) {
+    if (src.empty()) return hipSuccess;
+    return hipMemcpyAsync(dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice, st);
+}
+template <class T> hipError_t put(const T *dst, const std::vector<T> &src, hipStream_t st) {
+    return put(const_cast<T *>(dst), src, st);
+}
+
+int shared_tables(EmbDevice *dev, const EmbSettings &s, int nx, int ny, const float **sx, const float **sy,
+                  DeviceTables &tab) {
+    std::lock_guard<std::mutex> guard(dev->lock);
+    for (int n : {nx, ny}) {
+        if (dev->sine.count(n)) continue;
+        std::vector<float> m = sine_matrix(n);
+        float *p = nullptr;
+        EMB_HIP(hipMalloc(&p, m.size() * sizeof(float)));
+        EMB_HIP(hipMemcpy(p, m.data(), m.size() * sizeof(float), hipMemcpyHostToDevice));
+        dev->sine[n] = p;
+    }
+    *sx = dev->sine[nx];
+    *sy = dev->sine[ny];
+    const auto key = std::make_tuple(s.electron_softening, (int)s.smoothing_passes, s.electron_cutoff);
+    auto it = dev->tables.find(key);
+    if (it == dev->tables.end()) {
+        DeviceTables t;
+        if (s.electron_cutoff > 0.0) {
+            auto h = host_tables(s.electron_softening, s.smoothing_passes, s.electron_cutoff);
+            std::vector<float> g(h->g.begin(), h->g.end()), sq(h->square.begin(), h->square.end());
+            EMB_HIP(hipMalloc(&t.g, g.size() * sizeof(float)));
+            EMB_HIP(hipMalloc(&t.square, sq.size() * sizeof(float)));
+            EMB_HIP(hipMemcpy(t.g, g.data(), g.size() * sizeof(float), hipMemcpyHostToDevice));
+            EMB_HIP(hipMemcpy(t.square, sq.data(), sq.size() * sizeof(float), hipMemcpyHostToDevice));
+            t.square_n = h->square_n;
+            t.half = h->half;
+            t.reach = h->reach;
+        }
+        it = dev->tables.emplace(key, t).first;
+    }
+    tab = it->second;
+    return EMB_OK;
+}
+
+}  // namespace
+
+size_t scene_bytes(const EmbSettings &s, int cols, int rows, int64_t n, int n_stacks, int n_stack_cells, int n_bats,
+                   int n_ac) {
+    Counts c{cols, rows, cols + 2 * s.absorbing_margin, rows + 2 * s.absorbing_margin, n, n_stacks, n_stack_cells, n_bats, n_ac};
+    Bump a{nullptr};
+    DevScene d{};
+    DevScene *ds;
+    int32_t *dp;
+    SceneCounters *ct;
+    float *tab;
+    layout(a, c, d, &ds, &dp, &ct, &tab);
+    return a.off;
+}
+
+void scene_blocks(const DevScene &d, int out[KC_COUNT]) {
+    const long long cells = (long long)d.cols * d.rows;
+    auto cdiv = [](long long a, long long b) { return (int)((a + b - 1) / b); };
+    out[KC_ELECTRONS] = cdiv(d.n, 256);
+    out[KC_SCAN] = cdiv(cells, SCAN_TILE);
+    out[KC_CELLS] = cdiv(cells, 256);
+    out[KC_SMOOTH] = cdiv(d.cols + 2 * d.passes, 32) * cdiv(d.rows + 2 * d.passes, 32);
+    out[KC_FDTD] = cdiv(d.nx + 1, 32) * cdiv(d.ny + 1, 64);
+    out[KC_GEMM] = 2 * cdiv(d.nx, 64) * cdiv(d.ny, 64);
+    out[KC_NODES] = cdiv((long long)(d.nx + 1) * (d.ny + 1), 256);
+    out[KC_STACKS] = std::max(1, d.n_stacks);
+    out[KC_SCENE] = 1;
+}
+
+int make_resident(EmbDevice *dev, const EmbSettings &s, HostScene &&hs_in, hipStream_t st, Resident &r) {
+    r.dev = dev;
+    r.s = s;
+    r.hs = std::move(hs_in);
+    HostScene &hs = r.hs;
+    const Counts c{hs.cols, hs.rows, hs.nx, hs.ny, hs.n, (int)hs.stacks.size(), (int)hs.stack_cells.size(), (int)hs.bats.size(),
+                   (int)hs.ac_freq.size()};
+    Bump dry{nullptr};
+    DevScene tmp{};
+    DevScene *dsp;
+    int32_t *dpp;
+    SceneCounters *ctp;
+    float *tabp;
+    layout(dry, c, tmp, &dsp, &dpp, &ctp, &tabp);
+    r.bytes = dry.off;
+    EMB_HIP(hipMalloc(&r.arena, r.bytes));
+    EMB_HIP(hipMemsetAsync(r.arena, 0, r.bytes, st));
+    Bump a{static_cast<char *>(r.arena)};
+    DevScene &d = r.ds;
+    std::memset(&d, 0, sizeof(d));
+    SceneCounters *ctr;
+    float *tab;
+    layout(a, c, d, &r.d_scene, &r.d_prefix, &ctr, &tab);
+    d.cols = hs.cols;
+    d.rows = hs.rows;
+    d.nx = hs.nx;
+    d.ny = hs.ny;
+    d.margin = hs.margin;
+    d.n = (int32_t)hs.n;
+    d.n_stacks = (int32_t)hs.stacks.size();
+    d.fdtd_substeps = hs.fdtd_substeps;
+    d.max_substeps = s.max_substeps;
+    d.passes = s.smoothing_passes;
+    d.n_ac = (int32_t)hs.ac_freq.size();
+    DeviceTables tables;
+    int rc = shared_tables(dev, s, hs.nx, hs.ny, &d.sine_x, &d.sine_y, tables);
+    if (rc != EMB_OK) return rc;
+    d.close_on = s.electron_cutoff > 0.0 ? 1 : 0;
+    d.reach = tables.reach;
+    d.square_n = tables.square_n;
+    d.g_table = tables.g;
+    d.square = tables.square;
+    const double q = hs.q, mass = hs.mass, k = s.coulomb_k;
+    d.dt = (float)s.dt;
+    d.max_step = (float)s.max_step;
+    d.restitution = (float)s.wall_restitution;
+    d.q = (float)q;
+    d.qm = (float)(q / mass);
+    d.inv_mass = (float)(1.0 / mass);
+    d.k_pair = (float)(k * q * q);
+    d.k_ion_q = (float)(k * s.cell_charge * q);
+    d.k_q = (float)(k * q);
+    d.cutoff2 = (float)(s.electron_cutoff * s.electron_cutoff);
+    d.g_scale = s.electron_cutoff > 0.0 ? (float)(4096.0 / (s.electron_cutoff * s.electron_cutoff)) : 0.0f;
+    d.half = (float)tables.half;
+    d.square_samples = 32.0f;
+    d.c2h = (float)(hs.h_fdtd * s.speed_of_light * s.speed_of_light);
+    d.half_h = (float)(0.5 * hs.h_fdtd);
+    d.jscale = (float)(0.5 * (4.0 * PI * k * hs.h_fdtd));
+    d.four_pi_k = (float)(4.0 * PI * k);
+    d.trim_rate = (float)(s.dt / s.battery_trim_time);
+    d.gain_low = (float)s.battery_gain_low;
+    d.gain_high = (float)s.battery_gain_high;
+    const double smooth = std::pow(4.0, 2 * s.smoothing_passes);
+    const double unit = s.cell_charge / (double)s.carriers_per_cell;
+    d.rho_from_fixed = (float)(unit / RHO_SCALE / smooth);
+    d.j_from_fixed = (float)(q / J_SCALE / smooth);
+    d.i_from_fixed = (float)(q / J_SCALE);
+    EMB_HIP(put(d.damp_ex_x, hs.damp[0], st));
+    EMB_HIP(put(d.damp_ex_y, hs.damp[1], st));
+    EMB_HIP(put(d.damp_ey_x, hs.damp[2], st));
+    EMB_HIP(put(d.damp_ey_y, hs.damp[3], st));
+    EMB_HIP(put(d.damp_bz_x, hs.damp[4], st));
+    EMB_HIP(put(d.damp_bz_y, hs.damp[5], st));
+    EMB_HIP(put(d.inv_lap, hs.inv_lap, st));
+    EMB_HIP(put(d.x, hs.x, st));
+    EMB_HIP(put(d.y, hs.y, st));
+    EMB_HIP(put(d.vx, hs.vx, st));
+    EMB_HIP(put(d.vy, hs.vy, st));
+    EMB_HIP(put(d.static_rho, hs.static_rho, st));
+    EMB_HIP(put(d.info, hs.info, st));
+    EMB_HIP(put(d.density, hs.density, st));
+    EMB_HIP(put(d.ion_pot, hs.ion_pot, st));
+    EMB_HIP(put(d.bats, hs.bats, st));
+    EMB_HIP(put(d.stacks, hs.stack_info, st));
+    EMB_HIP(put(d.stack_cells, hs.stack_cells, st));
+    EMB_HIP(put(d.gains, hs.gains, st));
+    EMB_HIP(put(d.has_gain, hs.has_gain, st));
+    EMB_HIP(hipMemcpyAsync(r.d_scene, &d, sizeof(DevScene), hipMemcpyHostToDevice, st));
+    int blocks[KC_COUNT];
+    scene_blocks(d, blocks);
+    std::vector<int32_t> prefix(2 * KC_COUNT);
+    for (int k2 = 0; k2 < KC_COUNT; k2++) {
+        prefix[2 * k2] = 0;
+        prefix[2 * k2 + 1] = blocks[k2];
+        r.batch.total[k2] = blocks[k2];
+    }
+    EMB_HIP(put(r.d_prefix, prefix, st));
+    r.batch.scenes = r.d_scene;
+    r.batch.prefix = r.d_prefix;
+    r.batch.n_scenes = 1;
+    r.tab_floats = (size_t)TICK_TABLE * (hs.ac_freq.size() + hs.stacks.size());
+    if (r.tab_floats > 0) {
+        EMB_HIP(hipHostMalloc((void **)&r.pin_tab, r.tab_floats * sizeof(float), hipHostMallocDefault));
+        std::fill(r.pin_tab, r.pin_tab + r.tab_floats, 0.0f);
+        EMB_HIP(hipEventCreateWithFlags(&r.tab_event, hipEventDisableTiming));
+        EMB_HIP(hipEventRecord(r.tab_event, st));
+    }
+    // The copies above read pageable host vectors: let them finish before the
+    // host copies of the electrons are dropped.
+    EMB_HIP(hipStreamSynchronize(st));
+    hs.x = hs.y = hs.vx = hs.vy = {};
+    return EMB_OK;
+}
+
+void free_resident(Resident &r) {
+    if (r.tab_event) (void)hipEventSynchronize(r.tab_event), (void)hipEventDestroy(r.tab_event);
+    if (r.pin_tab) (void)hipHostFree(r.pin_tab);
+    if (r.arena) (void)hipFree(r.arena);
+    if (r.dep_dump) (void)hipFree(r.dep_dump);
+    r.tab_event = nullptr;
+    r.pin_tab = nullptr;
+    r.arena = nullptr;
+    r.dep_dump = nullptr;
+}
+
+int enqueue_tick_tables(Resident &r, int count, hipStream_t st) {
+    const int n_ac = (int)r.hs.ac_freq.size(), n_st = (int)r.hs.stacks.size();
+    if (r.tab_floats > 0) EMB_HIP(hipEventSynchronize(r.tab_event));
+    for (int j = 0; j < count; j++) {
+        const uint64_t row = (r.tick + (uint64_t)j) % TICK_TABLE;
+        const double t = r.time;
+        // AC phase frac(f t) in FP64, so FP32 keeps sin(2 pi f t) exact at any t
+        // (the reference keeps t mod 1000 for the same reason, sim.rs:30-32, 614-622).
+        for (int k = 0; k < n_ac; k++) {
+            const double ph = r.hs.ac_freq[(size_t)k] * t;
+            r.pin_tab[row * n_ac + k] = (float)(ph - std::floor(ph));
+        }
+        for (int k = 0; k < n_st; k++)
+            r.pin_tab[(size_t)TICK_TABLE * n_ac + row * n_st + k] = (float)r.hs.stacks[(size_t)k].emf_at(t);
+        r.time += r.s.dt;   // Simulation::step: time += dt (simulation.rs:70)
+    }
+    r.tick += (uint64_t)count;
+    if (r.tab_floats > 0) {
+        EMB_HIP(hipMemcpyAsync(const_cast<float *>(r.ds.phase_tab), r.pin_tab, r.tab_floats * sizeof(float),
+                               hipMemcpyHostToDevice, st));
+        EMB_HIP(hipEventRecord(r.tab_event, st));
+    }
+    return EMB_OK;
+}
+
+}  // namespace emb
+
+// =====================================================================================
+// C ABI
+// =====================================================================================
+
+using namespace emb;
+
+struct EmbSim {
+    EmbDevice *dev = nullptr;
+    hipStream_t stream = nullptr;
+    Resident r;
+    double *diag = nullptr;
+};
+
+#define EMB_GUARD(...)                                                         \
+    try {                                                                      \
+        __VA_ARGS__                                                            \
+    } catch (const std::exception &e) {                                        \
+        return fail(EMB_E_INVALID, std::string("exception: ") + e.what());     \
+    }
+
+extern "C" {
+
+const char *emb_last_error(void) { return g_error.c_str(); }
+
+void emb_settings_default(EmbSettings *s) {
+    std::memset(s, 0, sizeof(*s));
+    s->dt = 1.0 / 30.0;
+    s->cell_charge = 4.0;
+    s->carriers_per_cell = 8;
+    s->battery_density = 4;
+    s->carrier_charge_to_mass = 1.0;
+    s->drag_per_resistance = 1.0;
+    s->battery_resistance = 0.1;
+    s->battery_trim_time = 0.25;
+    s->battery_gain_low = 0.5;
+    s->battery_gain_high = 4.0;
+    s->current_smoothing_time = 2.0;
+    s->wall_restitution = 0.3;
+    s->max_step = 0.4;
+    s->max_substeps = 16;
+    s->smoothing_passes = 2;
+    s->coulomb_k = 1.0;
+    s->speed_of_light = 8.0;
+    s->absorbing_margin = 16;
+    s->absorbing_strength = 3.0;
+    s->electron_softening = 0.25;
+    s->electron_cutoff = 3.0;
+    s->display_smoothing_time = 0.5;
+}
+
+int emb_device_open(int ordinal, EmbDevice **out) {
+    EMB_GUARD({
+        if (!out) return fail(EMB_E_INVALID, "null output");
+        int count = 0;
+        EMB_HIP(hipGetDeviceCount(&count));
+        if (ordinal < 0 || ordinal >= count) return fail(EMB_E_INVALID, "no such device");
+        EMB_HIP(hipSetDevice(ordinal));
+        hipDeviceProp_t p;
+        EMB_HIP(hipGetDeviceProperties(&p, ordinal));
+        if (std::strncmp(p.gcnArchName, "gfx950", 6) != 0)
+            return fail(EMB_E_DEVICE, std::string("built for gfx950, device is ") + p.gcnArchName);
+        auto *d = new EmbDevice();
+        d->ordinal = ordinal;
+        d->n_cu = p.multiProcessorCount;
+        *out = d;
+        return EMB_OK;
+    })
+}
+
+int emb_device_close(EmbDevice *dev) {
+    if (!dev) return EMB_OK;
+    (void)hipSetDevice(dev->ordinal);
+    for (auto &kv : dev->sine) (void)hipFree(kv.second);
+    for (auto &kv : dev->tables) {
+        if (kv.second.g) (void)hipFree(kv.second.g);
+        if (kv.second.square) (void)hipFree(kv.second.square);
+    }
+    delete dev;
+    return EMB_OK;
+}
+
+int64_t emb_scene_footprint(const EmbSettings *s, int32_t cols, int32_t rows, int64_t n_electrons) {
+    if (!s || cols <= 0 || rows <= 0 || n_electrons < 0) return -1;
+    // Stacks and battery cells are small; budget a generous fixed share for them.
+    return (int64_t)scene_bytes(*s, cols, rows, n_electrons, 64, 4096, 256, 256) + (1 << 20);
+}
+
+int emb_sim_create(EmbDevice *dev, const EmbSettings *settings, const EmbScene *scene, EmbSim **out) {
+    EMB_GUARD({
+        if (!dev || !settings || !scene || !out) return fail(EMB_E_INVALID, "null argument");
+        HostScene hs;
+        const std::string why = prepare_scene(*settings, *scene, hs);
+        if (!why.empty()) return fail(EMB_E_INVALID, why);
+        EMB_HIP(hipSetDevice(dev->ordinal));
+        auto *sim = new EmbSim();
+        sim->dev = dev;
+        hipError_t e = hipStreamCreateWithFlags(&sim->stream, hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            delete sim;
+            return fail(EMB_E_DEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+        }
+        int rc = make_resident(dev, *settings, std::move(hs), sim->stream, sim->r);
+        if (rc == EMB_OK) {
+            sim->r.time = scene->time;
+            e = setup_kernels(sim->r.batch, sim->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(sim->stream);
+            if (e != hipSuccess) rc = fail(EMB_E_DEVICE, std::string("setup: ") + hipGetErrorString(e));
+        }
+        if (rc != EMB_OK) {
+            const std::string keep = g_error;
+            free_resident(sim->r);
+            (void)hipStreamDestroy(sim->stream);
+            delete sim;
+            g_error = keep;
+            return rc;
+        }
+        *out = sim;
+        return EMB_OK;
+    })
+}
+
+int emb_sim_upload(EmbSim *sim, const EmbScene *scene, int reset_display) {
+    (void)reset_display;   // the display average is not kept on this backend
+    EMB_GUARD({
+        if (!sim || !scene) return fail(EMB_E_INVALID, "null argument");
+        if (scene->cols != sim->r.hs.cols || scene->rows != sim->r.hs.rows)
+            return fail(EMB_E_INVALID, "upload must keep the grid shape");
+        HostScene hs;
+        const std::string why = prepare_scene(sim->r.s, *scene, hs);
+        if (!why.empty()) return fail(EMB_E_INVALID, why);
+        EMB_HIP(hipSetDevice(sim->dev->ordinal));
+        Resident fresh;
+        int rc = make_resident(sim->dev, sim->r.s, std::move(hs), sim->stream, fresh);
+        if (rc != EMB_OK) {
+            free_resident(fresh);
+            return rc;
+        }
+        const DevScene &o = sim->r.ds, &d = fresh.ds;
+        const size_t nx = o.nx, ny = o.ny;
+        // The fields carry on (GpuSim::upload keeps them, sim.rs:391-395).
+        EMB_HIP(hipMemcpyAsync(d.ex, o.ex, (nx + 1) * ny * 4, hipMemcpyDeviceToDevice, sim->stream));
+        EMB_HIP(hipMemcpyAsync(d.ey, o.ey, nx * (ny + 1) * 4, hipMemcpyDeviceToDevice, sim->stream));
+        EMB_HIP(hipMemcpyAsync(d.bz, o.bz, (nx + 1) * (ny + 1) * 4, hipMemcpyDeviceToDevice, sim->stream));
+        EMB_HIP(hipMemcpyAsync(&d.ctr->tick, &o.ctr->tick, sizeof(uint32_t), hipMemcpyDeviceToDevice, sim->stream));
+        fresh.time = sim->r.time;
+        fresh.tick = sim->r.tick;
+        EMB_HIP(setup_kernels(fresh.batch, sim->stream));
+        EMB_HIP(hipStreamSynchronize(sim->stream));
+        free_resident(sim->r);
+        sim->r = std::move(fresh);
+        return EMB_OK;
+    })
+}
+
+int emb_sim_step(EmbSim *sim, int ticks) {
+    EMB_GUARD({
+        if (!sim || ticks < 0) return fail(EMB_E_INVALID, "bad argument");
+        EMB_HIP(hipSetDevice(sim->dev->ordinal));
+        while (ticks > 0) {
+            const int chunk = std::min(ticks, TICK_TABLE);
+            int rc = enqueue_tick_tables(sim->r, chunk, sim->stream);
+            if (rc != EMB_OK) return rc;
+            for (int t = 0; t < chunk; t++) EMB_HIP(tick_kernels(sim->r.batch, sim->r.ds.fdtd_substeps, sim->stream));
+            ticks -= chunk;
+        }
+        return EMB_OK;
+    })
+}
+
+int emb_sim_sync(EmbSim *sim) {
+    if (!sim) return fail(EMB_E_INVALID, "null sim");
+    EMB_HIP(hipStreamSynchronize(sim->stream));
+    return EMB_OK;
+}
+
+int emb_sim_dims(EmbSim *sim, EmbDims *out) {
+    if (!sim || !out) return fail(EMB_E_INVALID, "null argument");
+    const HostScene &h = sim->r.hs;
+    std::memset(out, 0, sizeof(*out));
+    out->cols = h.cols;
+    out->rows = h.rows;
+    out->margin = h.margin;
+    out->nx = h.nx;
+    out->ny = h.ny;
+    out->n_stacks = (int32_t)h.stacks.size();
+    out->n_electrons = h.n;
+    out->fdtd_substeps = h.fdtd_substeps;
+    out->time = sim->r.time;
+    int32_t sub = 0;
+    EMB_HIP(hipStreamSynchronize(sim->stream));
+    EMB_HIP(hipMemcpy(&sub, &sim->r.ds.ctr->substeps, sizeof(sub), hipMemcpyDeviceToHost));
+    out->last_substeps = sub;
+    return EMB_OK;
+}
+
+static int copy_back(EmbSim *sim, const float *src, size_t n, double *dst, size_t stride = 1, size_t offset = 0) {
+    std::vector<float> tmp(n);
+    if (n) EMB_HIP(hipMemcpy(tmp.data(), src, n * sizeof(float), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < n; i++) dst[i * stride + offset] = tmp[i];
+    (void)sim;
+    return EMB_OK;
+}
+
+int emb_sim_read_state(EmbSim *sim, double *pos, double *vel, double *gains) {
+    EMB_GUARD({
+        if (!sim) return fail(EMB_E_INVALID, "null sim");
+        EMB_HIP(hipStreamSynchronize(sim->stream));
+        const DevScene &d = sim->r.ds;
+        const size_t n = (size_t)d.n;
+        int rc = EMB_OK;
+        if (pos && rc == EMB_OK) rc = copy_back(sim, d.x, n, pos, 2, 0);
+        if (pos && rc == EMB_OK) rc = copy_back(sim, d.y, n, pos, 2, 1);
+        if (vel && rc == EMB_OK) rc = copy_back(sim, d.vx, n, vel, 2, 0);
+        if (vel && rc == EMB_OK) rc = copy_back(sim, d.vy, n, vel, 2, 1);
+        if (rc != EMB_OK) return rc;
+        if (gains && d.n_stacks > 0) {
+            std::vector<float> g((size_t)d.n_stacks);
+            std::vector<int32_t> has((size_t)d.n_stacks);
+            EMB_HIP(hipMemcpy(g.data(), d.gains, g.size() * 4, hipMemcpyDeviceToHost));
+            EMB_HIP(hipMemcpy(has.data(), d.has_gain, has.size() * 4, hipMemcpyDeviceToHost));
+            for (size_t k = 0; k < g.size(); k++) gains[k] = has[k] ? (double)g[k] : std::nan("");
+        }
+        return EMB_OK;
+    })
+}
+
+int emb_sim_read_yee(EmbSim *sim, double *ex, double *ey, double *bz) {
+    EMB_GUARD({
+        if (!sim) return fail(EMB_E_INVALID, "null sim");
+        EMB_HIP(hipStreamSynchronize(sim->stream));
+        const DevScene &d = sim->r.ds;
+        const size_t nx = d.nx, ny = d.ny;
+        int rc = EMB_OK;
+        if (ex) rc = copy_back(sim, d.ex, (nx + 1) * ny, ex);
+        if (ey && rc == EMB_OK) rc = copy_back(sim, d.ey, nx * (ny + 1), ey);
+        if (bz && rc == EMB_OK) rc = copy_back(sim, d.bz, (nx + 1) * (ny + 1), bz);
+        return rc;
+    })
+}
+
+int emb_sim_write_yee(EmbSim *sim, const double *ex, const double *ey, const double *bz) {
+    EMB_GUARD({
+        if (!sim) return fail(EMB_E_INVALID, "null sim");
+        EMB_HIP(hipStreamSynchronize(sim->stream));
+        const DevScene &d = sim->r.ds;
+        const size_t nx = d.nx, ny = d.ny;
+        auto up = [&](float *dst, const double *src, size_t n) -> int {
+            if (!src) return EMB_OK;
+            std::vector<float> tmp(src, src + n);
+            EMB_HIP(hipMemcpy(dst, tmp.data(), n * 4, hipMemcpyHostToDevice));
+            return EMB_OK;
+        };
+        int rc = up(d.ex, ex, (nx + 1) * ny);
+        if (rc == EMB_OK) rc = up(d.ey, ey, nx * (ny + 1));
+        if (rc == EMB_OK) rc = up(d.bz, bz, (nx + 1) * (ny + 1));
+        return rc;
+    })
+}
+
+int emb_sim_read_potential(EmbSim *sim, double *potential) {
+    EMB_GUARD({
+        if (!sim || !potential) return fail(EMB_E_INVALID, "null argument");
+        EMB_HIP(hipStreamSynchronize(sim->stream));
+        const DevScene &d = sim->r.ds;
+        return copy_back(sim, d.phi, (size_t)d.nx * d.ny, potential);
+    })
+}
+
+int emb_sim_read_current(EmbSim *sim, double *current) {
+    EMB_GUARD({
+        if (!sim || !current) return fail(EMB_E_INVALID, "null argument");
+        EMB_HIP(hipStreamSynchronize(sim->stream));
+        const DevScene &d = sim->r.ds;
+        const size_t cells = (size_t)d.cols * d.rows;
+        std::vector<long long> tmp(2 * cells);
+        EMB_HIP(hipMemcpy(tmp.data(), d.cur, tmp.size() * 8, hipMemcpyDeviceToHost));
+        const double scale = sim->r.hs.q / J_SCALE;
+        for (size_t c = 0; c < cells; c++) {
+            current[2 * c] = (double)tmp[c] * scale;
+            current[2 * c + 1] = (double)tmp[cells + c] * scale;
+        }
+        return EMB_OK;
+    })
+}
+
+int emb_sim_read_charge_fixed(EmbSim *sim, int64_t *rho) {
+    EMB_GUARD({
+        if (!sim || !rho) return fail(EMB_E_INVALID, "null argument");
+        Resident &r = sim->r;
+        const size_t cells = (size_t)r.ds.cols * r.ds.rows;
+        EMB_HIP(hipStreamSynchronize(sim->stream));
+        if (!r.dep_dump) EMB_HIP(hipMalloc(&r.dep_dump, 3 * cells * sizeof(long long)));
+        r.ds.dep_dump = r.dep_dump;
+        EMB_HIP(hipMemcpyAsync(r.d_scene, &r.ds, sizeof(DevScene), hipMemcpyHostToDevice, sim->stream));
+        // Re-running the deposit on the unchanged electrons is idempotent.
+        EMB_HIP(deposit_kernel(r.batch, sim->stream));
+        r.ds.dep_dump = nullptr;
+        EMB_HIP(hipMemcpyAsync(r.d_scene, &r.ds, sizeof(DevScene), hipMemcpyHostToDevice, sim->stream));
+        EMB_HIP(hipStreamSynchronize(sim->stream));
+        std::vector<long long> tmp(3 * cells);
+        EMB_HIP(hipMemcpy(tmp.data(), r.dep_dump, tmp.size() * 8, hipMemcpyDeviceToHost));
+        std::memcpy(rho, tmp.data(), cells * 8);
+        return EMB_OK;
+    })
+}
+
+int emb_sim_diagnostics(EmbSim *sim, double *gauss_residual, double *electron_charge, double *field_energy) {
+    EMB_GUARD({
+        if (!sim) return fail(EMB_E_INVALID, "null sim");
+        Resident &r = sim->r;
+        const int blocks = 1024;
+        if (!sim->diag) EMB_HIP(hipMalloc(&sim->diag, 2 * blocks * sizeof(double)));
+        EMB_HIP(diagnostics_kernel(r.d_scene, r.s.speed_of_light * r.s.speed_of_light, sim->diag, blocks, sim->stream));
+        std::vector<double> out(2 * blocks);
+        EMB_HIP(hipStreamSynchronize(sim->stream));
+        EMB_HIP(hipMemcpy(out.data(), sim->diag, out.size() * 8, hipMemcpyDeviceToHost));
+        double res = 0.0, en = 0.0;
+        for (int b = 0; b < blocks; b++) {
+            res = std::max(res, out[2 * b]);
+            en += out[2 * b + 1];
+        }
+        if (gauss_residual) *gauss_residual = res;
+        if (field_energy) *field_energy = en;
+        if (electron_charge) {
+            const size_t cells = (size_t)r.ds.cols * r.ds.rows;
+            std::vector<int64_t> rho(cells);
+            int rc = emb_sim_read_charge_fixed(sim, rho.data());
+            if (rc != EMB_OK) return rc;
+            long long total = 0;
+            for (auto v : rho) total += v;
+            // In electron charges: the sum of integer weights is exactly -N * 2^40.
+            *electron_charge = (double)total / RHO_SCALE * (r.s.cell_charge / r.s.carriers_per_cell);
+        }
+        return EMB_OK;
+    })
+}
+
+int emb_sim_kernel_times(EmbSim *sim, int ticks, double *avg_ms, int capacity) {
+    EMB_GUARD({
+        if (!sim || ticks < 1) return fail(EMB_E_INVALID, "bad argument");
+        if (!avg_ms) return TICK_PHASES;
+        EMB_HIP(hipSetDevice(sim->dev->ordinal));
+        std::vector<hipEvent_t> ev((size_t)ticks * (TICK_PHASES + 1));
+        for (auto &e : ev) EMB_HIP(hipEventCreate(&e));
+        int rc = EMB_OK;
+        for (int done = 0; done < ticks && rc == EMB_OK;) {
+            const int chunk = std::min(ticks - done, TICK_TABLE);
+            rc = enqueue_tick_tables(sim->r, chunk, sim->stream);
+            for (int t = 0; t < chunk && rc == EMB_OK; t++) {
+                hipError_t e = tick_kernels(sim->r.batch, sim->r.ds.fdtd_substeps, sim->stream,
+                                            &ev[(size_t)(done + t) * (TICK_PHASES + 1)]);
+                if (e != hipSuccess) rc = fail(EMB_E_DEVICE, hipGetErrorString(e));
+            }
+            done += chunk;
+        }
+        if (rc == EMB_OK) {
+            hipError_t e = hipStreamSynchronize(sim->stream);
+            if (e != hipSuccess) rc = fail(EMB_E_DEVICE, hipGetErrorString(e));
+        }
+        std::vector<double> sum(TICK_PHASES, 0.0);
+        for (int t = 0; t < ticks && rc == EMB_OK; t++)
+            for (int p = 0; p < TICK_PHASES; p++) {
+                float ms = 0.0f;
+                hipError_t e = hipEventElapsedTime(&ms, ev[(size_t)t * (TICK_PHASES + 1) + p], ev[(size_t)t * (TICK_PHASES + 1) + p + 1]);
+                if (e != hipSuccess) rc = fail(EMB_E_DEVICE, hipGetErrorString(e));
+                sum[(size_t)p] += ms;
+            }
+        for (auto &e : ev) (void)hipEventDestroy(e);
+        if (rc != EMB_OK) return rc;
+        for (int p = 0; p < std::min(capacity, TICK_PHASES); p++) avg_ms[p] = sum[(size_t)p] / ticks;
+        return TICK_PHASES;
+    })
+}
+
+const char *emb_tick_phase_name(int phase) {
+    return (phase >= 0 && phase < TICK_PHASES) ? TICK_PHASE_NAMES[phase] : nullptr;
+}
+
+int emb_sim_destroy(EmbSim *sim) {
+    if (!sim) return EMB_OK;
+    (void)hipSetDevice(sim->dev->ordinal);
+    (void)hipStreamSynchronize(sim->stream);
+    free_resident(sim->r);
+    if (sim->diag) (void)hipFree(sim->diag);
+    (void)hipStreamDestroy(sim->stream);
+    delete sim;
+    return EMB_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,91 @@
+// runtime.hpp - host runtime: device handle, resident scenes, kernel launches.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/embloks_hip.h"
+#include "common.hpp"
+#include "setup.hpp"
+
+namespace emb {
+
+void set_error(const std::string &msg);
+int fail(int code, const std::string &msg);
+
+#define EMB_HIP(call)                                                                          \
+    do {                                                                                       \
+        hipError_t e_ = (call);                                                                \
+        if (e_ != hipSuccess) return ::emb::fail(EMB_E_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+struct DeviceTables {
+    float *g = nullptr;
+    float *square = nullptr;
+    int square_n = 0;
+    double half = 0.0;
+    int reach = 0;
+};
+
+}  // namespace emb
+
+struct EmbDevice {
+    int ordinal = 0;
+    int n_cu = 0;
+    std::mutex lock;
+    std::map<int, float *> sine;   // DST-I matrices per length, shared by every scene
+    std::map<std::tuple<double, int, double>, emb::DeviceTables> tables;
+};
+
+namespace emb {
+
+// One scene resident on the device: its arena, descriptor and host-side state.
+struct Resident {
+    EmbDevice *dev = nullptr;
+    EmbSettings s{};
+    HostScene hs;                 // FP64 preparation (electron arrays dropped after upload)
+    void *arena = nullptr;
+    size_t bytes = 0;
+    DevScene ds{};                // host copy of the descriptor
+    DevScene *d_scene = nullptr;  // the descriptor in device memory
+    int32_t *d_prefix = nullptr;  // single-scene block prefix
+    Batch batch{};                // single-scene batch
+    float *pin_tab = nullptr;     // pinned staging of the per-tick tables
+    hipEvent_t tab_event = nullptr;
+    size_t tab_floats = 0;
+    double time = 0.0;
+    uint64_t tick = 0;            // ticks enqueued
+    int last_substeps = 0;
+    long long *dep_dump = nullptr;
+};
+
+// Bytes a scene of this shape needs (arena only; shared tables excluded).
+size_t scene_bytes(const EmbSettings &s, int cols, int rows, int64_t n, int n_stacks, int n_stack_cells, int n_bats, int n_ac);
+
+// Allocate the arena, fill the descriptor and upload the prepared scene on `stream`.
+int make_resident(EmbDevice *dev, const EmbSettings &s, HostScene &&hs, hipStream_t stream, Resident &out);
+void free_resident(Resident &r);
+
+// Blocks per kernel class for one scene.
+void scene_blocks(const DevScene &d, int out[KC_COUNT]);
+
+// Kernel sequences over a batch (kernels.hip).
+hipError_t sort_kernels(const Batch &b, hipStream_t stream);
+hipError_t setup_kernels(const Batch &b, hipStream_t stream);   // sort, deposit, Gauss (no FDTD)
+constexpr int TICK_PHASES = 11;
+extern const char *const TICK_PHASE_NAMES[];
+hipError_t tick_kernels(const Batch &b, int fdtd_substeps, hipStream_t stream, hipEvent_t *marks = nullptr);
+hipError_t deposit_kernel(const Batch &b, hipStream_t stream);
+hipError_t diagnostics_kernel(const DevScene *scene, double c2, double *out, int blocks, hipStream_t stream);
+
+// Write the per-tick AC phases and EMF targets of ticks [r.tick, r.tick + count)
+// into the pinned staging and enqueue the copy. Advances r.time and r.tick.
+int enqueue_tick_tables(Resident &r, int count, hipStream_t stream);
+
+}  // namespace emb

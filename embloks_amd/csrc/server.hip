@@ -1,0 +1,380 @@
+// server.hip - continuous-batching serving loop (no counterpart in the reference).
+//
+// Requests (a scene plus a step count) queue up; each poll admits queued
+// requests while their arena fits the HBM budget and a slot is free, advances
+// every resident scene by the same number of ticks, and retires the finished
+// ones. In batched mode one launch per kernel covers every resident scene (the
+// kernels walk a device table of scene descriptors), and one tick of the whole
+// batch is captured once as a HIP graph and replayed until the batch changes.
+// In plain mode each scene's tick is launched on its own, one scene per launch:
+// the same kernels, the comparison the batched path is measured against.
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+
+#include "runtime.hpp"
+
+using namespace emb;
+
+namespace {
+
+enum ReqState { QUEUED, RUNNING, DONE, CANCELLED };
+
+struct Request {
+    int64_t id = 0;
+    HostScene hs;
+    int64_t steps = 0, done = 0;
+    size_t footprint = 0;
+    double time0 = 0.0;
+    ReqState state = QUEUED;
+    Resident r;
+    std::vector<float> pos, vel, ex, ey, bz;   // kept results
+    bool has_result = false;
+};
+
+double now_s() {
+    using namespace std::chrono;
+    return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace
+
+struct EmbServer {
+    EmbDevice *dev = nullptr;
+    EmbSettings s{};
+    EmbServerConfig cfg{};
+    hipStream_t stream = nullptr;
+    std::map<int64_t, std::unique_ptr<Request>> all;
+    std::deque<Request *> queue;
+    std::vector<Request *> active;
+    int64_t next_id = 1;
+    size_t used_bytes = 0;
+    // batch
+    DevScene *d_scenes = nullptr;
+    int32_t *d_prefix = nullptr;
+    Batch batch{};
+    int fdtd_max = 1;
+    bool dirty = true;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // stats
+    std::vector<float> latencies_ms;   // one sample per scene-step
+    int64_t cell_steps = 0, completed = 0;
+    double t_first = -1.0, t_last = 0.0;
+    int peak = 0;
+};
+
+namespace {
+
+int drop_graph(EmbServer *sv) {
+    if (sv->graph_exec) (void)hipGraphExecDestroy(sv->graph_exec);
+    if (sv->graph) (void)hipGraphDestroy(sv->graph);
+    sv->graph_exec = nullptr;
+    sv->graph = nullptr;
+    return EMB_OK;
+}
+
+// Scene table and block prefixes of the resident scenes.
+int rebuild_batch(EmbServer *sv) {
+    const int n = (int)sv->active.size();
+    std::vector<DevScene> scenes;
+    std::vector<int32_t> prefix((size_t)KC_COUNT * (n + 1), 0);
+    sv->fdtd_max = 1;
+    for (int k = 0; k < KC_COUNT; k++) sv->batch.total[k] = 0;
+    for (int i = 0; i < n; i++) {
+        const DevScene &d = sv->active[(size_t)i]->r.ds;
+        scenes.push_back(d);
+        int blocks[KC_COUNT];
+        scene_blocks(d, blocks);
+        for (int k = 0; k < KC_COUNT; k++) prefix[(size_t)k * (n + 1) + i + 1] = prefix[(size_t)k * (n + 1) + i] + blocks[k];
+        sv->fdtd_max = std::max(sv->fdtd_max, d.fdtd_substeps);
+    }
+    for (int k = 0; k < KC_COUNT; k++) sv->batch.total[k] = prefix[(size_t)k * (n + 1) + n];
+    if (n > 0) {
+        EMB_HIP(hipMemcpyAsync(sv->d_scenes, scenes.data(), scenes.size() * sizeof(DevScene), hipMemcpyHostToDevice, sv->stream));
+        EMB_HIP(hipMemcpyAsync(sv->d_prefix, prefix.data(), prefix.size() * 4, hipMemcpyHostToDevice, sv->stream));
+        EMB_HIP(hipStreamSynchronize(sv->stream));
+    }
+    sv->batch.scenes = sv->d_scenes;
+    sv->batch.prefix = sv->d_prefix;
+    sv->batch.n_scenes = n;
+    drop_graph(sv);
+    sv->dirty = false;
+    return EMB_OK;
+}
+
+int capture_graph(EmbServer *sv) {
+    EMB_HIP(hipStreamBeginCapture(sv->stream, hipStreamCaptureModeThreadLocal));
+    hipError_t e = tick_kernels(sv->batch, sv->fdtd_max, sv->stream);
+    hipGraph_t g = nullptr;
+    hipError_t e2 = hipStreamEndCapture(sv->stream, &g);
+    if (e != hipSuccess || e2 != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return fail(EMB_E_DEVICE, std::string("graph capture: ") + hipGetErrorString(e != hipSuccess ? e : e2));
+    }
+    sv->graph = g;
+    EMB_HIP(hipGraphInstantiate(&sv->graph_exec, g, nullptr, nullptr, 0));
+    return EMB_OK;
+}
+
+int retire(EmbServer *sv, Request *q) {
+    Resident &r = q->r;
+    if (q->state == RUNNING && sv->cfg.keep_results) {
+        const DevScene &d = r.ds;
+        const size_t n = (size_t)d.n, nx = d.nx, ny = d.ny;
+        q->pos.resize(2 * n);
+        q->vel.resize(2 * n);
+        q->ex.resize((nx + 1) * ny);
+        q->ey.resize(nx * (ny + 1));
+        q->bz.resize((nx + 1) * (ny + 1));
+        std::vector<float> a(n), b(n);
+        EMB_HIP(hipMemcpy(a.data(), d.x, n * 4, hipMemcpyDeviceToHost));
+        EMB_HIP(hipMemcpy(b.data(), d.y, n * 4, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < n; i++) q->pos[2 * i] = a[i], q->pos[2 * i + 1] = b[i];
+        EMB_HIP(hipMemcpy(a.data(), d.vx, n * 4, hipMemcpyDeviceToHost));
+        EMB_HIP(hipMemcpy(b.data(), d.vy, n * 4, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < n; i++) q->vel[2 * i] = a[i], q->vel[2 * i + 1] = b[i];
+        EMB_HIP(hipMemcpy(q->ex.data(), d.ex, q->ex.size() * 4, hipMemcpyDeviceToHost));
+        EMB_HIP(hipMemcpy(q->ey.data(), d.ey, q->ey.size() * 4, hipMemcpyDeviceToHost));
+        EMB_HIP(hipMemcpy(q->bz.data(), d.bz, q->bz.size() * 4, hipMemcpyDeviceToHost));
+        q->has_result = true;
+    }
+    free_resident(r);
+    sv->used_bytes -= q->footprint;
+    return EMB_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int emb_server_create(EmbDevice *dev, const EmbSettings *settings, const EmbServerConfig *config, EmbServer **out) {
+    try {
+        if (!dev || !settings || !config || !out) return fail(EMB_E_INVALID, "null argument");
+        if (config->max_slots < 1 || config->max_slots > 4096) return fail(EMB_E_INVALID, "max_slots must be 1..4096");
+        if (config->ticks_per_round < 1 || config->ticks_per_round > TICK_TABLE)
+            return fail(EMB_E_INVALID, "ticks_per_round must be 1..64");
+        if (config->hbm_budget_bytes <= 0) return fail(EMB_E_INVALID, "hbm_budget_bytes must be positive");
+        EMB_HIP(hipSetDevice(dev->ordinal));
+        auto sv = std::make_unique<EmbServer>();
+        sv->dev = dev;
+        sv->s = *settings;
+        sv->cfg = *config;
+        EMB_HIP(hipStreamCreateWithFlags(&sv->stream, hipStreamNonBlocking));
+        EMB_HIP(hipMalloc(&sv->d_scenes, sizeof(DevScene) * config->max_slots));
+        EMB_HIP(hipMalloc(&sv->d_prefix, sizeof(int32_t) * KC_COUNT * (config->max_slots + 1)));
+        EMB_HIP(hipEventCreate(&sv->ev0));
+        EMB_HIP(hipEventCreate(&sv->ev1));
+        *out = sv.release();
+        return EMB_OK;
+    } catch (const std::exception &e) {
+        return fail(EMB_E_INVALID, e.what());
+    }
+}
+
+int emb_server_submit(EmbServer *sv, const EmbScene *scene, int64_t steps, int64_t *id) {
+    try {
+        if (!sv || !scene || !id || steps < 0) return fail(EMB_E_INVALID, "bad argument");
+        auto q = std::make_unique<Request>();
+        const std::string why = prepare_scene(sv->s, *scene, q->hs);
+        if (!why.empty()) return fail(EMB_E_INVALID, why);
+        q->footprint = scene_bytes(sv->s, q->hs.cols, q->hs.rows, q->hs.n, (int)q->hs.stacks.size(),
+                                   (int)q->hs.stack_cells.size(), (int)q->hs.bats.size(), (int)q->hs.ac_freq.size());
+        if ((int64_t)q->footprint > sv->cfg.hbm_budget_bytes)
+            return fail(EMB_E_NOMEM, "scene footprint exceeds the HBM budget");
+        q->steps = steps;
+        q->time0 = scene->time;
+        q->id = sv->next_id++;
+        *id = q->id;
+        sv->queue.push_back(q.get());
+        sv->all[q->id] = std::move(q);
+        if (sv->t_first < 0.0) sv->t_first = now_s();
+        return EMB_OK;
+    } catch (const std::exception &e) {
+        return fail(EMB_E_INVALID, e.what());
+    }
+}
+
+int emb_server_cancel(EmbServer *sv, int64_t id) {
+    if (!sv) return fail(EMB_E_INVALID, "null server");
+    auto it = sv->all.find(id);
+    if (it == sv->all.end()) return fail(EMB_E_INVALID, "no such request");
+    Request *q = it->second.get();
+    if (q->state == QUEUED) {
+        sv->queue.erase(std::find(sv->queue.begin(), sv->queue.end(), q));
+    } else if (q->state == RUNNING) {
+        EMB_HIP(hipStreamSynchronize(sv->stream));
+        sv->active.erase(std::find(sv->active.begin(), sv->active.end(), q));
+        q->state = CANCELLED;
+        int rc = retire(sv, q);
+        if (rc != EMB_OK) return rc;
+        sv->dirty = true;
+    }
+    q->state = CANCELLED;
+    return EMB_OK;
+}
+
+int emb_server_poll(EmbServer *sv, int32_t *active_out) {
+    try {
+        if (!sv) return fail(EMB_E_INVALID, "null server");
+        EMB_HIP(hipSetDevice(sv->dev->ordinal));
+        // Admission by footprint: in arrival order, while a slot and the budget allow.
+        while (!sv->queue.empty() && (int)sv->active.size() < sv->cfg.max_slots) {
+            Request *q = sv->queue.front();
+            if ((int64_t)(sv->used_bytes + q->footprint) > sv->cfg.hbm_budget_bytes) break;
+            sv->queue.pop_front();
+            int rc = make_resident(sv->dev, sv->s, std::move(q->hs), sv->stream, q->r);
+            if (rc != EMB_OK) {
+                free_resident(q->r);
+                q->state = CANCELLED;
+                return rc;
+            }
+            q->r.time = q->time0;
+            EMB_HIP(setup_kernels(q->r.batch, sv->stream));
+            sv->used_bytes += q->footprint;
+            q->state = RUNNING;
+            if (q->steps == 0) {   // nothing to run: finished on admission
+                EMB_HIP(hipStreamSynchronize(sv->stream));
+                q->state = RUNNING;
+                int rc2 = retire(sv, q);
+                q->state = DONE;
+                sv->completed++;
+                if (rc2 != EMB_OK) return rc2;
+                continue;
+            }
+            sv->active.push_back(q);
+            sv->dirty = true;
+        }
+        sv->peak = std::max(sv->peak, (int)sv->active.size());
+        if (!sv->active.empty()) {
+            if (sv->dirty) {
+                int rc = rebuild_batch(sv);
+                if (rc != EMB_OK) return rc;
+            }
+            int64_t k = sv->cfg.ticks_per_round;
+            for (Request *q : sv->active) k = std::min(k, q->steps - q->done);
+            for (Request *q : sv->active) {
+                int rc = enqueue_tick_tables(q->r, (int)k, sv->stream);
+                if (rc != EMB_OK) return rc;
+            }
+            const bool graphs = sv->cfg.batched && sv->cfg.use_graph;
+            if (graphs && !sv->graph_exec) {
+                int rc = capture_graph(sv);
+                if (rc != EMB_OK) return rc;
+            }
+            EMB_HIP(hipEventRecord(sv->ev0, sv->stream));
+            for (int64_t t = 0; t < k; t++) {
+                if (graphs) {
+                    EMB_HIP(hipGraphLaunch(sv->graph_exec, sv->stream));
+                } else if (sv->cfg.batched) {
+                    EMB_HIP(tick_kernels(sv->batch, sv->fdtd_max, sv->stream));
+                } else {
+                    for (Request *q : sv->active) EMB_HIP(tick_kernels(q->r.batch, q->r.ds.fdtd_substeps, sv->stream));
+                }
+            }
+            EMB_HIP(hipEventRecord(sv->ev1, sv->stream));
+            EMB_HIP(hipEventSynchronize(sv->ev1));
+            float ms = 0.0f;
+            EMB_HIP(hipEventElapsedTime(&ms, sv->ev0, sv->ev1));
+            const float per_step = ms / (float)k;
+            for (Request *q : sv->active) {
+                q->done += k;
+                sv->cell_steps += (int64_t)q->r.ds.nx * q->r.ds.ny * k;
+                for (int64_t t = 0; t < k; t++) sv->latencies_ms.push_back(per_step);
+            }
+            // Retire finished requests.
+            std::vector<Request *> keep;
+            for (Request *q : sv->active) {
+                if (q->done >= q->steps) {
+                    int rc = retire(sv, q);
+                    q->state = DONE;
+                    sv->completed++;
+                    sv->dirty = true;
+                    if (rc != EMB_OK) return rc;
+                } else {
+                    keep.push_back(q);
+                }
+            }
+            sv->active.swap(keep);
+            sv->t_last = now_s();
+        }
+        if (active_out) *active_out = (int32_t)(sv->active.size() + sv->queue.size());
+        return EMB_OK;
+    } catch (const std::exception &e) {
+        return fail(EMB_E_INVALID, e.what());
+    }
+}
+
+int emb_server_done(EmbServer *sv, int64_t id) {
+    if (!sv) return fail(EMB_E_INVALID, "null server");
+    auto it = sv->all.find(id);
+    if (it == sv->all.end()) return fail(EMB_E_INVALID, "no such request");
+    if (it->second->state == CANCELLED) return fail(EMB_E_STATE, "request was cancelled");
+    return it->second->state == DONE ? 1 : 0;
+}
+
+int emb_server_result(EmbServer *sv, int64_t id, double *pos, double *vel, double *ex, double *ey, double *bz) {
+    if (!sv) return fail(EMB_E_INVALID, "null server");
+    auto it = sv->all.find(id);
+    if (it == sv->all.end()) return fail(EMB_E_INVALID, "no such request");
+    Request *q = it->second.get();
+    if (q->state != DONE) return fail(EMB_E_STATE, "request not finished");
+    if (!q->has_result) return fail(EMB_E_STATE, "result not kept (keep_results = 0) or already read");
+    auto out = [](double *dst, const std::vector<float> &src) {
+        if (dst)
+            for (size_t i = 0; i < src.size(); i++) dst[i] = src[i];
+    };
+    out(pos, q->pos);
+    out(vel, q->vel);
+    out(ex, q->ex);
+    out(ey, q->ey);
+    out(bz, q->bz);
+    q->pos = q->vel = q->ex = q->ey = q->bz = {};
+    q->has_result = false;
+    return EMB_OK;
+}
+
+int emb_server_stats(EmbServer *sv, EmbServerStats *out) {
+    if (!sv || !out) return fail(EMB_E_INVALID, "null argument");
+    std::memset(out, 0, sizeof(*out));
+    out->completed = sv->completed;
+    out->cell_steps = sv->cell_steps;
+    out->elapsed_s = sv->t_first < 0.0 ? 0.0 : std::max(0.0, sv->t_last - sv->t_first);
+    out->cell_steps_per_s = out->elapsed_s > 0.0 ? (double)sv->cell_steps / out->elapsed_s : 0.0;
+    out->step_samples = (int64_t)sv->latencies_ms.size();
+    if (!sv->latencies_ms.empty()) {
+        std::vector<float> v = sv->latencies_ms;
+        auto pct = [&](double p) {
+            const size_t k = std::min(v.size() - 1, (size_t)std::floor(p / 100.0 * (double)(v.size() - 1) + 0.5));
+            std::nth_element(v.begin(), v.begin() + (long)k, v.end());
+            return (double)v[k];
+        };
+        out->step_latency_p50_ms = pct(50.0);
+        out->step_latency_p99_ms = pct(99.0);
+    }
+    out->peak_resident = sv->peak;
+    return EMB_OK;
+}
+
+int emb_server_destroy(EmbServer *sv) {
+    if (!sv) return EMB_OK;
+    (void)hipSetDevice(sv->dev->ordinal);
+    (void)hipStreamSynchronize(sv->stream);
+    for (Request *q : sv->active) free_resident(q->r);
+    drop_graph(sv);
+    if (sv->d_scenes) (void)hipFree(sv->d_scenes);
+    if (sv->d_prefix) (void)hipFree(sv->d_prefix);
+    if (sv->ev0) (void)hipEventDestroy(sv->ev0);
+    if (sv->ev1) (void)hipEventDestroy(sv->ev1);
+    (void)hipStreamDestroy(sv->stream);
+    delete sv;
+    return EMB_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,79 @@
+// setup.hpp - host-side FP64 preparation of a scene for the device.
+//
+// Everything here runs once per scene (or once per table parameter set) on the
+// host, in FP64, and is uploaded as FP32 / int64 arrays:
+//   * close-range tables: grid force, radial tables, square table, ion stencil
+//     (crates/embloks-core/src/close_range.rs:67-242);
+//   * islands and battery stacks (connectivity.rs:20-62, battery.rs:100-190);
+//   * per-cell drag, static charge (ions + fixed charges) in fixed point, the
+//     ions' close-range potential (physics.rs:17-21, fields.rs:91-112,
+//     close_range.rs:379-403);
+//   * absorbing-margin damping profiles, DST-I matrices and the inverse
+//     Laplacian (maxwell.rs:56-134).
+#pragma once
+
+#include <stdint.h>
+
+#include <memory>
+#include <vector>
+
+#include "../../include/embloks_hip.h"
+#include "common.hpp"
+
+namespace emb {
+
+struct HostTables {
+    double softening, cutoff;
+    int passes;
+    std::vector<double> grid_r, grid_f;   // grid_force
+    std::vector<double> g, potential;     // PAIR_SAMPLES + 2
+    std::vector<double> square;           // n * n * 3
+    int square_n = 0;
+    double half = 0.0;
+    std::vector<double> ion_stencil;      // (2 reach + 1)^2
+    int reach = 0;
+};
+
+// close_range::tables, cached per parameter set for the process.
+std::shared_ptr<const HostTables> host_tables(double softening, int passes, double cutoff);
+
+struct HostStack {
+    std::vector<std::pair<int, int>> cells;
+    int low[2], high[2];
+    double emf, internal_resistance, amplitude;
+    std::vector<std::pair<double, double>> sources;   // (signed EMF, frequency) per cell
+    double emf_at(double t) const;                     // battery.rs:86-96
+};
+
+// Everything about one scene that the device needs, prepared in FP64.
+struct HostScene {
+    int cols = 0, rows = 0, margin = 0, nx = 0, ny = 0, fdtd_substeps = 1;
+    int64_t n = 0;
+    double h_fdtd = 0.0;
+    double q = 0.0, mass = 0.0;
+    std::vector<int64_t> labels;
+    std::vector<HostStack> stacks;
+    std::vector<StackInfo> stack_info;
+    std::vector<int32_t> stack_cells;
+    std::vector<CellInfo> info;
+    std::vector<BatteryCell> bats;
+    std::vector<double> ac_freq;              // per AC slot
+    std::vector<float> density, ion_pot;
+    std::vector<long long> static_rho;
+    std::vector<float> damp[6];               // ex_x, ex_y, ey_x, ey_y, bz_x, bz_y
+    std::vector<float> inv_lap;
+    std::vector<float> gains;
+    std::vector<int32_t> has_gain;
+    std::vector<float> x, y, vx, vy;
+};
+
+// Returns an empty string on success, else what is wrong with the scene.
+std::string prepare_scene(const EmbSettings &s, const EmbScene &scene, HostScene &out);
+
+// The DST-I matrix sin(pi (k+1)(j+1) / (n+1)), FP64 rounded to FP32.
+std::vector<float> sine_matrix(int n);
+
+// Electron substeps of the FDTD step (maxwell.rs:376-378).
+int fdtd_substeps_of(const EmbSettings &s);
+
+}  // namespace emb

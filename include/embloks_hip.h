@@ -128,6 +128,10 @@ int emb_sim_dims(EmbSim *sim, EmbDims *out);
 int emb_sim_read_state(EmbSim *sim, double *pos, double *vel, double *gains);
 /* read_yee (sim.rs:818-823). */
 int emb_sim_read_yee(EmbSim *sim, double *ex, double *ey, double *bz);
+/* Overwrite the Yee fields (NULL leaves a component as is). The CPU backend
+ * exposes them the same way (`Simulation::maxwell` is public, simulation.rs:15-20;
+ * crates/embloks-gpu/tests/sim.rs:208 assigns cpu.maxwell.ex). */
+int emb_sim_write_yee(EmbSim *sim, const double *ex, const double *ey, const double *bz);
 /* read_potential (sim.rs:826-830): the potential of the charges (nx, ny). */
 int emb_sim_read_potential(EmbSim *sim, double *potential);
 /* read_current (sim.rs:843-849): physics::current_density, (cols, rows, 2). */
@@ -142,6 +146,13 @@ int emb_sim_read_charge_fixed(EmbSim *sim, int64_t *rho);
  * total deposited electron charge (exact, in electron charges), field energy
  * sum(Ex^2) + sum(Ey^2) + c^2 sum(Bz^2). No counterpart in the reference. */
 int emb_sim_diagnostics(EmbSim *sim, double *gauss_residual, double *electron_charge, double *field_energy);
+/* Run `ticks` more ticks with a HIP event between the kernel phases of each tick
+ * on the sim's stream and write each phase's average duration (ms) to avg_ms.
+ * Returns the number of phases (avg_ms NULL: just the count). Names from
+ * emb_tick_phase_name. No counterpart in the reference (which times nothing
+ * but the wall clock, crates/embloks-gpu/examples/bench.rs:47-54). */
+int emb_sim_kernel_times(EmbSim *sim, int ticks, double *avg_ms, int capacity);
+const char *emb_tick_phase_name(int phase);
 int emb_sim_destroy(EmbSim *sim);
 
 /* --- Continuous-batching serving loop (no counterpart in the reference) ----------
@@ -154,8 +165,10 @@ typedef struct EmbServerConfig {
     int64_t hbm_budget_bytes;
     int32_t max_slots;          /* scenes resident at once */
     int32_t batched;            /* 1: batched kernels + graphs, 0: one scene per launch */
-    int32_t ticks_per_round;    /* ticks advanced between admission rounds */
-    int32_t use_graph;
+    int32_t ticks_per_round;    /* ticks advanced between admission rounds (<= 64) */
+    int32_t use_graph;          /* batched mode: replay each tick as a captured HIP graph */
+    int32_t keep_results;       /* 1: keep each finished request's state until read */
+    int32_t reserved0;
 } EmbServerConfig;
 
 typedef struct EmbServerStats {
@@ -184,8 +197,9 @@ int emb_server_cancel(EmbServer *srv, int64_t id);
 int emb_server_poll(EmbServer *srv, int32_t *active);
 /* 1 if request `id` finished (its result can be read), 0 if not yet, <0 error. */
 int emb_server_done(EmbServer *srv, int64_t id);
-/* Result of a finished request: electrons and fields as emb_sim_read_state /
- * emb_sim_read_yee would give them. Any pointer may be NULL. */
+/* Result of a finished request (kept when keep_results is set): electrons and
+ * fields as emb_sim_read_state / emb_sim_read_yee would give them. Any pointer
+ * may be NULL. Reading releases the result. */
 int emb_server_result(EmbServer *srv, int64_t id, double *pos, double *vel, double *ex, double *ey, double *bz);
 int emb_server_stats(EmbServer *srv, EmbServerStats *out);
 int emb_server_destroy(EmbServer *srv);

@@ -1,0 +1,184 @@
+"""The HIP backend against the FP64 oracle, through the C ABI.
+
+Mirrors crates/embloks-gpu/tests/sim.rs (static fields, ticks following the CPU)
+with the reference's own tolerances, and adds the stricter checks this backend
+is built to: per-tick reset-and-compare at 1e-5 relative (max|a - e| / max|e|,
+fixtures.rs:39-46), a bit-exact integer deposit, Gauss's law, determinism.
+"""
+import numpy as np
+import pytest
+
+from embloks_amd import world as W
+from oracle import orc
+from tests.golden.fixtures import SCENES, array, load, relative_error, world_from
+
+gpu = pytest.mark.gpu
+
+
+def gpusim():
+    from embloks_amd.gpusim import GpuSim
+    return GpuSim
+
+
+def fixture_world(name):
+    return world_from(load(name))
+
+
+@gpu
+@pytest.mark.parametrize("name", ["scene_f1", "scene_f2", "scene_f4", "scene_f7"])
+def test_static_fields_match_the_oracle(name):
+    # crates/embloks-gpu/tests/sim.rs:45-60 (potential 1e-4, E 1e-3); held here to 1e-5.
+    world = fixture_world(name)
+    cpu = orc.Sim(world)
+    sim = gpusim()(world)
+    ex, ey, bz = sim.read_yee()
+    cex, cey, cbz = cpu.yee()
+    assert relative_error(sim.read_potential(), cpu.potential_nodes()) < 1e-5
+    assert relative_error(ex, cex) < 1e-5
+    assert relative_error(ey, cey) < 1e-5
+    assert np.all(bz == 0.0)
+    assert relative_error(sim.read_current(), cpu.current()) < 1e-6
+
+
+def reset_and_compare(world, ticks_before=0):
+    """Both backends from the same state; compare after one tick."""
+    cpu = orc.Sim(world)
+    if ticks_before:
+        cpu.step(ticks_before)
+        pos, vel, gains = cpu.state()
+        ex, ey, bz = cpu.yee()
+        w = world.copy()
+        w.pos, w.vel, w.gains, w.time = pos, vel, gains, cpu.time
+        sim = gpusim()(w)
+        sim.write_yee(ex, ey, bz)
+    else:
+        sim = gpusim()(world)
+    cpu.step(1)
+    sim.run_ticks(1)
+    return cpu, sim
+
+
+@gpu
+@pytest.mark.parametrize("name", SCENES)
+@pytest.mark.parametrize("before", [0, 20])
+def test_one_tick_matches_the_oracle(name, before):
+    cpu, sim = reset_and_compare(fixture_world(name), before)
+    ex, ey, bz = sim.read_yee()
+    cex, cey, cbz = cpu.yee()
+    err = {
+        "ex": relative_error(ex, cex),
+        "ey": relative_error(ey, cey),
+        "bz": relative_error(bz, cbz),
+        "phi": relative_error(sim.read_potential(), cpu.potential_nodes()),
+    }
+    pos, vel, gains = sim.read_state()
+    cpos, cvel, cgains = cpu.state()
+    err["pos"] = float(np.max(np.abs(pos - cpos)))
+    print(name, before, err)
+    for k in ("ex", "ey", "phi"):
+        assert err[k] < 1e-5, err
+    assert err["bz"] < 1e-4, err
+    assert err["pos"] < 1e-4, err
+    for g, c in zip(gains, cgains):
+        assert (g is None) == (c is None)
+        if g is not None:
+            assert abs(g - c) < 1e-5
+
+
+@gpu
+@pytest.mark.parametrize("index", [0, 1, 5])
+def test_ticks_follow_the_oracle(index):
+    # crates/embloks-gpu/tests/sim.rs:62-90 with its tolerances.
+    world = W.scene(index)
+    cpu = orc.Sim(world)
+    sim = gpusim()(world)
+    ticks = 10
+    sim.run_ticks(ticks)
+    cpu.step(ticks)
+    pos, vel, gains = sim.read_state()
+    cpos, cvel, cgains = cpu.state()
+    worst = float(np.max(np.abs(pos - cpos)))
+    assert worst < 2e-3, worst
+    ex, ey, bz = sim.read_yee()
+    cex, cey, cbz = cpu.yee()
+    assert relative_error(ex, cex) < 2e-2
+    assert relative_error(ey, cey) < 2e-2
+    assert relative_error(bz, cbz) < 5e-2
+    assert relative_error(sim.read_current(), cpu.current()) < 5e-2
+    for g, c in zip(gains, cgains):
+        assert abs((g if g is not None else 1.0) - (c if c is not None else 1.0)) < 1e-3
+
+
+def fixed_point_deposit(pos32, cols, rows):
+    """The integer CIC deposit (fields.rs:27-51 with 20-bit weights), restated in
+    numpy from the same FP32 positions: electron charge in units of 2^-40."""
+    x = pos32[:, 0].astype(np.float32)
+    y = pos32[:, 1].astype(np.float32)
+    out = np.zeros(cols * rows, dtype=np.int64)
+    u = (x - np.float32(0.5)).astype(np.float32)
+    v = (y - np.float32(0.5)).astype(np.float32)
+    cu, cv = np.floor(u), np.floor(v)
+    fu = np.rint((u - cu) * np.float32(1048576.0)).astype(np.int64)
+    fv = np.rint((v - cv) * np.float32(1048576.0)).astype(np.int64)
+    c0, r0 = cu.astype(np.int64), cv.astype(np.int64)
+    for a in (0, 1):
+        wx = fu if a else (1 << 20) - fu
+        cc = np.clip(c0 + a, 0, cols - 1)
+        for b in (0, 1):
+            wy = fv if b else (1 << 20) - fv
+            rr = np.clip(r0 + b, 0, rows - 1)
+            np.add.at(out, cc * rows + rr, -(wx * wy))
+    return out.reshape(cols, rows)
+
+
+@gpu
+@pytest.mark.parametrize("index", [0, 2, 5])
+def test_deposit_is_bit_exact_and_conserves_charge(index):
+    world = W.scene(index)
+    sim = gpusim()(world)
+    sim.run_ticks(7)
+    pos, _, _ = sim.read_state()
+    got = sim.read_charge_fixed()
+    want = fixed_point_deposit(pos, world.cols, world.rows)
+    np.testing.assert_array_equal(got, want)
+    assert int(got.sum()) == -world.n_electrons * (1 << 40)
+    residual, charge, energy = sim.diagnostics()
+    q0 = world.settings["cell_charge"] / world.settings["carriers_per_cell"]
+    assert charge == -world.n_electrons * q0
+    assert np.isfinite(energy)
+
+
+@gpu
+def test_gauss_law_holds_after_ticks():
+    world = W.scene(0)
+    sim = gpusim()(world)
+    sim.run_ticks(30)
+    residual, _, _ = sim.diagnostics()
+    # div E - 4 pi k rho in FP32 fields: round-off of the O(10) values involved.
+    assert residual < 1e-4, residual
+
+
+@gpu
+def test_runs_are_bitwise_deterministic():
+    world = W.scene(5)
+    a, b = gpusim()(world), gpusim()(world)
+    a.run_ticks(25)
+    b.run_ticks(25)
+    for x, y in zip(a.read_yee() + a.read_state()[:2], b.read_yee() + b.read_state()[:2]):
+        np.testing.assert_array_equal(x, y)
+
+
+@gpu
+def test_fixture_scene_thirty_ticks():
+    # The golden 30-tick runs (fixtures.rs:128-186) in FP32: positions and fields
+    # at the reference GPU's own 10-tick bar.
+    for name in ("scene_f1", "scene_f7"):
+        f = load(name)
+        sim = gpusim()(world_from(f))
+        sim.run_ticks(f["ticks"])
+        pos, _, gains = sim.read_state()
+        assert np.max(np.abs(pos - array(f["final_pos"]))) < 2e-3
+        e, b = sim.node_fields()
+        assert relative_error(e, array(f["final_e"])) < 2e-2
+        for g, want in zip(gains, f["gains"]):
+            assert abs(g - want) < 1e-3
