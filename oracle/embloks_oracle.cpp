@@ -1494,6 +1494,22 @@ int orc_sim_stacks(const OrcSim *sim, int *low_high, int *n_cells) {
     return (int)st.size();
 }
 
+void orc_sim_confine(const OrcSim *sim, const double *old_pos, const double *new_pos, const double *vel,
+                     double restitution, double *out_pos, double *out_vel) {
+    double pos[2] = {new_pos[0], new_pos[1]}, v[2] = {vel[0], vel[1]};
+    confine(old_pos, pos, v, sim->world, restitution);
+    out_pos[0] = pos[0];
+    out_pos[1] = pos[1];
+    out_vel[0] = v[0];
+    out_vel[1] = v[1];
+}
+
+void orc_kick(const double *vel, const double *acc, double damping, double dt, double *out) {
+    const double scale = 1.0 / (1.0 + damping * dt);  // physics.rs:57-60
+    out[0] = (vel[0] + acc[0] * dt) * scale;
+    out[1] = (vel[1] + acc[1] * dt) * scale;
+}
+
 void orc_sim_labels(const OrcSim *sim, int64_t *labels) {
     std::copy(sim->world.labels.begin(), sim->world.labels.end(), labels);
 }

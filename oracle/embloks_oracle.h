@@ -75,6 +75,11 @@ void orc_sim_close_force(const OrcSim *sim, double *force);
 /* Stacks found in the scene: low/high terminals (c, r) and cell counts. */
 int orc_sim_stacks(const OrcSim *sim, int *low_high, int *n_cells);
 void orc_sim_labels(const OrcSim *sim, int64_t *labels);
+/* physics::confine (physics.rs:66-115) against the sim's islands. */
+void orc_sim_confine(const OrcSim *sim, const double *old_pos, const double *new_pos, const double *vel,
+                     double restitution, double *out_pos, double *out_vel);
+/* physics::kick (physics.rs:57-60). */
+void orc_kick(const double *vel, const double *acc, double damping, double dt, double *out);
 
 #ifdef __cplusplus
 }

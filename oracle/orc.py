@@ -76,6 +76,8 @@ def lib():
             "orc_sim_close_force": (None, [vp, D]),
             "orc_sim_stacks": (C.c_int, [vp, I32, I32]),
             "orc_sim_labels": (None, [vp, I64]),
+            "orc_sim_confine": (None, [vp, D, D, D, C.c_double, D, D]),
+            "orc_kick": (None, [D, D, C.c_double, C.c_double, D]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -216,6 +218,12 @@ def tables(softening=0.25, passes=2, cutoff=3.0):
     return g, pot, square, half.value, stencil
 
 
+def kick(vel, acc, damping, dt):
+    out = np.zeros(2)
+    lib().orc_kick(ptr(f64(vel)), ptr(f64(acc)), float(damping), float(dt), ptr(out))
+    return out
+
+
 def compact_g(r2, softening=0.25):
     return lib().orc_compact_g(float(r2), float(softening))
 
@@ -294,6 +302,12 @@ class Sim:
         nc = np.zeros(max(self.n_stacks, 1), dtype=np.int32)
         lib().orc_sim_stacks(self.h, lh.ctypes.data_as(I32), nc.ctypes.data_as(I32))
         return lh[: 4 * self.n_stacks].reshape(-1, 4), nc[: self.n_stacks]
+
+    def confine(self, old, new, vel, restitution):
+        old, new, vel = f64(old), f64(new), f64(vel)
+        p, v = np.zeros(2), np.zeros(2)
+        lib().orc_sim_confine(self.h, ptr(old), ptr(new), ptr(vel), float(restitution), ptr(p), ptr(v))
+        return p, v
 
     def labels(self):
         out = np.zeros((self.cols, self.rows), dtype=np.int64)
