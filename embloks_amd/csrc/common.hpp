@@ -14,15 +14,29 @@
 
 #include <stdint.h>
 
+#ifndef __HIPCC__
+#define __host__
+#define __device__
+#endif
+
 namespace emb {
 
 // Fixed-point scales of the deterministic deposit: electron charge in units of
-// 2^-40 of one electron (CIC weights carry 20 fractional bits per axis), q*v in
-// units of 2^-28 (charge velocity of one electron).
-constexpr int W_BITS = 20;
-constexpr double RHO_SCALE = 1099511627776.0;   // 2^40
+// 2^-44 of one electron (CIC weights carry 22 fractional bits per axis), q*v in
+// units of 2^-28 (charge velocity of one electron). With two smoothing passes
+// (x 256) a cell holds at most 2^(63-44-8) = 2048 electron charges of either sign.
+constexpr int W_BITS = 22;
+constexpr double RHO_SCALE = 17592186044416.0;  // 2^44
 constexpr double J_SCALE = 268435456.0;         // 2^28
-constexpr float J_FROM_W = 1.0f / 4096.0f;      // 2^28 / 2^40
+constexpr float J_FROM_W = 1.0f / 65536.0f;     // 2^28 / 2^44
+
+// Electron positions are kept as an integer cell (two int16 packed in an int32,
+// x in the low half) plus FP32 fractions in [0, 1): 2^-24 cells of resolution
+// anywhere on the grid, where a plain FP32 coordinate near x = 1000 resolves
+// only 6e-5 cells.
+__host__ __device__ inline int32_t pack_cell(int cx, int cy) { return (int32_t)(((uint32_t)(uint16_t)cx) | ((uint32_t)cy << 16)); }
+__host__ __device__ inline int cell_x(int32_t p) { return (int)(int16_t)(p & 0xffff); }
+__host__ __device__ inline int cell_y(int32_t p) { return (int)(p >> 16); }
 
 constexpr int MAX_STACKS_PER_BLOCK = 64;
 constexpr int SCAN_TILE = 1024;   // cells per block of the cell-list scan
@@ -91,7 +105,7 @@ struct DevScene {
     float jscale;             // 0.5 * 4 pi k h (maxwell.rs:240-248)
     float four_pi_k;
     float trim_rate, gain_low, gain_high;
-    float rho_from_fixed;     // q_e / 2^40 / 4^(2 passes): int64 charge -> FP32
+    float rho_from_fixed;     // |q_e| / 2^44 / 4^(2 passes): int64 charge -> FP32
     float j_from_fixed;       // q_e / 2^28 / 4^(2 passes)
     float i_from_fixed;       // q_e / 2^28 (cell-binned current)
     float pad_f;
@@ -104,10 +118,12 @@ struct DevScene {
     const float *damp_ex_x, *damp_ex_y, *damp_ey_x, *damp_ey_y, *damp_bz_x, *damp_bz_y;
     const float *sine_x, *sine_y;     // DST-I matrices (nx^2, ny^2)
     const float *inv_lap;             // norm / (lambda_x + lambda_y), (nx, ny) (maxwell.rs:60-67)
-    // --- electrons (SoA, upload order)
-    float *x, *y, *vx, *vy;
+    // --- electrons (SoA, upload order): cell + fraction, velocity
+    int32_t *cxy;
+    float *fx, *fy, *vx, *vy;
     // --- cell list
-    float *xs, *ys, *vxs, *vys;   // electrons in sorted order (snapshot of the substep start)
+    int32_t *scell;           // per sorted slot: flat cell index (c * rows + r)
+    float *sfx, *sfy, *svx, *svy; // electrons in sorted order (snapshot of the substep start)
     int32_t *order;           // electron index per sorted slot
     int32_t *ecell, *erank;   // per electron: cell and rank within it
     int32_t *count;           // per cell (zero between sorts)
@@ -116,7 +132,7 @@ struct DevScene {
     int32_t *cstart;          // per cell + 1: absolute start in the sorted order
     // --- deposit
     long long *cur;           // [2 * cells]: cell-binned sum of v, units 2^-28 (x q_e)
-    const long long *static_rho;   // ions + fixed charges per cell, units 2^-40 electron charges
+    const long long *static_rho;   // ions + fixed charges per cell, units 2^-44 electron charges
     long long *dep_dump;      // optional [3 * cells]: unsmoothed electron deposit (diagnostics)
     // --- materials, tables, batteries
     const CellInfo *info;

@@ -28,10 +28,10 @@ __device__ __forceinline__ int block_in_scene(const Batch &b, int kc, int s, int
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
-// Cell of a position, clamped onto the grid (physics.rs:24-29).
-__device__ __forceinline__ int cell_of(const DevScene &S, float x, float y) {
-    const int c = clampi((int)floorf(x), 0, S.cols - 1);
-    const int r = clampi((int)floorf(y), 0, S.rows - 1);
+// Flat cell of an electron, clamped onto the grid (physics.rs:24-29).
+__device__ __forceinline__ int cell_of(const DevScene &S, int32_t cxy) {
+    const int c = clampi(cell_x(cxy), 0, S.cols - 1);
+    const int r = clampi(cell_y(cxy), 0, S.rows - 1);
     return c * S.rows + r;
 }
 

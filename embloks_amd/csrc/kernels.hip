@@ -34,7 +34,7 @@ __global__ __launch_bounds__(256) void k_count(Batch b) {
         const DevScene &S = b.scenes[s];
         const int i = block_in_scene(b, KC_ELECTRONS, s, blk) * 256 + (int)threadIdx.x;
         if (i < S.n) {
-            const int cell = cell_of(S, S.x[i], S.y[i]);
+            const int cell = cell_of(S, S.cxy[i]);
             S.erank[i] = atomicAdd(&S.count[cell], 1);
             S.ecell[i] = cell;
         }
@@ -133,10 +133,11 @@ __device__ __forceinline__ void canon_cell(const DevScene &S, int c, int s0, int
     }
     for (int a = s0; a < s1; a++) {
         const int e = order[a];
-        S.xs[a] = S.x[e];
-        S.ys[a] = S.y[e];
-        S.vxs[a] = S.vx[e];
-        S.vys[a] = S.vy[e];
+        S.scell[a] = c;
+        S.sfx[a] = S.fx[e];
+        S.sfy[a] = S.fy[e];
+        S.svx[a] = S.vx[e];
+        S.svy[a] = S.vy[e];
     }
 }
 
@@ -219,10 +220,11 @@ __global__ __launch_bounds__(64) void k_trim(Batch b) {
                 const int cc = c + (sub - 1) / side - R, rr = r + (sub - 1) % side - R;
                 if (cc >= 0 && rr >= 0 && cc < S.cols && rr < S.rows) {
                     const int nb = cc * S.rows + rr;
-                    const float cx = (float)c + 0.5f, cy = (float)r + 0.5f;
+                    // Offset of the electron from the centre of cell (c, r).
+                    const float ox = (float)(cc - c) - 0.5f + S.half, oy = (float)(rr - r) - 0.5f + S.half;
                     for (int k = S.cstart[nb]; k < S.cstart[nb + 1]; k++) {
-                        const float u = (S.xs[k] - cx + S.half) * S.square_samples;
-                        const float w = (S.ys[k] - cy + S.half) * S.square_samples;
+                        const float u = (ox + S.sfx[k]) * S.square_samples;
+                        const float w = (oy + S.sfy[k]) * S.square_samples;
                         v += square_floor(S, 0, u, w);
                     }
                     v *= S.k_q;
@@ -259,14 +261,13 @@ __global__ __launch_bounds__(64) void k_trim(Batch b) {
 // ---------------------------------------------------------------------------------------
 // Electron push (physics.rs:124-162; particles.wgsl:193-227).
 
-// Bilinear interpolation of a (w, h) array at (cell coordinate + offset + margin),
-// with the index clamped like Maxwell::bilinear (maxwell.rs:381-390). The integer
-// and fractional parts are split before adding the margin, so no bits are lost.
-__device__ __forceinline__ float gather(const float *a, int w, int h, float gx, float gy, int m) {
-    const float fgx = floorf(gx), fgy = floorf(gy);
-    const int i = clampi((int)fgx + m, 0, w - 2), j = clampi((int)fgy + m, 0, h - 2);
-    const float fx = gx - (float)(i - m), fy = gy - (float)(j - m);
-    const float *p = a + (size_t)i * h + j;
+// Bilinear interpolation of a (w, h) array at array coordinates (i + fi, j + fj)
+// with fi, fj in [0, 1], the index clamped like Maxwell::bilinear
+// (maxwell.rs:381-390).
+__device__ __forceinline__ float gather(const float *a, int w, int h, int i, float fi, int j, float fj) {
+    const int ic = clampi(i, 0, w - 2), jc = clampi(j, 0, h - 2);
+    const float fx = (float)(i - ic) + fi, fy = (float)(j - jc) + fj;
+    const float *p = a + (size_t)ic * h + jc;
     return p[0] * (1.0f - fx) * (1.0f - fy) + p[h] * fx * (1.0f - fy) + p[1] * (1.0f - fx) * fy + p[h + 1] * fx * fy;
 }
 
@@ -274,32 +275,32 @@ __device__ __forceinline__ bool conductive(const DevScene &S, int c, int r) {
     return (S.info[c * S.rows + r].flags & F_CONDUCTIVE) != 0u;
 }
 
-// Lowest and highest FP32 positions inside cell `start` at least 1e-6 from its
-// faces (physics.rs:10, 113), or one ulp where FP32 cannot resolve 1e-6.
-__device__ __forceinline__ float wall_low(int start) {
-    const float s = (float)start, v = s + 1e-6f;
-    return v > s ? v : nextafterf(s, INFINITY);
-}
-__device__ __forceinline__ float wall_high(int start) {
-    const float s = (float)(start + 1), v = s - 1e-6f;
-    return v < s ? v : nextafterf(s, -INFINITY);
-}
-
-// physics.rs:83-115 for one axis.
+// physics.rs:83-115 for one axis, in coordinates relative to the start cell:
+// p is the target's offset from the start cell's low face. A move that stays
+// in the cell or crosses an open face is kept; a closed face (or the grid edge)
+// reflects it with restitution, 1e-6 inside the cell (WALL_GAP, physics.rs:10).
 __device__ __forceinline__ void resolve(const DevScene &S, int axis, int start, int other, float &p, float &v) {
-    const int end = (int)floorf(p);
-    if (end == start) return;
-    const bool forward = end > start;
+    const int step = (int)floorf(p);
+    if (step == 0) return;
+    const bool forward = step > 0;
     const int face = forward ? start : start - 1;
     const int n = axis == 0 ? S.cols : S.rows, n_other = axis == 0 ? S.rows : S.cols;
     bool open = face >= 0 && face < n - 1 && other >= 0 && other < n_other;
     if (open) open = axis == 0 ? (conductive(S, face, other) && conductive(S, face + 1, other))
                                : (conductive(S, other, face) && conductive(S, other, face + 1));
     if (open) return;
-    const float wall = (float)(forward ? start + 1 : start);
+    const float wall = forward ? 1.0f : 0.0f;
     const float bounced = wall - S.restitution * (p - wall);
-    p = fminf(fmaxf(bounced, wall_low(start)), wall_high(start));
+    p = fminf(fmaxf(bounced, 1e-6f), 1.0f - 1e-6f);
     v *= -S.restitution;
+}
+
+// Split a cell-relative coordinate back into cell and fraction in [0, 1).
+__device__ __forceinline__ void renormalise(int &cell, float &f) {
+    const float fl = floorf(f);
+    cell += (int)fl;
+    f -= fl;
+    if (f >= 1.0f) f = 0.99999994f;   // -tiny + 1 rounded up: stay below the face
 }
 
 __device__ __forceinline__ int substeps_of(const DevScene &S, uint32_t t) {
@@ -311,16 +312,19 @@ __device__ __forceinline__ int substeps_of(const DevScene &S, uint32_t t) {
 
 __device__ void push_one(const DevScene &S, int slot, int k, float h, uint32_t t) {
     const int id = S.order[slot];
-    const float px = S.xs[slot], py = S.ys[slot];
-    const float vx = S.vxs[slot], vy = S.vys[slot];
-    const int hc = clampi((int)floorf(px), 0, S.cols - 1), hr = clampi((int)floorf(py), 0, S.rows - 1);
-    const int home = hc * S.rows + hr;
+    const int home = S.scell[slot];
+    const int hc = home / S.rows, hr = home - (home / S.rows) * S.rows;
+    const float px = S.sfx[slot], py = S.sfy[slot];   // fractions within the home cell
+    const float vx = S.svx[slot], vy = S.svy[slot];
     const CellInfo info = S.info[home];
     const int m = S.margin;
-    // Maxwell::gather (maxwell.rs:336-350): each component from its staggered grid.
-    float ex = gather(S.ex, S.nx + 1, S.ny, px, py - 0.5f, m);
-    float ey = gather(S.ey, S.nx, S.ny + 1, px - 0.5f, py, m);
-    const float bz = gather(S.bz, S.nx + 1, S.ny + 1, px, py, m);
+    // Maxwell::gather (maxwell.rs:336-350): Ex at (x + m, y - 1/2 + m), Ey at
+    // (x - 1/2 + m, y + m), Bz at (x + m, y + m), each from its staggered grid.
+    const int hx = px >= 0.5f ? hc : hc - 1, hy = py >= 0.5f ? hr : hr - 1;
+    const float qx = px >= 0.5f ? px - 0.5f : px + 0.5f, qy = py >= 0.5f ? py - 0.5f : py + 0.5f;
+    float ex = gather(S.ex, S.nx + 1, S.ny, hc + m, px, hy + m, qy);
+    float ey = gather(S.ey, S.nx, S.ny + 1, hx + m, qx, hr + m, py);
+    const float bz = gather(S.bz, S.nx + 1, S.ny + 1, hc + m, px, hr + m, py);
     // battery::impressed_field_grid (battery.rs:210-228).
     if (info.flags & F_BATTERY) {
         const BatteryCell bat = S.bats[(info.flags >> F_BAT_SHIFT) - 1];
@@ -343,12 +347,14 @@ __device__ void push_one(const DevScene &S, int slot, int k, float h, uint32_t t
         const int c_lo = max(hc - R, 0), c_hi = min(hc + R, S.cols - 1);
         const int r_lo = max(hr - R, 0), r_hi = min(hr + R, S.rows - 1);
         for (int cc = c_lo; cc <= c_hi; cc++) {
+            const float ox = (float)(hc - cc);
             for (int rr = r_lo; rr <= r_hi; rr++) {
+                const float oy = (float)(hr - rr);
                 const int cell = cc * S.rows + rr;
                 const float dens = S.density[cell];
                 if (dens > 0.0f) {
-                    const float u = (px - (float)cc - 0.5f + S.half) * S.square_samples;
-                    const float w = (py - (float)rr - 0.5f + S.half) * S.square_samples;
+                    const float u = (ox + (px - 0.5f) + S.half) * S.square_samples;
+                    const float w = (oy + (py - 0.5f) + S.half) * S.square_samples;
                     const float coef = S.k_ion_q * dens;
                     fx += coef * square_trunc(S, 1, u, w);
                     fy += coef * square_trunc(S, 2, u, w);
@@ -356,7 +362,7 @@ __device__ void push_one(const DevScene &S, int slot, int k, float h, uint32_t t
                 const int e = S.cstart[cell + 1];
                 for (int j = S.cstart[cell]; j < e; j++) {
                     if (j == slot) continue;
-                    const float dx = px - S.xs[j], dy = py - S.ys[j];
+                    const float dx = ox + (px - S.sfx[j]), dy = oy + (py - S.sfy[j]);
                     const float d2 = dx * dx + dy * dy;
                     if (d2 >= S.cutoff2) continue;
                     const float tt = d2 * S.g_scale;
@@ -381,11 +387,14 @@ __device__ void push_one(const DevScene &S, int slot, int k, float h, uint32_t t
     my *= limit;
     // confine (physics.rs:66-79): x first, then y from the resulting column.
     float tx = px + mx, ty = py + my;
-    const int ocx = (int)floorf(px), ocy = (int)floorf(py);
-    resolve(S, 0, ocx, ocy, tx, nvx);
-    resolve(S, 1, ocy, (int)floorf(tx), ty, nvy);
-    S.x[id] = tx;
-    S.y[id] = ty;
+    int cx = hc, cy = hr;
+    resolve(S, 0, hc, hr, tx, nvx);
+    renormalise(cx, tx);
+    resolve(S, 1, hr, cx, ty, nvy);
+    renormalise(cy, ty);
+    S.cxy[id] = pack_cell(cx, cy);
+    S.fx[id] = tx;
+    S.fy[id] = ty;
     S.vx[id] = nvx;
     S.vy[id] = nvy;
 }
@@ -422,7 +431,7 @@ __global__ __launch_bounds__(1024) void k_rest(Batch b) {
             __threadfence();
             __syncthreads();
             for (int i = tid; i < S.n; i += 1024) {
-                const int cell = cell_of(S, S.x[i], S.y[i]);
+                const int cell = cell_of(S, S.cxy[i]);
                 S.erank[i] = atomicAdd(&S.count[cell], 1);
                 S.ecell[i] = cell;
             }
@@ -509,14 +518,18 @@ __global__ __launch_bounds__(256) void k_deposit_smooth(Batch b, int bias) {
         for (int c = sc0; c <= sc1 && sr0 <= sr1; c++) {
             const int e0 = S.cstart[c * S.rows + sr0], e1 = S.cstart[c * S.rows + sr1 + 1];
             for (int k = e0 + (int)threadIdx.x; k < e1; k += 256) {
-                const float x = S.xs[k], y = S.ys[k];
-                const float u = x - 0.5f, v = y - 0.5f;
-                const float cu = floorf(u), cv = floorf(v);
-                const int fu = (int)rintf((u - cu) * 1048576.0f), fv = (int)rintf((v - cv) * 1048576.0f);
+                // CIC weights (fields.rs:27-40) from the cell + fraction position: the
+                // lower corner cell is floor(x - 1/2), with 22-bit fixed-point weights.
+                const int cell = S.scell[k];
+                const int ce = cell / S.rows, re = cell - (cell / S.rows) * S.rows;
+                const float x = S.sfx[k], y = S.sfy[k];
+                const int c0 = x >= 0.5f ? ce : ce - 1, r0 = y >= 0.5f ? re : re - 1;
+                const float ux = x >= 0.5f ? x - 0.5f : x + 0.5f, uy = y >= 0.5f ? y - 0.5f : y + 0.5f;
+                const int fu = (int)rintf(ux * 4194304.0f), fv = (int)rintf(uy * 4194304.0f);
                 const int wx[2] = {(1 << W_BITS) - fu, fu}, wy[2] = {(1 << W_BITS) - fv, fv};
-                const int ca = clampi((int)cu, 0, S.cols - 1), cb = clampi((int)cu + 1, 0, S.cols - 1);
-                const int ra = clampi((int)cv, 0, S.rows - 1), rb = clampi((int)cv + 1, 0, S.rows - 1);
-                const float vx = S.vxs[k], vy = S.vys[k];
+                const int ca = clampi(c0, 0, S.cols - 1), cb = clampi(c0 + 1, 0, S.cols - 1);
+                const int ra = clampi(r0, 0, S.rows - 1), rb = clampi(r0 + 1, 0, S.rows - 1);
+                const float vx = S.svx[k], vy = S.svy[k];
                 for (int a = 0; a < 2; a++) {
                     const int li = (a ? cb : ca) - wc0;
                     if (li < 0 || li >= W || wx[a] == 0) continue;
@@ -543,7 +556,7 @@ __global__ __launch_bounds__(256) void k_deposit_smooth(Batch b, int bias) {
             const int cell = c * S.rows + r;
             long long ix = 0, iy = 0;
             for (int k = S.cstart[cell]; k < S.cstart[cell + 1]; k++) {
-                const float vx = S.vxs[k], vy = S.vys[k];
+                const float vx = S.svx[k], vy = S.svy[k];
                 ix += __float2ll_rn(__fmul_rn(vx, (float)J_SCALE));
                 iy += __float2ll_rn(__fmul_rn(vy, (float)J_SCALE));
                 vmax = fmaxf(vmax, sqrtf(vx * vx + vy * vy));

@@ -59,14 +59,16 @@ void layout(Bump &a, const Counts &c, DevScene &d, DevScene **d_scene, int32_t *
     d.damp_bz_x = a.take<float>(nx + 1);
     d.damp_bz_y = a.take<float>(ny + 1);
     d.inv_lap = a.take<float>(nodes);
-    d.x = a.take<float>(n);
-    d.y = a.take<float>(n);
+    d.cxy = a.take<int32_t>(n);
+    d.fx = a.take<float>(n);
+    d.fy = a.take<float>(n);
     d.vx = a.take<float>(n);
     d.vy = a.take<float>(n);
-    d.xs = a.take<float>(n);
-    d.ys = a.take<float>(n);
-    d.vxs = a.take<float>(n);
-    d.vys = a.take<float>(n);
+    d.scell = a.take<int32_t>(n);
+    d.sfx = a.take<float>(n);
+    d.sfy = a.take<float>(n);
+    d.svx = a.take<float>(n);
+    d.svy = a.take<float>(n);
     d.order = a.take<int32_t>(n);
     d.ecell = a.take<int32_t>(n);
     d.erank = a.take<int32_t>(n);
@@ -239,8 +241,9 @@ int make_resident(EmbDevice *dev, const EmbSettings &s, HostScene &&hs_in, hipSt
     EMB_HIP(put(d.damp_bz_x, hs.damp[4], st));
     EMB_HIP(put(d.damp_bz_y, hs.damp[5], st));
     EMB_HIP(put(d.inv_lap, hs.inv_lap, st));
-    EMB_HIP(put(d.x, hs.x, st));
-    EMB_HIP(put(d.y, hs.y, st));
+    EMB_HIP(put(d.cxy, hs.cxy, st));
+    EMB_HIP(put(d.fx, hs.fx, st));
+    EMB_HIP(put(d.fy, hs.fy, st));
     EMB_HIP(put(d.vx, hs.vx, st));
     EMB_HIP(put(d.vy, hs.vy, st));
     EMB_HIP(put(d.static_rho, hs.static_rho, st));
@@ -275,7 +278,8 @@ int make_resident(EmbDevice *dev, const EmbSettings &s, HostScene &&hs_in, hipSt
     // The copies above read pageable host vectors: let them finish before the
     // host copies of the electrons are dropped.
     EMB_HIP(hipStreamSynchronize(st));
-    hs.x = hs.y = hs.vx = hs.vy = {};
+    hs.cxy = {};
+    hs.fx = hs.fy = hs.vx = hs.vy = {};
     return EMB_OK;
 }
 
@@ -311,6 +315,23 @@ int enqueue_tick_tables(Resident &r, int count, hipStream_t st) {
         EMB_HIP(hipMemcpyAsync(const_cast<float *>(r.ds.phase_tab), r.pin_tab, r.tab_floats * sizeof(float),
                                hipMemcpyHostToDevice, st));
         EMB_HIP(hipEventRecord(r.tab_event, st));
+    }
+    return EMB_OK;
+}
+
+// Electron positions as FP64 cell + fraction (exact).
+int read_positions(const DevScene &d, double *pos) {
+    const size_t n = (size_t)d.n;
+    std::vector<int32_t> c(n);
+    std::vector<float> fx(n), fy(n);
+    if (n) {
+        EMB_HIP(hipMemcpy(c.data(), d.cxy, n * 4, hipMemcpyDeviceToHost));
+        EMB_HIP(hipMemcpy(fx.data(), d.fx, n * 4, hipMemcpyDeviceToHost));
+        EMB_HIP(hipMemcpy(fy.data(), d.fy, n * 4, hipMemcpyDeviceToHost));
+    }
+    for (size_t i = 0; i < n; i++) {
+        pos[2 * i] = (double)cell_x(c[i]) + (double)fx[i];
+        pos[2 * i + 1] = (double)cell_y(c[i]) + (double)fy[i];
     }
     return EMB_OK;
 }
@@ -527,8 +548,7 @@ int emb_sim_read_state(EmbSim *sim, double *pos, double *vel, double *gains) {
         const DevScene &d = sim->r.ds;
         const size_t n = (size_t)d.n;
         int rc = EMB_OK;
-        if (pos && rc == EMB_OK) rc = copy_back(sim, d.x, n, pos, 2, 0);
-        if (pos && rc == EMB_OK) rc = copy_back(sim, d.y, n, pos, 2, 1);
+        if (pos && rc == EMB_OK) rc = read_positions(d, pos);
         if (vel && rc == EMB_OK) rc = copy_back(sim, d.vx, n, vel, 2, 0);
         if (vel && rc == EMB_OK) rc = copy_back(sim, d.vy, n, vel, 2, 1);
         if (rc != EMB_OK) return rc;

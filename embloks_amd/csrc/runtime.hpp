@@ -72,6 +72,9 @@ size_t scene_bytes(const EmbSettings &s, int cols, int rows, int64_t n, int n_st
 int make_resident(EmbDevice *dev, const EmbSettings &s, HostScene &&hs, hipStream_t stream, Resident &out);
 void free_resident(Resident &r);
 
+// Electron positions (n x 2, FP64) of a resident scene; synchronous.
+int read_positions(const DevScene &d, double *pos);
+
 // Blocks per kernel class for one scene.
 void scene_blocks(const DevScene &d, int out[KC_COUNT]);
 

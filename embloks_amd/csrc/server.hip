@@ -33,7 +33,8 @@ struct Request {
     double time0 = 0.0;
     ReqState state = QUEUED;
     Resident r;
-    std::vector<float> pos, vel, ex, ey, bz;   // kept results
+    std::vector<double> pos;                   // kept results
+    std::vector<float> vel, ex, ey, bz;
     bool has_result = false;
 };
 
@@ -133,10 +134,11 @@ int retire(EmbServer *sv, Request *q) {
         q->ex.resize((nx + 1) * ny);
         q->ey.resize(nx * (ny + 1));
         q->bz.resize((nx + 1) * (ny + 1));
+        std::vector<double> p(2 * n);
+        int rc = read_positions(d, p.data());
+        if (rc != EMB_OK) return rc;
+        for (size_t i = 0; i < 2 * n; i++) q->pos[i] = p[i];
         std::vector<float> a(n), b(n);
-        EMB_HIP(hipMemcpy(a.data(), d.x, n * 4, hipMemcpyDeviceToHost));
-        EMB_HIP(hipMemcpy(b.data(), d.y, n * 4, hipMemcpyDeviceToHost));
-        for (size_t i = 0; i < n; i++) q->pos[2 * i] = a[i], q->pos[2 * i + 1] = b[i];
         EMB_HIP(hipMemcpy(a.data(), d.vx, n * 4, hipMemcpyDeviceToHost));
         EMB_HIP(hipMemcpy(b.data(), d.vy, n * 4, hipMemcpyDeviceToHost));
         for (size_t i = 0; i < n; i++) q->vel[2 * i] = a[i], q->vel[2 * i + 1] = b[i];
@@ -326,7 +328,7 @@ int emb_server_result(EmbServer *sv, int64_t id, double *pos, double *vel, doubl
     Request *q = it->second.get();
     if (q->state != DONE) return fail(EMB_E_STATE, "request not finished");
     if (!q->has_result) return fail(EMB_E_STATE, "result not kept (keep_results = 0) or already read");
-    auto out = [](double *dst, const std::vector<float> &src) {
+    auto out = [](double *dst, const auto &src) {
         if (dst)
             for (size_t i = 0; i < src.size(); i++) dst[i] = src[i];
     };
@@ -335,7 +337,8 @@ int emb_server_result(EmbServer *sv, int64_t id, double *pos, double *vel, doubl
     out(ex, q->ex);
     out(ey, q->ey);
     out(bz, q->bz);
-    q->pos = q->vel = q->ex = q->ey = q->bz = {};
+    q->pos = {};
+    q->vel = q->ex = q->ey = q->bz = {};
     q->has_result = false;
     return EMB_OK;
 }

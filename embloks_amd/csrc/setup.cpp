@@ -276,7 +276,7 @@ std::string prepare_scene(const EmbSettings &s, const EmbScene &sc, HostScene &o
     if (s.max_substeps < 1 || s.dt <= 0.0 || s.max_step <= 0.0) return "bad time stepping settings";
     const int cols = sc.cols, rows = sc.rows;
     const long long cells = (long long)cols * rows;
-    if (cells > (1LL << 26)) return "grid too large";
+    if (cells > (1LL << 26) || cols > 32767 || rows > 32767) return "grid too large";
     o.cols = cols;
     o.rows = rows;
     o.margin = s.absorbing_margin;
@@ -530,16 +530,24 @@ std::string prepare_scene(const EmbSettings &s, const EmbScene &sc, HostScene &o
             o.gains[(size_t)k] = (float)sc.gains[k];
             o.has_gain[(size_t)k] = 1;
         }
-    o.x.resize((size_t)o.n);
-    o.y.resize((size_t)o.n);
+    o.cxy.resize((size_t)o.n);
+    o.fx.resize((size_t)o.n);
+    o.fy.resize((size_t)o.n);
     o.vx.resize((size_t)o.n);
     o.vy.resize((size_t)o.n);
     for (int64_t i = 0; i < o.n; i++) {
-        o.x[(size_t)i] = (float)sc.pos[2 * i];
-        o.y[(size_t)i] = (float)sc.pos[2 * i + 1];
+        const double x = sc.pos[2 * i], y = sc.pos[2 * i + 1];
+        if (!std::isfinite(x) || !std::isfinite(y)) return "electron position not finite";
+        const double cx = std::floor(x), cy = std::floor(y);
+        if (cx < 0 || cy < 0 || cx >= cols || cy >= rows) return "electron outside the grid";
+        float fx = (float)(x - cx), fy = (float)(y - cy);
+        if (fx >= 1.0f) fx = 0.99999994f;   // x just below a face: stay in its cell
+        if (fy >= 1.0f) fy = 0.99999994f;
+        o.cxy[(size_t)i] = pack_cell((int)cx, (int)cy);
+        o.fx[(size_t)i] = fx;
+        o.fy[(size_t)i] = fy;
         o.vx[(size_t)i] = (float)sc.vel[2 * i];
         o.vy[(size_t)i] = (float)sc.vel[2 * i + 1];
-        if (!std::isfinite(o.x[(size_t)i]) || !std::isfinite(o.y[(size_t)i])) return "electron position not finite";
     }
     return "";
 }

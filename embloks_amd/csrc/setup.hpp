@@ -64,7 +64,8 @@ struct HostScene {
     std::vector<float> inv_lap;
     std::vector<float> gains;
     std::vector<int32_t> has_gain;
-    std::vector<float> x, y, vx, vy;
+    std::vector<int32_t> cxy;                 // electrons: cell + fraction, velocity
+    std::vector<float> fx, fy, vx, vy;
 };
 
 // Returns an empty string on success, else what is wrong with the scene.

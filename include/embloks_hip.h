@@ -137,7 +137,7 @@ int emb_sim_read_potential(EmbSim *sim, double *potential);
 /* read_current (sim.rs:843-849): physics::current_density, (cols, rows, 2). */
 int emb_sim_read_current(EmbSim *sim, double *current);
 /* The charge deposit as the int64 fixed-point grid the kernels hold: electron
- * charge in units of 2^-40 of one electron (cols * rows, unsmoothed, from the
+ * charge in units of 2^-44 of one electron (cols * rows, unsmoothed, from the
  * electrons' current positions). No counterpart in the reference, which
  * deposits in floating point (fields.rs:43-63); exported so that the deposit
  * can be checked bit for bit. */

@@ -109,24 +109,24 @@ def test_ticks_follow_the_oracle(index):
         assert abs((g if g is not None else 1.0) - (c if c is not None else 1.0)) < 1e-3
 
 
-def fixed_point_deposit(pos32, cols, rows):
-    """The integer CIC deposit (fields.rs:27-51 with 20-bit weights), restated in
-    numpy from the same FP32 positions: electron charge in units of 2^-40."""
-    x = pos32[:, 0].astype(np.float32)
-    y = pos32[:, 1].astype(np.float32)
+def fixed_point_deposit(pos, cols, rows):
+    """The integer CIC deposit (fields.rs:27-51 with 22-bit weights per axis),
+    restated in numpy from the same cell + FP32-fraction positions: electron
+    charge in units of 2^-44."""
+    cell = np.floor(pos).astype(np.int64)
+    frac = (pos - cell).astype(np.float32)   # exact: the backend stores these fractions
+    half = np.float32(0.5)
+    low = frac >= half
+    c0 = np.where(low, cell, cell - 1)
+    u = np.where(low, frac - half, frac + half).astype(np.float32)
+    f = np.rint(u * np.float32(4194304.0)).astype(np.int64)
     out = np.zeros(cols * rows, dtype=np.int64)
-    u = (x - np.float32(0.5)).astype(np.float32)
-    v = (y - np.float32(0.5)).astype(np.float32)
-    cu, cv = np.floor(u), np.floor(v)
-    fu = np.rint((u - cu) * np.float32(1048576.0)).astype(np.int64)
-    fv = np.rint((v - cv) * np.float32(1048576.0)).astype(np.int64)
-    c0, r0 = cu.astype(np.int64), cv.astype(np.int64)
     for a in (0, 1):
-        wx = fu if a else (1 << 20) - fu
-        cc = np.clip(c0 + a, 0, cols - 1)
+        wx = f[:, 0] if a else (1 << 22) - f[:, 0]
+        cc = np.clip(c0[:, 0] + a, 0, cols - 1)
         for b in (0, 1):
-            wy = fv if b else (1 << 20) - fv
-            rr = np.clip(r0 + b, 0, rows - 1)
+            wy = f[:, 1] if b else (1 << 22) - f[:, 1]
+            rr = np.clip(c0[:, 1] + b, 0, rows - 1)
             np.add.at(out, cc * rows + rr, -(wx * wy))
     return out.reshape(cols, rows)
 
@@ -141,7 +141,7 @@ def test_deposit_is_bit_exact_and_conserves_charge(index):
     got = sim.read_charge_fixed()
     want = fixed_point_deposit(pos, world.cols, world.rows)
     np.testing.assert_array_equal(got, want)
-    assert int(got.sum()) == -world.n_electrons * (1 << 40)
+    assert int(got.sum()) == -world.n_electrons * (1 << 44)
     residual, charge, energy = sim.diagnostics()
     q0 = world.settings["cell_charge"] / world.settings["carriers_per_cell"]
     assert charge == -world.n_electrons * q0
