@@ -14,9 +14,15 @@
 
 #include <stdint.h>
 
-#ifndef __HIPCC__
+#ifdef __HIPCC__
+#include <hip/hip_runtime.h>
+typedef float2 lerp2;
+#else
 #define __host__
 #define __device__
+struct lerp2 {
+    float x, y;
+};
 #endif
 
 namespace emb {
@@ -41,12 +47,15 @@ __host__ __device__ inline int cell_y(int32_t p) { return (int)(p >> 16); }
 constexpr int MAX_STACKS_PER_BLOCK = 64;
 constexpr int SCAN_TILE = 1024;   // cells per block of the cell-list scan
 constexpr int TICK_TABLE = 64;    // ticks of host-computed AC phases / EMF targets
+constexpr int PUSH_TILE = 8;      // cells per side of a push tile
+constexpr int ION_SUM = 33;       // samples per axis of the summed ion table
 
 // Cell flags (CellInfo.flags).
 constexpr uint32_t F_CONDUCTIVE = 1u;
 constexpr uint32_t F_WIRE = 2u;
 constexpr uint32_t F_BATTERY = 4u;
 constexpr int F_ORI_SHIFT = 3;      // 2 bits of orientation
+constexpr uint32_t F_ION_UNIFORM = 32u;   // the (2 reach + 1)^2 cells around share one ion density
 constexpr int F_BAT_SHIFT = 8;      // battery-cell index + 1 in the upper bits
 
 struct CellInfo {
@@ -113,6 +122,7 @@ struct DevScene {
     int32_t pad1;
     // --- fields (FP32)
     float *ex, *ey, *bz;
+    float *ex2, *ey2, *bz2;   // second buffers of the FDTD step (written, then read back by k_correct)
     float *rho, *jx, *jy;     // smoothed sources on nodes
     float *w1, *w2, *phi;     // Poisson work: 2 channels each, channel stride nodes
     const float *damp_ex_x, *damp_ex_y, *damp_ey_x, *damp_ey_y, *damp_bz_x, *damp_bz_y;
@@ -139,7 +149,11 @@ struct DevScene {
     const float *density;     // per cell (ion density)
     const float *ion_pot;     // per cell: close-range potential of all ions
     const float *g_table;     // PAIR_SAMPLES + 2
+    const lerp2 *g_lerp;     // PAIR_SAMPLES + 1: (g[i], g[i+1] - g[i])
     const float *square;      // square_n^2 * 3
+    const float *ion_sum;     // ION_SUM^2 * 2: the square table's force channels summed over
+                              // the (2 reach + 1)^2 cell offsets (uniform-density neighbourhoods)
+    int32_t ion_a0;           // square-table index of the home cell's low corner offset
     const BatteryCell *bats;
     const StackInfo *stacks;
     const int32_t *stack_cells;
@@ -163,6 +177,7 @@ enum KernelClass {
     KC_NODES,           // 256 corners of (nx+1) x (ny+1) per block
     KC_STACKS,          // one block per battery stack (at least one per scene)
     KC_SCENE,           // one block per scene
+    KC_PUSH,            // PUSH_TILE x PUSH_TILE cell tiles
     KC_COUNT
 };
 

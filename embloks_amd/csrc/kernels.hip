@@ -141,18 +141,58 @@ __device__ __forceinline__ void canon_cell(const DevScene &S, int c, int s0, int
     }
 }
 
+// 256 cells per block: their slots are loaded into LDS, each cell's slots are
+// ordered by electron index there, and the ordered slots and the snapshot are
+// written back with coalesced stores. Blocks whose cells hold more than
+// CANON_CAP electrons sort in global memory instead.
+constexpr int CANON_CAP = 6144;
+
 __global__ __launch_bounds__(256) void k_canon(Batch b) {
+    __shared__ int lorder[CANON_CAP];
     for (int blk = blockIdx.x; blk < b.total[KC_CELLS]; blk += gridDim.x) {
         const int s = scene_of(b, KC_CELLS, blk);
         const DevScene &S = b.scenes[s];
-        const int c = block_in_scene(b, KC_CELLS, s, blk) * 256 + (int)threadIdx.x;
         const int cells = S.cols * S.rows;
-        if (c < cells) {
-            const int s0 = scan_start(S, c), s1 = scan_start(S, c + 1);
+        const int c0 = block_in_scene(b, KC_CELLS, s, blk) * 256;
+        const int c1 = min(c0 + 256, cells);
+        const int S0 = scan_start(S, c0), S1 = scan_start(S, c1);
+        const int c = c0 + (int)threadIdx.x;
+        const bool mine = c < c1;
+        const int s0 = mine ? scan_start(S, c) : 0, s1 = mine ? scan_start(S, c + 1) : 0;
+        if (mine) {
             S.cstart[c] = s0;
             if (c == cells - 1) S.cstart[cells] = S.n;
-            canon_cell(S, c, s0, s1);
         }
+        if (S1 - S0 > CANON_CAP) {
+            if (mine) canon_cell(S, c, s0, s1);
+            __syncthreads();
+            continue;
+        }
+        for (int q = S0 + (int)threadIdx.x; q < S1; q += 256) lorder[q - S0] = S.order[q];
+        __syncthreads();
+        if (mine) {
+            int *o = lorder - S0;
+            for (int a = s0 + 1; a < s1; a++) {
+                const int key = o[a];
+                int p = a - 1;
+                while (p >= s0 && o[p] > key) {
+                    o[p + 1] = o[p];
+                    p--;
+                }
+                o[p + 1] = key;
+            }
+        }
+        __syncthreads();
+        for (int q = S0 + (int)threadIdx.x; q < S1; q += 256) {
+            const int e = lorder[q - S0];
+            S.order[q] = e;
+            S.scell[q] = S.ecell[e];
+            S.sfx[q] = S.fx[e];
+            S.sfy[q] = S.fy[e];
+            S.svx[q] = S.vx[e];
+            S.svy[q] = S.vy[e];
+        }
+        __syncthreads();
     }
 }
 
@@ -186,7 +226,8 @@ __device__ __forceinline__ float square_floor(const DevScene &S, int k, float u,
 // close-range potential of the ions and electrons near it (Simulation::potential,
 // simulation.rs:48-55; CloseRange::potential, close_range.rs:415-464).
 
-__global__ __launch_bounds__(64) void k_trim(Batch b) {
+__global__ __launch_bounds__(256) void k_trim(Batch b) {
+    __shared__ float part[2][4];
     for (int blk = blockIdx.x; blk < b.total[KC_STACKS]; blk += gridDim.x) {
         const int s = scene_of(b, KC_STACKS, blk);
         const DevScene &S = b.scenes[s];
@@ -207,7 +248,7 @@ __global__ __launch_bounds__(64) void k_trim(Batch b) {
         const int per = 1 + (S.close_on ? side * side : 0);
         const int lists = I.low_n + I.high_n;
         float acc_lo = 0.0f, acc_hi = 0.0f;
-        for (int it = (int)threadIdx.x; it < lists * per; it += 64) {
+        for (int it = (int)threadIdx.x; it < lists * per; it += 256) {
             const int li = it / per, sub = it - li * per;
             const bool high = li >= I.low_n;
             const int cell = S.stack_cells[high ? I.high_off + (li - I.low_n) : I.low_off + li];
@@ -237,6 +278,14 @@ __global__ __launch_bounds__(64) void k_trim(Batch b) {
             acc_lo += __shfl_xor(acc_lo, o, 64);
             acc_hi += __shfl_xor(acc_hi, o, 64);
         }
+        if ((threadIdx.x & 63) == 0) {
+            part[0][threadIdx.x >> 6] = acc_lo;
+            part[1][threadIdx.x >> 6] = acc_hi;
+        }
+        __syncthreads();
+        acc_lo = (part[0][0] + part[0][1]) + (part[0][2] + part[0][3]);
+        acc_hi = (part[1][0] + part[1][1]) + (part[1][2] + part[1][3]);
+        __syncthreads();
         if (threadIdx.x == 0) {
             const float v_low = acc_lo / (float)I.low_n, v_high = acc_hi / (float)I.high_n;
             const int cells = S.cols * S.rows;
@@ -310,7 +359,36 @@ __device__ __forceinline__ int substeps_of(const DevScene &S, uint32_t t) {
     return sub < 1 ? 1 : sub;
 }
 
-__device__ void push_one(const DevScene &S, int slot, int k, float h, uint32_t t) {
+// Neighbour access of the close-range loop: the whole sorted snapshot in global
+// memory, or a window of it staged in LDS.
+struct GlobalNeighbours {
+    const DevScene &S;
+    __device__ void range(int cc, int rr, int &b, int &e) const {
+        const int cell = cc * S.rows + rr;
+        b = S.cstart[cell];
+        e = S.cstart[cell + 1];
+    }
+    __device__ float x(int j) const { return S.sfx[j]; }
+    __device__ float y(int j) const { return S.sfy[j]; }
+};
+
+struct LdsNeighbours {
+    const float *lx, *ly;        // LDS
+    const int *lb, *le;          // LDS: per window cell, range in lx/ly
+    int wc0, wr0, W;
+    __device__ void range(int cc, int rr, int &b, int &e) const {
+        const int w = (cc - wc0) * W + (rr - wr0);
+        b = lb[w];
+        e = le[w];
+    }
+    __device__ float x(int j) const { return lx[j]; }
+    __device__ float y(int j) const { return ly[j]; }
+};
+
+// One electron's substep (physics.rs:139-158): `slot` is its place in the sorted
+// snapshot.
+template <class NB>
+__device__ void push_core(const DevScene &S, const NB &nb, int slot, int k, float h, uint32_t t) {
     const int id = S.order[slot];
     const int home = S.scell[slot];
     const int hc = home / S.rows, hr = home - (home / S.rows) * S.rows;
@@ -346,34 +424,61 @@ __device__ void push_one(const DevScene &S, int slot, int k, float h, uint32_t t
         const int R = S.reach;
         const int c_lo = max(hc - R, 0), c_hi = min(hc + R, S.cols - 1);
         const int r_lo = max(hr - R, 0), r_hi = min(hr + R, S.rows - 1);
+        const bool uniform = (info.flags & F_ION_UNIFORM) != 0u;
+        if (uniform) {
+            // Every neighbour cell carries the same ion density and all share the
+            // same fractional table offset, so the sum of their bilinear lookups
+            // is one lookup in the table summed over the offsets.
+            const float u = (px - 0.5f + S.half) * S.square_samples, w = (py - 0.5f + S.half) * S.square_samples;
+            const int a = (int)u, bb = (int)w;
+            const float fu = u - (float)a, fv = w - (float)bb;
+            const float *tab = S.ion_sum + ((size_t)(a - S.ion_a0) * ION_SUM + (bb - S.ion_a0)) * 2;
+            const float coef = S.k_ion_q * S.density[home];
+            const float w00 = (1.0f - fu) * (1.0f - fv), w10 = fu * (1.0f - fv), w01 = (1.0f - fu) * fv, w11 = fu * fv;
+            fx += coef * (tab[0] * w00 + tab[ION_SUM * 2] * w10 + tab[2] * w01 + tab[ION_SUM * 2 + 2] * w11);
+            fy += coef * (tab[1] * w00 + tab[ION_SUM * 2 + 1] * w10 + tab[3] * w01 + tab[ION_SUM * 2 + 3] * w11);
+        }
+        // Pairs: sum g(d^2) d over the neighbours within the cutoff, times k q q at
+        // the end. The electron itself contributes g(0) * 0 = 0. Neighbour cells
+        // entirely beyond the cutoff from this electron are skipped.
+        float sx = 0.0f, sy = 0.0f;
+        const float cut2 = S.cutoff2;
         for (int cc = c_lo; cc <= c_hi; cc++) {
             const float ox = (float)(hc - cc);
+            const float gapx = ox > 0.0f ? ox + px - 1.0f : (ox < 0.0f ? -ox - px : 0.0f);
             for (int rr = r_lo; rr <= r_hi; rr++) {
                 const float oy = (float)(hr - rr);
-                const int cell = cc * S.rows + rr;
-                const float dens = S.density[cell];
-                if (dens > 0.0f) {
-                    const float u = (ox + (px - 0.5f) + S.half) * S.square_samples;
-                    const float w = (oy + (py - 0.5f) + S.half) * S.square_samples;
-                    const float coef = S.k_ion_q * dens;
-                    fx += coef * square_trunc(S, 1, u, w);
-                    fy += coef * square_trunc(S, 2, u, w);
+                if (!uniform) {
+                    const float dens = S.density[cc * S.rows + rr];
+                    if (dens > 0.0f) {
+                        const float u = (ox + (px - 0.5f) + S.half) * S.square_samples;
+                        const float w = (oy + (py - 0.5f) + S.half) * S.square_samples;
+                        const float coef = S.k_ion_q * dens;
+                        fx += coef * square_trunc(S, 1, u, w);
+                        fy += coef * square_trunc(S, 2, u, w);
+                    }
                 }
-                const int e = S.cstart[cell + 1];
-                for (int j = S.cstart[cell]; j < e; j++) {
-                    if (j == slot) continue;
-                    const float dx = ox + (px - S.sfx[j]), dy = oy + (py - S.sfy[j]);
+                const float gapy = oy > 0.0f ? oy + py - 1.0f : (oy < 0.0f ? -oy - py : 0.0f);
+                if (gapx * gapx + gapy * gapy >= cut2 * 1.0001f) continue;
+                const float bx = ox + px, by = oy + py;
+                int j, e;
+                nb.range(cc, rr, j, e);
+                for (; j < e; j++) {
+                    const float dx = bx - nb.x(j), dy = by - nb.y(j);
                     const float d2 = dx * dx + dy * dy;
-                    if (d2 >= S.cutoff2) continue;
-                    const float tt = d2 * S.g_scale;
-                    const int si = (int)tt;
-                    const float g0 = S.g_table[si];
-                    const float f = S.k_pair * (g0 + (S.g_table[si + 1] - g0) * (tt - (float)si));
-                    fx += f * dx;
-                    fy += f * dy;
+                    if (d2 < cut2) {
+                        const float tt = d2 * S.g_scale;
+                        const int si = (int)tt;
+                        const float2 g = S.g_lerp[si];
+                        const float l = g.x + g.y * (tt - (float)si);
+                        sx += l * dx;
+                        sy += l * dy;
+                    }
                 }
             }
         }
+        fx += S.k_pair * sx;
+        fy += S.k_pair * sy;
         ax += fx * S.inv_mass;
         ay += fy * S.inv_mass;
     }
@@ -397,6 +502,98 @@ __device__ void push_one(const DevScene &S, int slot, int k, float h, uint32_t t
     S.fy[id] = ty;
     S.vx[id] = nvx;
     S.vy[id] = nvy;
+}
+
+__device__ void push_one(const DevScene &S, int slot, int k, float h, uint32_t t) {
+    push_core(S, GlobalNeighbours{S}, slot, k, h, t);
+}
+
+// Substep k for every electron, per PUSH_TILE x PUSH_TILE tile of cells: the
+// positions of the tile's neighbourhood (reach cells around it) are staged in
+// LDS, so the ~(2 reach + 1)^2 x carriers pair candidates of each electron are
+// read from LDS. A tile whose window holds more than PUSH_STAGE electrons reads
+// them from global memory instead.
+constexpr int PUSH_WIN = PUSH_TILE + 2 * 4;   // window side for reach <= 4
+constexpr int PUSH_STAGE = 3584;
+
+__global__ __launch_bounds__(256) void k_push_tiled(Batch b, int k) {
+    __shared__ float lx[PUSH_STAGE], ly[PUSH_STAGE];
+    __shared__ int lb[PUSH_WIN * PUSH_WIN], le[PUSH_WIN * PUSH_WIN];
+    __shared__ int col_base[PUSH_WIN + 1], col_g0[PUSH_WIN];   // LDS base and first global slot per column
+    __shared__ int own_pref[PUSH_TILE + 1];                     // prefix of the tile's electrons per column
+    for (int blk = blockIdx.x; blk < b.total[KC_PUSH]; blk += gridDim.x) {
+        const int s = scene_of(b, KC_PUSH, blk);
+        const DevScene &S = b.scenes[s];
+        const uint32_t t = S.ctr->tick;
+        const int sub = substeps_of(S, t);
+        if (k >= sub) continue;
+        const float h = S.dt / (float)sub;
+        if (k == 0 && threadIdx.x == 0 && block_in_scene(b, KC_PUSH, s, blk) == 0) S.ctr->substeps = sub;
+        const int local = block_in_scene(b, KC_PUSH, s, blk);
+        const int tiles_r = (S.rows + PUSH_TILE - 1) / PUSH_TILE;
+        const int tc0 = (local / tiles_r) * PUSH_TILE, tr0 = (local % tiles_r) * PUSH_TILE;
+        const int tc1 = min(tc0 + PUSH_TILE, S.cols), tr1 = min(tr0 + PUSH_TILE, S.rows);
+        const int R = S.close_on ? S.reach : 0;
+        const int W = PUSH_TILE + 2 * R;
+        const int wc0 = tc0 - R, wr0 = tr0 - R;
+        const int rlo = max(wr0, 0), rhi = min(wr0 + W, S.rows) - 1;
+        // Column ranges of the window (each column's cells are contiguous in the sorted order).
+        if (threadIdx.x == 0) {
+            int base = 0;
+            for (int q = 0; q < W; q++) {
+                const int cc = wc0 + q;
+                col_base[q] = base;
+                col_g0[q] = 0;
+                if (cc >= 0 && cc < S.cols && rlo <= rhi) {
+                    const int g0 = S.cstart[cc * S.rows + rlo], g1 = S.cstart[cc * S.rows + rhi + 1];
+                    col_g0[q] = g0;
+                    base += g1 - g0;
+                }
+            }
+            col_base[W] = base;
+            int own = 0;
+            for (int q = 0; q < PUSH_TILE; q++) {
+                own_pref[q] = own;
+                const int cc = tc0 + q;
+                if (cc < tc1) own += S.cstart[cc * S.rows + tr1 - 1 + 1] - S.cstart[cc * S.rows + tr0];
+            }
+            own_pref[PUSH_TILE] = own;
+        }
+        __syncthreads();
+        const bool staged = col_base[W] <= PUSH_STAGE;
+        if (staged) {
+            for (int q = threadIdx.x; q < W * W; q += 256) {
+                const int ci = q / W, ri = q - (q / W) * W;
+                const int cc = wc0 + ci, rr = wr0 + ri;
+                if (cc < 0 || cc >= S.cols || rr < 0 || rr >= S.rows) {
+                    lb[q] = 0;
+                    le[q] = 0;
+                } else {
+                    const int cell = cc * S.rows + rr;
+                    lb[q] = col_base[ci] + (S.cstart[cell] - col_g0[ci]);
+                    le[q] = col_base[ci] + (S.cstart[cell + 1] - col_g0[ci]);
+                }
+            }
+            for (int q = 0; q < W; q++) {
+                const int n = col_base[q + 1] - col_base[q];
+                for (int j = threadIdx.x; j < n; j += 256) {
+                    lx[col_base[q] + j] = S.sfx[col_g0[q] + j];
+                    ly[col_base[q] + j] = S.sfy[col_g0[q] + j];
+                }
+            }
+        }
+        __syncthreads();
+        const int n_own = own_pref[PUSH_TILE];
+        for (int e = threadIdx.x; e < n_own; e += 256) {
+            int q = 0;
+            while (own_pref[q + 1] <= e) q++;
+            const int cc = tc0 + q;
+            const int slot = S.cstart[cc * S.rows + tr0] + (e - own_pref[q]);
+            if (staged) push_core(S, LdsNeighbours{lx, ly, lb, le, wc0, wr0, W}, slot, k, h, t);
+            else push_core(S, GlobalNeighbours{S}, slot, k, h, t);
+        }
+        __syncthreads();
+    }
 }
 
 __global__ __launch_bounds__(256) void k_push(Batch b, int k) {
@@ -619,13 +816,13 @@ __global__ __launch_bounds__(256) void k_deposit_smooth(Batch b, int bias) {
 // FDTD leapfrog with the absorbing margin, fused (maxwell.rs:235-270;
 // fields.wgsl:106-186): per 32 x 64 tile of the corner space, Bz' = Bz - h/2 curl E
 // on the tile plus halo, E' from curl Bz' and the edge currents, Bz'' = Bz' - h/2
-// curl E', then the damping factors. With `rhs` set (the last FDTD substep), the
-// tile also writes the right-hand sides of the Gauss solve: -4 pi k rho and div E
+// curl E', then the damping factors. On the scene's last FDTD substep the tile
+// also writes the right-hand sides of the Gauss solve: -4 pi k rho and div E
 // (maxwell.rs:170-173, 200-204; fields.wgsl:192-203).
 
 constexpr int TI = 32, TJ = 64;
 
-__global__ __launch_bounds__(256) void k_fdtd(Batch b, int rhs) {
+__global__ __launch_bounds__(256) void k_fdtd(Batch b, int f) {
     __shared__ float EX[(TI + 3) * (TJ + 3)];
     __shared__ float EY[(TI + 3) * (TJ + 3)];
     __shared__ float BZ[(TI + 2) * (TJ + 2)];
@@ -635,22 +832,29 @@ __global__ __launch_bounds__(256) void k_fdtd(Batch b, int rhs) {
     for (int blk = blockIdx.x; blk < b.total[KC_FDTD]; blk += gridDim.x) {
         const int s = scene_of(b, KC_FDTD, blk);
         const DevScene &S = b.scenes[s];
+        if (f >= S.fdtd_substeps) continue;
         const int nx = S.nx, ny = S.ny;
         const int local = block_in_scene(b, KC_FDTD, s, blk);
         const int tiles_j = (ny + 1 + TJ - 1) / TJ;
+        // Neighbouring tiles read halos of the fields this one writes: substep f
+        // reads one buffer of the pair and writes the other.
+        const bool odd = (f & 1) != 0;
+        const float *src_ex = odd ? S.ex2 : S.ex, *src_ey = odd ? S.ey2 : S.ey, *src_bz = odd ? S.bz2 : S.bz;
+        float *dst_ex = odd ? S.ex : S.ex2, *dst_ey = odd ? S.ey : S.ey2, *dst_bz = odd ? S.bz : S.bz2;
+        const bool rhs = f == S.fdtd_substeps - 1;
         const int i0 = (local / tiles_j) * TI, j0 = (local % tiles_j) * TJ;
         // Loads: E at [i0-2, i0+TI] x [j0-2, j0+TJ], Bz at [i0-1, i0+TI] x [j0-1, j0+TJ],
         // node J at [i0-2, i0+TI) x [j0-2, j0+TJ).
         for (int q = threadIdx.x; q < (TI + 3) * WE; q += 256) {
             const int a = q / WE, c = q - (q / WE) * WE;
             const int i = i0 - 2 + a, j = j0 - 2 + c;
-            EX[q] = (i >= 0 && i <= nx && j >= 0 && j < ny) ? S.ex[(size_t)i * ny + j] : 0.0f;
-            EY[q] = (i >= 0 && i < nx && j >= 0 && j <= ny) ? S.ey[(size_t)i * (ny + 1) + j] : 0.0f;
+            EX[q] = (i >= 0 && i <= nx && j >= 0 && j < ny) ? src_ex[(size_t)i * ny + j] : 0.0f;
+            EY[q] = (i >= 0 && i < nx && j >= 0 && j <= ny) ? src_ey[(size_t)i * (ny + 1) + j] : 0.0f;
         }
         for (int q = threadIdx.x; q < (TI + 2) * WB; q += 256) {
             const int a = q / WB, c = q - (q / WB) * WB;
             const int i = i0 - 1 + a, j = j0 - 1 + c;
-            BZ[q] = (i >= 0 && i <= nx && j >= 0 && j <= ny) ? S.bz[(size_t)i * (ny + 1) + j] : 0.0f;
+            BZ[q] = (i >= 0 && i <= nx && j >= 0 && j <= ny) ? src_bz[(size_t)i * (ny + 1) + j] : 0.0f;
             const int in = i - 1, jn = j - 1;
             const bool node = in >= 0 && in < nx && jn >= 0 && jn < ny;
             JX[q] = node ? S.jx[(size_t)in * ny + jn] : 0.0f;
@@ -688,7 +892,7 @@ __global__ __launch_bounds__(256) void k_fdtd(Batch b, int rhs) {
             if (i > nx || j > ny) continue;
             const float curl = ((EY[(a + 1) * WE + c + 1] - EY[a * WE + c + 1]) - EX[(a + 1) * WE + c + 1]) + EX[(a + 1) * WE + c];
             const float v = BZ[a * WB + c] + (-S.half_h) * curl;
-            S.bz[(size_t)i * (ny + 1) + j] = v * fminf(S.damp_bz_x[i], S.damp_bz_y[j]);
+            dst_bz[(size_t)i * (ny + 1) + j] = v * fminf(S.damp_bz_x[i], S.damp_bz_y[j]);
         }
         __syncthreads();
         // Damp E' in place over its whole range, then write the owned edges.
@@ -704,8 +908,8 @@ __global__ __launch_bounds__(256) void k_fdtd(Batch b, int rhs) {
             const int a = q / TJ, c = q % TJ;
             const int i = i0 + a, j = j0 + c;
             const int e = (a + 2) * WE + c + 2;
-            if (i <= nx && j < ny) S.ex[(size_t)i * ny + j] = EX[e];
-            if (i < nx && j <= ny) S.ey[(size_t)i * (ny + 1) + j] = EY[e];
+            if (i <= nx && j < ny) dst_ex[(size_t)i * ny + j] = EX[e];
+            if (i < nx && j <= ny) dst_ey[(size_t)i * (ny + 1) + j] = EY[e];
             if (rhs) {
                 // Node (i - 1, j - 1): its four edges lie in [i0-1, i0+TI) x [j0-1, j0+TJ).
                 const int in = i - 1, jn = j - 1;
@@ -751,17 +955,21 @@ __global__ __launch_bounds__(256) void k_correct(Batch b, int tick) {
         const int nx = S.nx, ny = S.ny;
         const size_t nodes = (size_t)nx * ny;
         if (tick && local == 0 && threadIdx.x == 0) S.ctr->tick += 1;
+        // After an odd number of FDTD substeps the fields are in the second buffer.
+        const bool alt = tick && (S.fdtd_substeps & 1);
+        const float *ex = alt ? S.ex2 : S.ex, *ey = alt ? S.ey2 : S.ey;
         if (q >= (nx + 1) * (ny + 1)) continue;
         const int i = q / (ny + 1), j = q - (q / (ny + 1)) * (ny + 1);
         auto p = [&](int a, int c) { return S.phi[nodes + (size_t)a * ny + c] + S.phi[(size_t)a * ny + c]; };
         if (j < ny) {
             const float high = i < nx ? p(i, j) : 0.0f, low = i > 0 ? p(i - 1, j) : 0.0f;
-            S.ex[(size_t)i * ny + j] -= high - low;
+            S.ex[(size_t)i * ny + j] = ex[(size_t)i * ny + j] - (high - low);
         }
         if (i < nx) {
             const float high = j < ny ? p(i, j) : 0.0f, low = j > 0 ? p(i, j - 1) : 0.0f;
-            S.ey[(size_t)i * (ny + 1) + j] -= high - low;
+            S.ey[(size_t)i * (ny + 1) + j] = ey[(size_t)i * (ny + 1) + j] - (high - low);
         }
+        if (alt) S.bz[q] = S.bz2[q];
     }
 }
 
@@ -851,9 +1059,9 @@ hipError_t setup_kernels(const Batch &b, hipStream_t stream) {
     } while (0)
 hipError_t tick_kernels(const Batch &b, int fdtd_substeps, hipStream_t stream, hipEvent_t *marks) {
     EMB_MARK(0);
-    EMB_LAUNCH(k_trim, KC_STACKS, 64, b);
+    EMB_LAUNCH(k_trim, KC_STACKS, 256, b);
     EMB_MARK(1);
-    EMB_LAUNCH(k_push, KC_ELECTRONS, 256, b, 0);
+    EMB_LAUNCH(k_push_tiled, KC_PUSH, 256, b, 0);
     EMB_MARK(2);
     if (b.total[KC_ELECTRONS] > 0) EMB_LAUNCH(k_rest, KC_SCENE, 1024, b);
     EMB_MARK(3);
@@ -867,7 +1075,7 @@ hipError_t tick_kernels(const Batch &b, int fdtd_substeps, hipStream_t stream, h
     EMB_MARK(7);
     EMB_LAUNCH(k_deposit_smooth, KC_SMOOTH, 256, b, 0);
     EMB_MARK(8);
-    for (int f = 0; f < fdtd_substeps; f++) EMB_LAUNCH(k_fdtd, KC_FDTD, 256, b, f == fdtd_substeps - 1 ? 1 : 0);
+    for (int f = 0; f < fdtd_substeps; f++) EMB_LAUNCH(k_fdtd, KC_FDTD, 256, b, f);
     EMB_MARK(9);
     hipError_t e = launch_dst_gemms(b, stream);
     if (e != hipSuccess) return e;

@@ -46,6 +46,9 @@ void layout(Bump &a, const Counts &c, DevScene &d, DevScene **d_scene, int32_t *
     d.ex = a.take<float>((nx + 1) * ny);
     d.ey = a.take<float>(nx * (ny + 1));
     d.bz = a.take<float>((nx + 1) * (ny + 1));
+    d.ex2 = a.take<float>((nx + 1) * ny);
+    d.ey2 = a.take<float>(nx * (ny + 1));
+    d.bz2 = a.take<float>((nx + 1) * (ny + 1));
     d.rho = a.take<float>(nodes);
     d.jx = a.take<float>(nodes);
     d.jy = a.take<float>(nodes);
@@ -127,9 +130,17 @@ int shared_tables(EmbDevice *dev, const EmbSettings &s, int nx, int ny, const fl
             EMB_HIP(hipMalloc(&t.square, sq.size() * sizeof(float)));
             EMB_HIP(hipMemcpy(t.g, g.data(), g.size() * sizeof(float), hipMemcpyHostToDevice));
             EMB_HIP(hipMemcpy(t.square, sq.data(), sq.size() * sizeof(float), hipMemcpyHostToDevice));
+            std::vector<float2> gl(h->g.size() - 1);
+            for (size_t i = 0; i + 1 < h->g.size(); i++) gl[i] = make_float2((float)h->g[i], (float)(h->g[i + 1] - h->g[i]));
+            EMB_HIP(hipMalloc(&t.g_lerp, gl.size() * sizeof(float2)));
+            EMB_HIP(hipMemcpy(t.g_lerp, gl.data(), gl.size() * sizeof(float2), hipMemcpyHostToDevice));
+            std::vector<float> is(h->ion_sum.begin(), h->ion_sum.end());
+            EMB_HIP(hipMalloc(&t.ion_sum, is.size() * sizeof(float)));
+            EMB_HIP(hipMemcpy(t.ion_sum, is.data(), is.size() * sizeof(float), hipMemcpyHostToDevice));
             t.square_n = h->square_n;
             t.half = h->half;
             t.reach = h->reach;
+            t.ion_a0 = h->ion_a0;
         }
         it = dev->tables.emplace(key, t).first;
     }
@@ -164,6 +175,7 @@ void scene_blocks(const DevScene &d, int out[KC_COUNT]) {
     out[KC_NODES] = cdiv((long long)(d.nx + 1) * (d.ny + 1), 256);
     out[KC_STACKS] = std::max(1, d.n_stacks);
     out[KC_SCENE] = 1;
+    out[KC_PUSH] = cdiv(d.cols, PUSH_TILE) * cdiv(d.rows, PUSH_TILE);
 }
 
 int make_resident(EmbDevice *dev, const EmbSettings &s, HostScene &&hs_in, hipStream_t st, Resident &r) {
@@ -208,6 +220,9 @@ int make_resident(EmbDevice *dev, const EmbSettings &s, HostScene &&hs_in, hipSt
     d.square_n = tables.square_n;
     d.g_table = tables.g;
     d.square = tables.square;
+    d.ion_sum = tables.ion_sum;
+    d.g_lerp = tables.g_lerp;
+    d.ion_a0 = tables.ion_a0;
     const double q = hs.q, mass = hs.mass, k = s.coulomb_k;
     d.dt = (float)s.dt;
     d.max_step = (float)s.max_step;
@@ -414,6 +429,8 @@ int emb_device_close(EmbDevice *dev) {
     for (auto &kv : dev->tables) {
         if (kv.second.g) (void)hipFree(kv.second.g);
         if (kv.second.square) (void)hipFree(kv.second.square);
+        if (kv.second.ion_sum) (void)hipFree(kv.second.ion_sum);
+        if (kv.second.g_lerp) (void)hipFree(kv.second.g_lerp);
     }
     delete dev;
     return EMB_OK;

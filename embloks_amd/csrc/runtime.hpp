@@ -31,6 +31,9 @@ struct DeviceTables {
     int square_n = 0;
     double half = 0.0;
     int reach = 0;
+    float *ion_sum = nullptr;
+    int ion_a0 = 0;
+    float2 *g_lerp = nullptr;
 };
 
 }  // namespace emb

@@ -224,6 +224,21 @@ std::shared_ptr<HostTables> build_tables(double softening, int passes, double cu
                 }
             t->ion_stencil[(size_t)i * side + j] = sum;
         }
+    // The force channels summed over every cell offset of the reach window: the
+    // close-range ion force of a uniform-density neighbourhood is one bilinear
+    // lookup in this table (the offsets shift the table index by whole multiples
+    // of SQUARE_SAMPLES, so every cell shares the electron's fractional offset).
+    t->ion_a0 = (int)std::llround((t->half - 0.5) * M);
+    t->ion_sum.assign((size_t)ION_SUM * ION_SUM * 2, 0.0);
+    for (int i = 0; i < ION_SUM; i++)
+        for (int j = 0; j < ION_SUM; j++)
+            for (int ox = -R; ox <= R; ox++)
+                for (int oy = -R; oy <= R; oy++) {
+                    const long long a = t->ion_a0 + i + (long long)M * ox, b = t->ion_a0 + j + (long long)M * oy;
+                    if (a < 0 || b < 0 || a >= n || b >= n) continue;
+                    for (int ch = 0; ch < 2; ch++)
+                        t->ion_sum[((size_t)i * ION_SUM + j) * 2 + ch] += t->square[((size_t)a * n + b) * 3 + 1 + ch];
+                }
     return t;
 }
 
@@ -274,6 +289,7 @@ std::string prepare_scene(const EmbSettings &s, const EmbScene &sc, HostScene &o
         return "absorbing_margin must be at least smoothing_passes";   // maxwell.rs:58
     if (s.carriers_per_cell <= 0 || s.cell_charge <= 0.0) return "carriers_per_cell and cell_charge must be positive";
     if (s.max_substeps < 1 || s.dt <= 0.0 || s.max_step <= 0.0) return "bad time stepping settings";
+    if (s.electron_cutoff > 4.0) return "electron_cutoff above 4 cells is not supported";
     const int cols = sc.cols, rows = sc.rows;
     const long long cells = (long long)cols * rows;
     if (cells > (1LL << 26) || cols > 32767 || rows > 32767) return "grid too large";
@@ -439,6 +455,20 @@ std::string prepare_scene(const EmbSettings &s, const EmbScene &sc, HostScene &o
         o.density[(size_t)k] = (float)sc.density[k];
     }
     if (o.bats.size() >= (1u << (32 - F_BAT_SHIFT)) - 1) return "too many battery cells";
+
+    // Cells whose whole reach window lies on the grid with one ion density.
+    if (s.electron_cutoff > 0.0) {
+        const int R = (int)std::ceil(s.electron_cutoff);
+        for (long long c = R; c + R < cols; c++)
+            for (long long r = R; r + R < rows; r++) {
+                const double d = sc.density[c * rows + r];
+                if (!(d > 0.0)) continue;
+                bool same = true;
+                for (long long dc = -R; dc <= R && same; dc++)
+                    for (long long dr = -R; dr <= R && same; dr++) same = sc.density[(c + dc) * rows + (r + dr)] == d;
+                if (same) o.info[(size_t)(c * rows + r)].flags |= F_ION_UNIFORM;
+            }
+    }
 
     // Static charge: ions spread over their cells plus the fixed charges' CIC
     // (fields.rs:91-112; maxwell.rs:307-310), in units of 2^-40 electron charges.

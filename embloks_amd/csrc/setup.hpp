@@ -32,6 +32,8 @@ struct HostTables {
     double half = 0.0;
     std::vector<double> ion_stencil;      // (2 reach + 1)^2
     int reach = 0;
+    std::vector<double> ion_sum;          // ION_SUM^2 * 2: force channels summed over cell offsets
+    int ion_a0 = 0;
 };
 
 // close_range::tables, cached per parameter set for the process.

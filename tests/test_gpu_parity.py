@@ -86,6 +86,22 @@ def test_one_tick_matches_the_oracle(name, before):
 
 
 @gpu
+def test_fast_light_takes_three_fdtd_substeps():
+    # c = 40: c dt sqrt 2 / 0.9 > 2, so three FDTD substeps per tick (maxwell.rs:376-378).
+    world = W.scene(0, speed_of_light=40.0)
+    cpu = orc.Sim(world)
+    sim = gpusim()(world)
+    assert sim.dims().fdtd_substeps == 3
+    for _ in range(3):
+        cpu.step(1)
+        sim.run_ticks(1)
+    ex, ey, bz = sim.read_yee()
+    cex, cey, cbz = cpu.yee()
+    assert relative_error(ex, cex) < 1e-4 and relative_error(ey, cey) < 1e-4
+    assert relative_error(bz, cbz) < 1e-3
+
+
+@gpu
 @pytest.mark.parametrize("index", [0, 1, 5])
 def test_ticks_follow_the_oracle(index):
     # crates/embloks-gpu/tests/sim.rs:62-90 with its tolerances.
@@ -159,11 +175,12 @@ def test_gauss_law_holds_after_ticks():
 
 
 @gpu
-def test_runs_are_bitwise_deterministic():
-    world = W.scene(5)
+@pytest.mark.parametrize("index", [0, 1, 5])
+def test_runs_are_bitwise_deterministic(index):
+    world = W.scene(index)
     a, b = gpusim()(world), gpusim()(world)
-    a.run_ticks(25)
-    b.run_ticks(25)
+    a.run_ticks(40)
+    b.run_ticks(40)
     for x, y in zip(a.read_yee() + a.read_state()[:2], b.read_yee() + b.read_state()[:2]):
         np.testing.assert_array_equal(x, y)
 
