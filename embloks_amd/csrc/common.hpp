@@ -48,7 +48,9 @@ constexpr int MAX_STACKS_PER_BLOCK = 64;
 constexpr int SCAN_TILE = 1024;   // cells per block of the cell-list scan
 constexpr int TICK_TABLE = 64;    // ticks of host-computed AC phases / EMF targets
 constexpr int PUSH_TILE = 8;      // cells per side of a push tile
+constexpr int FDTD_TI = 16, FDTD_TJ = 32;   // corners per FDTD tile (x, y)
 constexpr int ION_SUM = 33;       // samples per axis of the summed ion table
+constexpr int PAIR_SAMPLES = 4096;   // samples of the radial pair table (close_range.rs:30)
 
 // Cell flags (CellInfo.flags).
 constexpr uint32_t F_CONDUCTIVE = 1u;
@@ -172,7 +174,7 @@ enum KernelClass {
     KC_SCAN,            // SCAN_TILE cells per block
     KC_CELLS,           // 256 cells per block
     KC_SMOOTH,          // 32 x 32 tiles of the smoothing region
-    KC_FDTD,            // 32 x 64 tiles of the (nx+1) x (ny+1) corner space
+    KC_FDTD,            // FDTD_TI x FDTD_TJ tiles of the (nx+1) x (ny+1) corner space
     KC_GEMM,            // 2 channels x 64 x 64 output tiles of nx x ny
     KC_NODES,           // 256 corners of (nx+1) x (ny+1) per block
     KC_STACKS,          // one block per battery stack (at least one per scene)

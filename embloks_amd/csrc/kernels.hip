@@ -29,13 +29,28 @@ namespace emb {
 // electron index, so the sorted order (and every sum over it) is deterministic.
 
 __global__ __launch_bounds__(256) void k_count(Batch b) {
+    const int lane = threadIdx.x & 63;
     for (int blk = blockIdx.x; blk < b.total[KC_ELECTRONS]; blk += gridDim.x) {
         const int s = scene_of(b, KC_ELECTRONS, blk);
         const DevScene &S = b.scenes[s];
         const int i = block_in_scene(b, KC_ELECTRONS, s, blk) * 256 + (int)threadIdx.x;
+        const int cell = i < S.n ? cell_of(S, S.cxy[i]) : -1;
+        // One atomic per distinct cell of the wave: electrons mostly arrive grouped
+        // by cell, so lanes of a cell share one counter update.
+        unsigned long long pending = __ballot(i < S.n);
+        int rank = 0;
+        while (pending) {
+            const int leader = __ffsll((long long)pending) - 1;
+            const int key = __shfl(cell, leader, 64);
+            const unsigned long long same = __ballot(cell == key) & pending;
+            int base = 0;
+            if (lane == leader) base = atomicAdd(&S.count[key], __popcll(same));
+            base = __shfl(base, leader, 64);
+            if ((same >> lane) & 1ull) rank = base + __popcll(same & ((1ull << lane) - 1ull));
+            pending &= ~same;
+        }
         if (i < S.n) {
-            const int cell = cell_of(S, S.cxy[i]);
-            S.erank[i] = atomicAdd(&S.count[cell], 1);
+            S.erank[i] = rank;
             S.ecell[i] = cell;
         }
     }
@@ -226,8 +241,8 @@ __device__ __forceinline__ float square_floor(const DevScene &S, int k, float u,
 // close-range potential of the ions and electrons near it (Simulation::potential,
 // simulation.rs:48-55; CloseRange::potential, close_range.rs:415-464).
 
-__global__ __launch_bounds__(256) void k_trim(Batch b) {
-    __shared__ float part[2][4];
+__global__ __launch_bounds__(1024) void k_trim(Batch b) {
+    __shared__ float part[2][16];
     for (int blk = blockIdx.x; blk < b.total[KC_STACKS]; blk += gridDim.x) {
         const int s = scene_of(b, KC_STACKS, blk);
         const DevScene &S = b.scenes[s];
@@ -248,7 +263,7 @@ __global__ __launch_bounds__(256) void k_trim(Batch b) {
         const int per = 1 + (S.close_on ? side * side : 0);
         const int lists = I.low_n + I.high_n;
         float acc_lo = 0.0f, acc_hi = 0.0f;
-        for (int it = (int)threadIdx.x; it < lists * per; it += 256) {
+        for (int it = (int)threadIdx.x; it < lists * per; it += 1024) {
             const int li = it / per, sub = it - li * per;
             const bool high = li >= I.low_n;
             const int cell = S.stack_cells[high ? I.high_off + (li - I.low_n) : I.low_off + li];
@@ -283,8 +298,12 @@ __global__ __launch_bounds__(256) void k_trim(Batch b) {
             part[1][threadIdx.x >> 6] = acc_hi;
         }
         __syncthreads();
-        acc_lo = (part[0][0] + part[0][1]) + (part[0][2] + part[0][3]);
-        acc_hi = (part[1][0] + part[1][1]) + (part[1][2] + part[1][3]);
+        acc_lo = 0.0f;
+        acc_hi = 0.0f;
+        for (int w = 0; w < 16; w++) {
+            acc_lo += part[0][w];
+            acc_hi += part[1][w];
+        }
         __syncthreads();
         if (threadIdx.x == 0) {
             const float v_low = acc_lo / (float)I.low_n, v_high = acc_hi / (float)I.high_n;
@@ -368,27 +387,25 @@ struct GlobalNeighbours {
         b = S.cstart[cell];
         e = S.cstart[cell + 1];
     }
-    __device__ float x(int j) const { return S.sfx[j]; }
-    __device__ float y(int j) const { return S.sfy[j]; }
+    __device__ float2 p(int j) const { return make_float2(S.sfx[j], S.sfy[j]); }
 };
 
 struct LdsNeighbours {
-    const float *lx, *ly;        // LDS
-    const int *lb, *le;          // LDS: per window cell, range in lx/ly
+    const float2 *lp;            // LDS: staged positions
+    const int *lb, *le;          // LDS: per window cell, range in lp
     int wc0, wr0, W;
     __device__ void range(int cc, int rr, int &b, int &e) const {
         const int w = (cc - wc0) * W + (rr - wr0);
         b = lb[w];
         e = le[w];
     }
-    __device__ float x(int j) const { return lx[j]; }
-    __device__ float y(int j) const { return ly[j]; }
+    __device__ float2 p(int j) const { return lp[j]; }
 };
 
 // One electron's substep (physics.rs:139-158): `slot` is its place in the sorted
 // snapshot.
 template <class NB>
-__device__ void push_core(const DevScene &S, const NB &nb, int slot, int k, float h, uint32_t t) {
+__device__ void push_core(const DevScene &S, const NB &nb, const float *gt, int slot, int k, float h, uint32_t t) {
     const int id = S.order[slot];
     const int home = S.scell[slot];
     const int hc = home / S.rows, hr = home - (home / S.rows) * S.rows;
@@ -442,7 +459,7 @@ __device__ void push_core(const DevScene &S, const NB &nb, int slot, int k, floa
         // the end. The electron itself contributes g(0) * 0 = 0. Neighbour cells
         // entirely beyond the cutoff from this electron are skipped.
         float sx = 0.0f, sy = 0.0f;
-        const float cut2 = S.cutoff2;
+        const float cut2 = S.cutoff2, gsc = S.g_scale;
         for (int cc = c_lo; cc <= c_hi; cc++) {
             const float ox = (float)(hc - cc);
             const float gapx = ox > 0.0f ? ox + px - 1.0f : (ox < 0.0f ? -ox - px : 0.0f);
@@ -464,16 +481,16 @@ __device__ void push_core(const DevScene &S, const NB &nb, int slot, int k, floa
                 int j, e;
                 nb.range(cc, rr, j, e);
                 for (; j < e; j++) {
-                    const float dx = bx - nb.x(j), dy = by - nb.y(j);
+                    const float2 q = nb.p(j);
+                    const float dx = bx - q.x, dy = by - q.y;
                     const float d2 = dx * dx + dy * dy;
-                    if (d2 < cut2) {
-                        const float tt = d2 * S.g_scale;
-                        const int si = (int)tt;
-                        const float2 g = S.g_lerp[si];
-                        const float l = g.x + g.y * (tt - (float)si);
-                        sx += l * dx;
-                        sy += l * dy;
-                    }
+                    const float tt = fminf(d2 * gsc, (float)PAIR_SAMPLES);
+                    const int si = (int)tt;
+                    const float g0 = gt[si], g1 = gt[si + 1];
+                    float l = g0 + (g1 - g0) * (tt - (float)si);
+                    l = d2 < cut2 ? l : 0.0f;
+                    sx += l * dx;
+                    sy += l * dy;
                 }
             }
         }
@@ -505,7 +522,7 @@ __device__ void push_core(const DevScene &S, const NB &nb, int slot, int k, floa
 }
 
 __device__ void push_one(const DevScene &S, int slot, int k, float h, uint32_t t) {
-    push_core(S, GlobalNeighbours{S}, slot, k, h, t);
+    push_core(S, GlobalNeighbours{S}, S.g_table, slot, k, h, t);
 }
 
 // Substep k for every electron, per PUSH_TILE x PUSH_TILE tile of cells: the
@@ -514,10 +531,12 @@ __device__ void push_one(const DevScene &S, int slot, int k, float h, uint32_t t
 // read from LDS. A tile whose window holds more than PUSH_STAGE electrons reads
 // them from global memory instead.
 constexpr int PUSH_WIN = PUSH_TILE + 2 * 4;   // window side for reach <= 4
-constexpr int PUSH_STAGE = 3584;
+constexpr int PUSH_STAGE = 3328;
 
 __global__ __launch_bounds__(256) void k_push_tiled(Batch b, int k) {
-    __shared__ float lx[PUSH_STAGE], ly[PUSH_STAGE];
+    __shared__ float2 lp[PUSH_STAGE];
+    __shared__ float gt[PAIR_SAMPLES + 2];
+    bool table_loaded = false;
     __shared__ int lb[PUSH_WIN * PUSH_WIN], le[PUSH_WIN * PUSH_WIN];
     __shared__ int col_base[PUSH_WIN + 1], col_g0[PUSH_WIN];   // LDS base and first global slot per column
     __shared__ int own_pref[PUSH_TILE + 1];                     // prefix of the tile's electrons per column
@@ -530,6 +549,10 @@ __global__ __launch_bounds__(256) void k_push_tiled(Batch b, int k) {
         const float h = S.dt / (float)sub;
         if (k == 0 && threadIdx.x == 0 && block_in_scene(b, KC_PUSH, s, blk) == 0) S.ctr->substeps = sub;
         const int local = block_in_scene(b, KC_PUSH, s, blk);
+        if (S.close_on && !table_loaded) {   // the pair table is shared by every scene
+            for (int q = threadIdx.x; q < PAIR_SAMPLES + 2; q += 256) gt[q] = S.g_table[q];
+            table_loaded = true;
+        }
         const int tiles_r = (S.rows + PUSH_TILE - 1) / PUSH_TILE;
         const int tc0 = (local / tiles_r) * PUSH_TILE, tr0 = (local % tiles_r) * PUSH_TILE;
         const int tc1 = min(tc0 + PUSH_TILE, S.cols), tr1 = min(tr0 + PUSH_TILE, S.rows);
@@ -576,10 +599,8 @@ __global__ __launch_bounds__(256) void k_push_tiled(Batch b, int k) {
             }
             for (int q = 0; q < W; q++) {
                 const int n = col_base[q + 1] - col_base[q];
-                for (int j = threadIdx.x; j < n; j += 256) {
-                    lx[col_base[q] + j] = S.sfx[col_g0[q] + j];
-                    ly[col_base[q] + j] = S.sfy[col_g0[q] + j];
-                }
+                for (int j = threadIdx.x; j < n; j += 256)
+                    lp[col_base[q] + j] = make_float2(S.sfx[col_g0[q] + j], S.sfy[col_g0[q] + j]);
             }
         }
         __syncthreads();
@@ -589,8 +610,8 @@ __global__ __launch_bounds__(256) void k_push_tiled(Batch b, int k) {
             while (own_pref[q + 1] <= e) q++;
             const int cc = tc0 + q;
             const int slot = S.cstart[cc * S.rows + tr0] + (e - own_pref[q]);
-            if (staged) push_core(S, LdsNeighbours{lx, ly, lb, le, wc0, wr0, W}, slot, k, h, t);
-            else push_core(S, GlobalNeighbours{S}, slot, k, h, t);
+            if (staged) push_core(S, LdsNeighbours{lp, lb, le, wc0, wr0, W}, gt, slot, k, h, t);
+            else push_core(S, GlobalNeighbours{S}, gt, slot, k, h, t);
         }
         __syncthreads();
     }
@@ -814,13 +835,13 @@ __global__ __launch_bounds__(256) void k_deposit_smooth(Batch b, int bias) {
 
 // ---------------------------------------------------------------------------------------
 // FDTD leapfrog with the absorbing margin, fused (maxwell.rs:235-270;
-// fields.wgsl:106-186): per 32 x 64 tile of the corner space, Bz' = Bz - h/2 curl E
+// fields.wgsl:106-186): per FDTD_TI x FDTD_TJ tile of the corner space, Bz' = Bz - h/2 curl E
 // on the tile plus halo, E' from curl Bz' and the edge currents, Bz'' = Bz' - h/2
 // curl E', then the damping factors. On the scene's last FDTD substep the tile
 // also writes the right-hand sides of the Gauss solve: -4 pi k rho and div E
 // (maxwell.rs:170-173, 200-204; fields.wgsl:192-203).
 
-constexpr int TI = 32, TJ = 64;
+constexpr int TI = FDTD_TI, TJ = FDTD_TJ;
 
 __global__ __launch_bounds__(256) void k_fdtd(Batch b, int f) {
     __shared__ float EX[(TI + 3) * (TJ + 3)];
@@ -1059,7 +1080,7 @@ hipError_t setup_kernels(const Batch &b, hipStream_t stream) {
     } while (0)
 hipError_t tick_kernels(const Batch &b, int fdtd_substeps, hipStream_t stream, hipEvent_t *marks) {
     EMB_MARK(0);
-    EMB_LAUNCH(k_trim, KC_STACKS, 256, b);
+    EMB_LAUNCH(k_trim, KC_STACKS, 1024, b);
     EMB_MARK(1);
     EMB_LAUNCH(k_push_tiled, KC_PUSH, 256, b, 0);
     EMB_MARK(2);

@@ -170,7 +170,7 @@ void scene_blocks(const DevScene &d, int out[KC_COUNT]) {
     out[KC_SCAN] = cdiv(cells, SCAN_TILE);
     out[KC_CELLS] = cdiv(cells, 256);
     out[KC_SMOOTH] = cdiv(d.cols + 2 * d.passes, 32) * cdiv(d.rows + 2 * d.passes, 32);
-    out[KC_FDTD] = cdiv(d.nx + 1, 32) * cdiv(d.ny + 1, 64);
+    out[KC_FDTD] = cdiv(d.nx + 1, FDTD_TI) * cdiv(d.ny + 1, FDTD_TJ);
     out[KC_GEMM] = 2 * cdiv(d.nx, 64) * cdiv(d.ny, 64);
     out[KC_NODES] = cdiv((long long)(d.nx + 1) * (d.ny + 1), 256);
     out[KC_STACKS] = std::max(1, d.n_stacks);

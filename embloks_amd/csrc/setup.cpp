@@ -15,7 +15,6 @@ namespace emb {
 namespace {
 
 constexpr double PI = 3.14159265358979323846;
-constexpr int PAIR_SAMPLES = 4096;     // close_range.rs:30
 constexpr int SQUARE_SAMPLES = 32;     // close_range.rs:32
 constexpr int SOURCE_SAMPLES = 8;      // close_range.rs:35
 constexpr int DIRECTIONS = 16;         // close_range.rs:36
