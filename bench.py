@@ -72,6 +72,30 @@ def run_single(args):
     return world, elapsed, phases, info
 
 
+# Kernel that carries each tick phase (the rocprofv3 name without the namespace).
+PHASE_KERNEL = {"trim": "k_trim", "push": "k_push_tiled", "rest": "k_rest", "count": "k_count", "scan": "k_scan",
+                "scatter": "k_scatter", "canon": "k_canon", "deposit_smooth": "k_deposit_smooth", "fdtd": "k_fdtd",
+                "correct": "k_correct"}
+
+
+def measured_traffic(phase: str):
+    """HBM bytes per launch of the phase's kernel on this workload, from the
+    committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary (copy-kernel calibrated)."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "traffic_512_1M.txt")
+    if not os.path.exists(path):
+        return None
+    table = {}
+    for line in open(path):
+        if line.startswith("#") or not line.strip():
+            continue
+        name, rd, wr = line.split()
+        table[name] = int(rd) + int(wr)
+    if phase == "dst_gemm":
+        parts = [table.get(f"k_dst_gemm<{i}>") for i in range(4)]
+        return sum(parts) if all(p is not None for p in parts) else None
+    return table.get(PHASE_KERNEL.get(phase, ""))
+
+
 def roofline(phases: dict, info: dict):
     dom = max(phases, key=phases.get)
     amount, unit, bound = algorithmic(dom, info["nx"], info["ny"], info["cols"], info["rows"], info["n"])
@@ -83,7 +107,7 @@ def roofline(phases: dict, info: dict):
         achieved = amount / sec / 1e9
         peak, u = HBM_PEAK_GBS, "GB/s"
     return dom, {"bound": bound, "achieved": round(achieved, 2), "peak": peak, "unit": u,
-                 "frac": round(achieved / peak, 4), "traffic": None, "kernel": dom,
+                 "frac": round(achieved / peak, 4), "traffic": measured_traffic(dom), "kernel": dom,
                  "avg_ms": round(phases[dom], 4)}
 
 
@@ -130,6 +154,20 @@ def run_serving(n_requests: int, steps_scale: float, batched: bool):
     return st
 
 
+def copy_calibration(nbytes: int = 1 << 30, reps: int = 5):
+    """A copy kernel of known size: each launch reads and writes `nbytes`, so the
+    HBM byte counters it shows give the scale of the same counters elsewhere."""
+    import torch
+
+    src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(f"copy calibration: {reps} copies of {nbytes} bytes", file=sys.stderr)
+    del src, dst
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -139,12 +177,16 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=48)
     ap.add_argument("--serve-scale", type=float, default=0.02, help="fraction of each request's 1000-20000 steps")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first run a plain 1 GiB device-to-device copy 5 times (scale for HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU design is not built.",
               file=sys.stderr)
         sys.exit(2)
 
+    if args.copy_calibration:
+        copy_calibration()
     world, elapsed, phases, info = run_single(args)
     nodes = info["nx"] * info["ny"]
     value = nodes * args.steps / elapsed
