@@ -29,6 +29,7 @@ import numpy as np
 
 HBM_PEAK_GBS = 8000.0
 FP32_MFMA_PEAK_TFLOPS = 157.3
+BF16_MFMA_PEAK_TFLOPS = 2500.0
 
 
 def algorithmic(phase: str, nx: int, ny: int, cols: int, rows: int, n: int):
@@ -52,6 +53,31 @@ def algorithmic(phase: str, nx: int, ny: int, cols: int, rows: int, n: int):
     if phase == "correct":
         return 16.0 * nodes, "byte", "hbm"
     return 0.0, "byte", "hbm"
+
+
+def fp64_poisson(rhs: np.ndarray) -> np.ndarray:
+    """Maxwell::poisson (crates/embloks-core/src/maxwell.rs:139-168) in FP64
+    through scipy's DST-I (scipy's type-1 DST is twice the reference's
+    unnormalised one): the error reference for the isolated solve."""
+    import scipy.fft as F
+
+    n, m = rhs.shape
+    kx = 2.0 * np.cos(np.pi * np.arange(1, n + 1) / (n + 1)) - 2.0
+    ky = 2.0 * np.cos(np.pi * np.arange(1, m + 1) / (m + 1)) - 2.0
+    inv = (4.0 / ((n + 1) * (m + 1))) / (kx[:, None] + ky[None, :])
+    s = F.dst(F.dst(rhs, type=1, axis=0), type=1, axis=1) / 4.0 * inv
+    return F.dst(F.dst(s, type=1, axis=0), type=1, axis=1) / 4.0
+
+
+def laplacian_residual(phi: np.ndarray, rhs: np.ndarray) -> float:
+    """max |lap(phi) - rhs| / max |rhs| with the 5-point Laplacian and phi = 0
+    outside the grid (the operator the DST-I solve inverts, maxwell.rs:56-67)."""
+    lap = -4.0 * phi
+    lap[1:, :] += phi[:-1, :]
+    lap[:-1, :] += phi[1:, :]
+    lap[:, 1:] += phi[:, :-1]
+    lap[:, :-1] += phi[:, 1:]
+    return float(np.abs(lap - rhs).max() / np.abs(rhs).max())
 
 
 def run_single(args):
@@ -109,6 +135,34 @@ def roofline(phases: dict, info: dict):
     return dom, {"bound": bound, "achieved": round(achieved, 2), "peak": peak, "unit": u,
                  "frac": round(achieved / peak, 4), "traffic": measured_traffic(dom), "kernel": dom,
                  "avg_ms": round(phases[dom], 4)}
+
+
+def run_poisson(n: int = 2048, repeats: int = 20):
+    """The DST Poisson solve in isolation on an n x n node grid (200 Gaussian
+    blobs, seed 3): FP32 and BF16x3 MFMA against FP64, with time per solve from
+    HIP events over `repeats` back-to-back solves on resident data."""
+    from embloks_amd import world as W
+    from embloks_amd.gpusim import Poisson
+
+    rho = W.gaussian_blobs(n)
+    want = fp64_poisson(rho)
+    flops = 8.0 * n ** 3   # four products of 2 n^3
+    p = Poisson(n, n)
+    p.set_rhs(rho)
+    out = {"grid": f"{n}x{n}", "rhs": "200 Gaussian blobs, sigma 2-40, seed 3", "flop_per_solve": flops,
+           "fp64_residual": laplacian_residual(want, rho),
+           "fp32_rounding_floor_residual": laplacian_residual(want.astype(np.float32).astype(np.float64), rho)}
+    for prec, name, peak, issued in ((0, "fp32", FP32_MFMA_PEAK_TFLOPS, 1.0), (1, "bf16x3", BF16_MFMA_PEAK_TFLOPS, 3.0)):
+        p.solve(prec, 0, 2)   # warm-up
+        ms = p.solve(prec, 0, repeats)
+        phi = p.read()
+        tflops = flops / (ms * 1e-3) / 1e12
+        out[name] = {"ms_per_solve": round(ms, 4), "tflops": round(tflops, 2),
+                     "mfma_frac": round(tflops * issued / peak, 4),
+                     "max_rel_err": float(np.abs(phi - want).max() / np.abs(want).max()),
+                     "residual": laplacian_residual(phi, rho)}
+    p.destroy()
+    return out
 
 
 def cpu_baseline(world, budget_s: float = 20.0):
@@ -177,6 +231,7 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=48)
     ap.add_argument("--serve-scale", type=float, default=0.02, help="fraction of each request's 1000-20000 steps")
+    ap.add_argument("--no-poisson", action="store_true", help="skip the isolated 2048^2 DST solve")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a plain 1 GiB device-to-device copy 5 times (scale for HBM byte counters)")
     args = ap.parse_args()
@@ -211,6 +266,8 @@ def main():
         "roofline": roof,
         "gauss_residual_fp64": info["gauss_residual"],
     }
+    if not args.no_poisson:
+        out["poisson"] = run_poisson()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(world)
     if not args.no_serve:

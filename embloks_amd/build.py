@@ -17,7 +17,7 @@ CSRC = HERE / "csrc"
 OUT = HERE / "libembloks_hip.so"
 BUILD = HERE.parent / "_build" / "embloks_amd"
 
-HIP_SOURCES = ["kernels.hip", "gemm.hip", "runtime.hip", "server.hip"]
+HIP_SOURCES = ["kernels.hip", "gemm.hip", "runtime.hip", "server.hip", "poisson.hip"]
 CXX_SOURCES = ["setup.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",

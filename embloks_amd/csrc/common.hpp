@@ -108,6 +108,7 @@ struct DevScene {
     float inv_mass;
     float q;                  // electron charge
     float k_pair;             // k q q (close_range.rs:366)
+    float k_pair_u;           // k q q / pair_unit: the pair sum in table units back to a force
     float k_ion_q;            // k * cell_charge * q (close_range.rs:348)
     float k_q;                // k * q (charges_potential, close_range.rs:463)
     float cutoff2, g_scale, half, square_samples;
@@ -119,7 +120,7 @@ struct DevScene {
     float rho_from_fixed;     // |q_e| / 2^44 / 4^(2 passes): int64 charge -> FP32
     float j_from_fixed;       // q_e / 2^28 / 4^(2 passes)
     float i_from_fixed;       // q_e / 2^28 (cell-binned current)
-    float pad_f;
+    float pair_unit;          // sqrt(g_scale): pair offsets in these units square to table coordinates
     int32_t passes;
     int32_t pad1;
     // --- fields (FP32)
@@ -150,7 +151,7 @@ struct DevScene {
     const CellInfo *info;
     const float *density;     // per cell (ion density)
     const float *ion_pot;     // per cell: close-range potential of all ions
-    const float *g_table;     // PAIR_SAMPLES + 2
+    const float *g_table;     // PAIR_SAMPLES + 4: the reference's PAIR_SAMPLES + 2 samples, then two zeros
     const lerp2 *g_lerp;     // PAIR_SAMPLES + 1: (g[i], g[i+1] - g[i])
     const float *square;      // square_n^2 * 3
     const float *ion_sum;     // ION_SUM^2 * 2: the square table's force channels summed over

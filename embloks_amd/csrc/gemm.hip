@@ -4,116 +4,263 @@
 // right-hand side with a type-I sine transform along both axes, scales by the
 // inverse Laplacian eigenvalues, and transforms back. A DST-I along an axis is a
 // product with the symmetric sine matrix S[k][n] = sin(pi (k+1)(n+1) / (N+1))
-// (poisson.wgsl:1-6), so the solve of both right-hand sides (the charge's
-// potential and the divergence correction, maxwell.rs:200-204) is four
-// products per channel:
+// (poisson.wgsl:1-6), so one solve is four products:
 //   stage 0: w2 = Sx . w1              stage 1: w1 = (w2 . Sy) * inv_lap
 //   stage 2: w2 = Sx . w1              stage 3: phi = w2 . Sy
-// Each block computes a 64 x 64 output tile with FP32 MFMA (v_mfma_f32_16x16x4f32):
-// four 64-wide waves, each a 32 x 32 sub-tile of 2 x 2 MFMA tiles, operands
-// staged through LDS in K slices of 16.
+// The tick solves two right-hand sides (the charge's potential and the
+// divergence correction, maxwell.rs:200-204) side by side.
+//
+// Two tile kernels compute C = A . B (row-major, optional elementwise scale):
+//   * FP32: v_mfma_f32_16x16x4f32, FP32 in and out; what the tick runs.
+//   * BF16x3: each operand split as x = hi + lo (hi = bf16(x), lo = bf16(x - hi)),
+//     A.B ~ lo.hi + hi.lo + hi.hi with v_mfma_f32_16x16x32_bf16 and FP32
+//     accumulation (the lo.lo term is dropped). Offered for the isolated solve;
+//     its error is measured against FP32 and FP64.
+// Both use 256 threads as a 2 x 2 grid of waves, operands staged through LDS
+// in K slices with the next slice prefetched into registers.
 
 #include "device.hpp"
+#include "gemm.hpp"
 
 namespace emb {
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
-constexpr int GM = 64, GN = 64, GK = 16;
-constexpr int LDA = GM + 4, LDB = GN + 4;
+// Four consecutive floats of a row, zero past `limit` (vector load when whole and aligned).
+__device__ __forceinline__ float4 load4(const float *p, int avail, bool vec) {
+    if (vec && avail >= 4) return *reinterpret_cast<const float4 *>(p);
+    float4 r;
+    r.x = avail > 0 ? p[0] : 0.0f;
+    r.y = avail > 1 ? p[1] : 0.0f;
+    r.z = avail > 2 ? p[2] : 0.0f;
+    r.w = avail > 3 ? p[3] : 0.0f;
+    return r;
+}
+
+// ---------------------------------------------------------------------------------------
+// FP32 tile: BM x BN output, K slices of 16, waves of (BM/2) x (BN/2).
+
+template <int BM, int BN>
+__device__ __forceinline__ void tile_f32(const GemmArgs &g, int m0, int n0, float *lds) {
+    constexpr int GK = 16, LA = BM + 4, LB = BN + 4;
+    constexpr int WM = BM / 2, WN = BN / 2, TM = WM / 16, TN = WN / 16;
+    constexpr int AV = BM * GK / 4 / 256, BV = BN * GK / 4 / 256;
+    static_assert(AV >= 1 && BV >= 1, "tile too small for 256 threads");
+    float *As = lds, *Bs = lds + 2 * GK * LA;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = (wave >> 1) * WM, wn = (wave & 1) * WN;
+    const int l16 = lane & 15, kq = lane >> 4;
+    const bool avec = (g.lda & 3) == 0 && (reinterpret_cast<uintptr_t>(g.A) & 15) == 0;
+    const bool bvec = (g.ldb & 3) == 0 && (reinterpret_cast<uintptr_t>(g.B) & 15) == 0;
+    floatx4 acc[TM][TN];
+#pragma unroll
+    for (int x = 0; x < TM; x++)
+#pragma unroll
+        for (int y = 0; y < TN; y++) acc[x][y] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+    float4 ra[AV], rb[BV];
+    auto load = [&](int k0) {
+#pragma unroll
+        for (int v = 0; v < AV; v++) {
+            const int idx = tid + 256 * v, row = idx >> 2, kk = (idx & 3) * 4;
+            const int gr = m0 + row, gk = k0 + kk;
+            ra[v] = gr < g.M ? load4(g.A + (size_t)gr * g.lda + gk, g.K - gk, avec) : float4{0.0f, 0.0f, 0.0f, 0.0f};
+        }
+#pragma unroll
+        for (int v = 0; v < BV; v++) {
+            const int idx = tid + 256 * v, kr = idx / (BN / 4), col = (idx % (BN / 4)) * 4;
+            const int gk = k0 + kr, gc = n0 + col;
+            rb[v] = gk < g.K ? load4(g.B + (size_t)gk * g.ldb + gc, g.N - gc, bvec) : float4{0.0f, 0.0f, 0.0f, 0.0f};
+        }
+    };
+    auto store = [&](int buf) {
+        float *a = As + buf * GK * LA, *b = Bs + buf * GK * LB;
+#pragma unroll
+        for (int v = 0; v < AV; v++) {
+            const int idx = tid + 256 * v, row = idx >> 2, kk = (idx & 3) * 4;
+            a[(kk + 0) * LA + row] = ra[v].x;
+            a[(kk + 1) * LA + row] = ra[v].y;
+            a[(kk + 2) * LA + row] = ra[v].z;
+            a[(kk + 3) * LA + row] = ra[v].w;
+        }
+#pragma unroll
+        for (int v = 0; v < BV; v++) {
+            const int idx = tid + 256 * v, kr = idx / (BN / 4), col = (idx % (BN / 4)) * 4;
+            *reinterpret_cast<float4 *>(b + kr * LB + col) = rb[v];
+        }
+    };
+    load(0);
+    store(0);
+    __syncthreads();
+    int buf = 0;
+    for (int k0 = 0; k0 < g.K; k0 += GK) {
+        const bool more = k0 + GK < g.K;
+        if (more) load(k0 + GK);
+        const float *a = As + buf * GK * LA, *b = Bs + buf * GK * LB;
+#pragma unroll
+        for (int kk = 0; kk < GK; kk += 4) {
+            float af[TM], bf[TN];
+#pragma unroll
+            for (int x = 0; x < TM; x++) af[x] = a[(kk + kq) * LA + wm + x * 16 + l16];
+#pragma unroll
+            for (int y = 0; y < TN; y++) bf[y] = b[(kk + kq) * LB + wn + y * 16 + l16];
+#pragma unroll
+            for (int x = 0; x < TM; x++)
+#pragma unroll
+                for (int y = 0; y < TN; y++)
+                    acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[x], bf[y], acc[x][y], 0, 0, 0);
+        }
+        if (more) store(buf ^ 1);
+        __syncthreads();
+        buf ^= 1;
+    }
+    // D[row][col]: lane holds rows 4 * (lane / 16) + i of column lane % 16.
+#pragma unroll
+    for (int x = 0; x < TM; x++)
+#pragma unroll
+        for (int y = 0; y < TN; y++)
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int row = m0 + wm + x * 16 + kq * 4 + i, col = n0 + wn + y * 16 + l16;
+                if (row < g.M && col < g.N) {
+                    float v = acc[x][y][i];
+                    if (g.scale) v *= g.scale[(size_t)row * g.ldc + col];
+                    g.C[(size_t)row * g.ldc + col] = v;
+                }
+            }
+}
+
+// ---------------------------------------------------------------------------------------
+// BF16x3 tile: BM x BN output, K slices of 32 (one MFMA), LDS rows K-contiguous:
+// A as [m][k], B transposed to [n][k], hi and lo halves side by side.
+
+template <int BM, int BN>
+__device__ __forceinline__ void tile_bf16x3(const GemmArgs &g, int m0, int n0, __bf16 *lds) {
+    constexpr int GK = 32, P = GK + 8;   // row pitch in bf16: 80 bytes keeps 16-byte reads aligned
+    constexpr int WM = BM / 2, WN = BN / 2, TM = WM / 16, TN = WN / 16;
+    constexpr int AV = BM * GK / 4 / 256, BV = BN * GK / 4 / 256;
+    static_assert(AV >= 1 && BV >= 1, "tile too small for 256 threads");
+    __bf16 *Ah = lds, *Al = Ah + BM * P, *Bh = Al + BM * P, *Bl = Bh + BN * P;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = (wave >> 1) * WM, wn = (wave & 1) * WN;
+    const int l16 = lane & 15, kq = lane >> 4;
+    const bool avec = (g.lda & 3) == 0 && (reinterpret_cast<uintptr_t>(g.A) & 15) == 0;
+    const bool bvec = (g.ldb & 3) == 0 && (reinterpret_cast<uintptr_t>(g.B) & 15) == 0;
+    floatx4 acc[TM][TN];
+#pragma unroll
+    for (int x = 0; x < TM; x++)
+#pragma unroll
+        for (int y = 0; y < TN; y++) acc[x][y] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+    float4 ra[AV], rb[BV];
+    auto load = [&](int k0) {
+#pragma unroll
+        for (int v = 0; v < AV; v++) {
+            const int idx = tid + 256 * v, row = idx >> 3, kk = (idx & 7) * 4;
+            const int gr = m0 + row, gk = k0 + kk;
+            ra[v] = gr < g.M ? load4(g.A + (size_t)gr * g.lda + gk, g.K - gk, avec) : float4{0.0f, 0.0f, 0.0f, 0.0f};
+        }
+#pragma unroll
+        for (int v = 0; v < BV; v++) {
+            const int idx = tid + 256 * v, kr = idx / (BN / 4), col = (idx % (BN / 4)) * 4;
+            const int gk = k0 + kr, gc = n0 + col;
+            rb[v] = gk < g.K ? load4(g.B + (size_t)gk * g.ldb + gc, g.N - gc, bvec) : float4{0.0f, 0.0f, 0.0f, 0.0f};
+        }
+    };
+    auto split = [](float x, __bf16 &h, __bf16 &l) {
+        h = (__bf16)x;
+        l = (__bf16)(x - (float)h);
+    };
+    auto store = [&]() {
+#pragma unroll
+        for (int v = 0; v < AV; v++) {
+            const int idx = tid + 256 * v, row = idx >> 3, kk = (idx & 7) * 4;
+            const float e[4] = {ra[v].x, ra[v].y, ra[v].z, ra[v].w};
+#pragma unroll
+            for (int q = 0; q < 4; q++) split(e[q], Ah[row * P + kk + q], Al[row * P + kk + q]);
+        }
+#pragma unroll
+        for (int v = 0; v < BV; v++) {
+            const int idx = tid + 256 * v, kr = idx / (BN / 4), col = (idx % (BN / 4)) * 4;
+            const float e[4] = {rb[v].x, rb[v].y, rb[v].z, rb[v].w};
+#pragma unroll
+            for (int q = 0; q < 4; q++) split(e[q], Bh[(col + q) * P + kr], Bl[(col + q) * P + kr]);
+        }
+    };
+    load(0);
+    for (int k0 = 0; k0 < g.K; k0 += GK) {
+        store();
+        __syncthreads();
+        if (k0 + GK < g.K) load(k0 + GK);
+        bf16x8 ah[TM], al[TM], bh[TN], bl[TN];
+#pragma unroll
+        for (int x = 0; x < TM; x++) {
+            const int r = (wm + x * 16 + l16) * P + kq * 8;
+            ah[x] = *reinterpret_cast<const bf16x8 *>(Ah + r);
+            al[x] = *reinterpret_cast<const bf16x8 *>(Al + r);
+        }
+#pragma unroll
+        for (int y = 0; y < TN; y++) {
+            const int r = (wn + y * 16 + l16) * P + kq * 8;
+            bh[y] = *reinterpret_cast<const bf16x8 *>(Bh + r);
+            bl[y] = *reinterpret_cast<const bf16x8 *>(Bl + r);
+        }
+#pragma unroll
+        for (int x = 0; x < TM; x++)
+#pragma unroll
+            for (int y = 0; y < TN; y++) {
+                acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[x], bh[y], acc[x][y], 0, 0, 0);
+                acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[x], bl[y], acc[x][y], 0, 0, 0);
+                acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[x], bh[y], acc[x][y], 0, 0, 0);
+            }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int x = 0; x < TM; x++)
+#pragma unroll
+        for (int y = 0; y < TN; y++)
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int row = m0 + wm + x * 16 + kq * 4 + i, col = n0 + wn + y * 16 + l16;
+                if (row < g.M && col < g.N) {
+                    float v = acc[x][y][i];
+                    if (g.scale) v *= g.scale[(size_t)row * g.ldc + col];
+                    g.C[(size_t)row * g.ldc + col] = v;
+                }
+            }
+}
+
+// ---------------------------------------------------------------------------------------
+// The tick's solve over a batch of scenes: blocks walk (scene, channel, tile).
+
+constexpr int TICK_BM = 64, TICK_BN = 64;
 
 template <int STAGE>
 __global__ __launch_bounds__(256) void k_dst_gemm(Batch b) {
     constexpr bool LEFT = (STAGE == 0 || STAGE == 2);
-    __shared__ float As[2][GK * LDA];
-    __shared__ float Bs[2][GK * LDB];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
-    const int l16 = lane & 15, kq = lane >> 4;
+    __shared__ __align__(16) float lds[2 * 16 * (TICK_BM + 4 + TICK_BN + 4)];
     for (int blk = blockIdx.x; blk < b.total[KC_GEMM]; blk += gridDim.x) {
         const int s = scene_of(b, KC_GEMM, blk);
         const DevScene &S = b.scenes[s];
         const int nx = S.nx, ny = S.ny;
         const int local = block_in_scene(b, KC_GEMM, s, blk);
-        const int tm_n = (nx + GM - 1) / GM, tn_n = (ny + GN - 1) / GN;
+        const int tm_n = (nx + TICK_BM - 1) / TICK_BM, tn_n = (ny + TICK_BN - 1) / TICK_BN;
         const int ch = local / (tm_n * tn_n);
         const int rem = local - ch * tm_n * tn_n;
-        const int m0 = (rem / tn_n) * GM, n0 = (rem % tn_n) * GN;
         const size_t nodes = (size_t)nx * ny;
-        const float *in = (STAGE == 0 || STAGE == 2) ? S.w1 : S.w2;
-        in += ch * nodes;
-        float *out = STAGE == 3 ? S.phi : (STAGE == 1 ? S.w1 : S.w2);
-        out += ch * nodes;
-        const int K = LEFT ? nx : ny;
-        // A (M x K): LEFT Sx[row][k], RIGHT in[row][k]. B (K x N): LEFT in[k][col], RIGHT Sy[k][col].
-        const float *A = LEFT ? S.sine_x : in;
-        const int lda = LEFT ? nx : ny;
-        const float *B = LEFT ? in : S.sine_y;
-        const int ldb = ny;
-        floatx4 acc[2][2];
-        for (int x = 0; x < 2; x++)
-            for (int y = 0; y < 2; y++) acc[x][y] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
-        // Global -> register staging of one K slice.
-        const int arow = tid >> 2, akq = (tid & 3) * 4;
-        const int bk = tid >> 4, bcol = (tid & 15) * 4;
-        float ra[4], rb[4];
-        auto load = [&](int k0) {
-            const int row = m0 + arow;
-#pragma unroll
-            for (int e = 0; e < 4; e++) {
-                const int k = k0 + akq + e;
-                ra[e] = (row < nx && k < K) ? A[(size_t)row * lda + k] : 0.0f;
-            }
-            const int k = k0 + bk;
-#pragma unroll
-            for (int e = 0; e < 4; e++) {
-                const int col = n0 + bcol + e;
-                rb[e] = (k < K && col < ny) ? B[(size_t)k * ldb + col] : 0.0f;
-            }
-        };
-        auto store = [&](int buf) {
-#pragma unroll
-            for (int e = 0; e < 4; e++) As[buf][(akq + e) * LDA + arow] = ra[e];
-#pragma unroll
-            for (int e = 0; e < 4; e++) Bs[buf][bk * LDB + bcol + e] = rb[e];
-        };
-        load(0);
-        store(0);
-        __syncthreads();
-        int buf = 0;
-        for (int k0 = 0; k0 < K; k0 += GK) {
-            const bool more = k0 + GK < K;
-            if (more) load(k0 + GK);
-#pragma unroll
-            for (int kk = 0; kk < GK; kk += 4) {
-                const float a0 = As[buf][(kk + kq) * LDA + wm + l16];
-                const float a1 = As[buf][(kk + kq) * LDA + wm + 16 + l16];
-                const float b0 = Bs[buf][(kk + kq) * LDB + wn + l16];
-                const float b1 = Bs[buf][(kk + kq) * LDB + wn + 16 + l16];
-                acc[0][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b0, acc[0][0], 0, 0, 0);
-                acc[0][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b1, acc[0][1], 0, 0, 0);
-                acc[1][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b0, acc[1][0], 0, 0, 0);
-                acc[1][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b1, acc[1][1], 0, 0, 0);
-            }
-            if (more) store(buf ^ 1);
-            __syncthreads();
-            buf ^= 1;
-        }
-        // D[row][col]: lane holds rows 4 * (lane / 16) + i of column lane % 16.
-#pragma unroll
-        for (int x = 0; x < 2; x++)
-#pragma unroll
-            for (int y = 0; y < 2; y++)
-#pragma unroll
-                for (int i = 0; i < 4; i++) {
-                    const int row = m0 + wm + x * 16 + kq * 4 + i;
-                    const int col = n0 + wn + y * 16 + l16;
-                    if (row < nx && col < ny) {
-                        float v = acc[x][y][i];
-                        if (STAGE == 1) v *= S.inv_lap[(size_t)row * ny + col];
-                        out[(size_t)row * ny + col] = v;
-                    }
-                }
+        const float *in = (LEFT ? S.w1 : S.w2) + ch * nodes;
+        float *out = (STAGE == 3 ? S.phi : (STAGE == 1 ? S.w1 : S.w2)) + ch * nodes;
+        GemmArgs g;
+        g.A = LEFT ? S.sine_x : in;
+        g.lda = LEFT ? nx : ny;
+        g.B = LEFT ? in : S.sine_y;
+        g.ldb = ny;
+        g.C = out;
+        g.ldc = ny;
+        g.scale = STAGE == 1 ? S.inv_lap : nullptr;
+        g.M = nx;
+        g.N = ny;
+        g.K = LEFT ? nx : ny;
+        tile_f32<TICK_BM, TICK_BN>(g, (rem / tn_n) * TICK_BM, (rem % tn_n) * TICK_BN, lds);
         __syncthreads();
     }
 }
@@ -122,10 +269,6 @@ template __global__ void k_dst_gemm<0>(Batch);
 template __global__ void k_dst_gemm<1>(Batch);
 template __global__ void k_dst_gemm<2>(Batch);
 template __global__ void k_dst_gemm<3>(Batch);
-
-}  // namespace emb
-
-namespace emb {
 
 // The four products of one Poisson solve for every scene of the batch.
 hipError_t launch_dst_gemms(const Batch &b, hipStream_t stream) {
@@ -136,6 +279,44 @@ hipError_t launch_dst_gemms(const Batch &b, hipStream_t stream) {
     k_dst_gemm<1><<<grid, 256, 0, stream>>>(b);
     k_dst_gemm<2><<<grid, 256, 0, stream>>>(b);
     k_dst_gemm<3><<<grid, 256, 0, stream>>>(b);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------------------
+// One product on plain pointers (the isolated solve).
+
+template <int BM, int BN>
+__global__ __launch_bounds__(256) void k_gemm_f32(GemmArgs g) {
+    __shared__ __align__(16) float lds[2 * 16 * (BM + 4 + BN + 4)];
+    const int tn = (g.N + BN - 1) / BN, tiles = ((g.M + BM - 1) / BM) * tn;
+    for (int blk = blockIdx.x; blk < tiles; blk += gridDim.x) {
+        tile_f32<BM, BN>(g, (blk / tn) * BM, (blk % tn) * BN, lds);
+        __syncthreads();
+    }
+}
+
+template <int BM, int BN>
+__global__ __launch_bounds__(256) void k_gemm_bf16x3(GemmArgs g) {
+    __shared__ __align__(16) __bf16 lds[2 * (BM + BN) * 40];
+    const int tn = (g.N + BN - 1) / BN, tiles = ((g.M + BM - 1) / BM) * tn;
+    for (int blk = blockIdx.x; blk < tiles; blk += gridDim.x) {
+        tile_bf16x3<BM, BN>(g, (blk / tn) * BM, (blk % tn) * BN, lds);
+        __syncthreads();
+    }
+}
+
+hipError_t launch_gemm(const GemmArgs &g, int precision, int tile, hipStream_t stream) {
+    const int t = tile == 128 ? 128 : 64;
+    const long long tiles = (long long)((g.M + t - 1) / t) * ((g.N + t - 1) / t);
+    const int grid = (int)(tiles < 8192 ? tiles : 8192);
+    if (grid == 0) return hipSuccess;
+    if (precision == GEMM_BF16X3) {
+        if (t == 128) k_gemm_bf16x3<128, 128><<<grid, 256, 0, stream>>>(g);
+        else k_gemm_bf16x3<64, 64><<<grid, 256, 0, stream>>>(g);
+    } else {
+        if (t == 128) k_gemm_f32<128, 128><<<grid, 256, 0, stream>>>(g);
+        else k_gemm_f32<64, 64><<<grid, 256, 0, stream>>>(g);
+    }
     return hipGetLastError();
 }
 

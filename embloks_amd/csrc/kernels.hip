@@ -378,8 +378,12 @@ __device__ __forceinline__ int substeps_of(const DevScene &S, uint32_t t) {
     return sub < 1 ? 1 : sub;
 }
 
-// Neighbour access of the close-range loop: the whole sorted snapshot in global
-// memory, or a window of it staged in LDS.
+// Neighbour access of the per-electron close-range loop: the whole sorted
+// snapshot in global memory (positions as fractions of their own cell), or a
+// window of it staged in LDS (positions relative to the tile's centre cell).
+// Positions are in pair units (cells times pair_unit, so that a squared offset
+// is a coordinate of the pair table); `origin` gives the electron's position in
+// the frame of neighbour cell (cc, rr)'s stored positions.
 struct GlobalNeighbours {
     const DevScene &S;
     __device__ void range(int cc, int rr, int &b, int &e) const {
@@ -387,31 +391,32 @@ struct GlobalNeighbours {
         b = S.cstart[cell];
         e = S.cstart[cell + 1];
     }
-    __device__ float2 p(int j) const { return make_float2(S.sfx[j], S.sfy[j]); }
+    __device__ float2 p(int j) const { return make_float2(S.sfx[j] * S.pair_unit, S.sfy[j] * S.pair_unit); }
+    __device__ float2 origin(int hc, int hr, float px, float py, int cc, int rr) const {
+        return make_float2(((float)(hc - cc) + px) * S.pair_unit, ((float)(hr - rr) + py) * S.pair_unit);
+    }
 };
 
 struct LdsNeighbours {
-    const float2 *lp;            // LDS: staged positions
+    const float2 *lp;            // LDS: staged positions, tile frame, pair units
     const int *lb, *le;          // LDS: per window cell, range in lp
-    int wc0, wr0, W;
+    int wc0, wr0, W, cmx, cmy;
+    float unit;
     __device__ void range(int cc, int rr, int &b, int &e) const {
         const int w = (cc - wc0) * W + (rr - wr0);
         b = lb[w];
         e = le[w];
     }
     __device__ float2 p(int j) const { return lp[j]; }
+    __device__ float2 origin(int hc, int hr, float px, float py, int, int) const {
+        return make_float2(((float)(hc - cmx) + px) * unit, ((float)(hr - cmy) + py) * unit);
+    }
 };
 
-// One electron's substep (physics.rs:139-158): `slot` is its place in the sorted
-// snapshot.
-template <class NB>
-__device__ void push_core(const DevScene &S, const NB &nb, const float *gt, int slot, int k, float h, uint32_t t) {
-    const int id = S.order[slot];
-    const int home = S.scell[slot];
-    const int hc = home / S.rows, hr = home - (home / S.rows) * S.rows;
-    const float px = S.sfx[slot], py = S.sfy[slot];   // fractions within the home cell
-    const float vx = S.svx[slot], vy = S.svy[slot];
-    const CellInfo info = S.info[home];
+// Lorentz force of the gathered fields plus the battery's impressed field, as an
+// acceleration (physics.rs:49-51, 139-149; battery.rs:210-228).
+__device__ __forceinline__ void field_accel(const DevScene &S, int hc, int hr, float px, float py, float vx, float vy,
+                                            uint32_t flags, int k, float h, uint32_t t, float &ax, float &ay) {
     const int m = S.margin;
     // Maxwell::gather (maxwell.rs:336-350): Ex at (x + m, y - 1/2 + m), Ey at
     // (x - 1/2 + m, y + m), Bz at (x + m, y + m), each from its staggered grid.
@@ -420,94 +425,74 @@ __device__ void push_core(const DevScene &S, const NB &nb, const float *gt, int 
     float ex = gather(S.ex, S.nx + 1, S.ny, hc + m, px, hy + m, qy);
     float ey = gather(S.ey, S.nx, S.ny + 1, hx + m, qx, hr + m, py);
     const float bz = gather(S.bz, S.nx + 1, S.ny + 1, hc + m, px, hr + m, py);
-    // battery::impressed_field_grid (battery.rs:210-228).
-    if (info.flags & F_BATTERY) {
-        const BatteryCell bat = S.bats[(info.flags >> F_BAT_SHIFT) - 1];
+    if (flags & F_BATTERY) {
+        const BatteryCell bat = S.bats[(flags >> F_BAT_SHIFT) - 1];
         float value = bat.emf;
         if (bat.stack >= 0 && S.has_gain[bat.stack]) value *= S.gains[bat.stack];
         if (bat.frequency > 0.0f) {
             const float phase = S.phase_tab[(size_t)(t % TICK_TABLE) * S.n_ac + bat.ac_slot] + bat.frequency * ((float)k * h);
             value *= sinf(6.283185307179586f * phase);
         }
-        const int o = (info.flags >> F_ORI_SHIFT) & 3;
+        const int o = (flags >> F_ORI_SHIFT) & 3;
         ex += o == 0 ? value : (o == 2 ? -value : 0.0f);
         ey += o == 1 ? value : (o == 3 ? -value : 0.0f);
     }
-    // physics.rs:49-51, 139-149
-    float ax = S.qm * (ex + vy * bz), ay = S.qm * (ey - vx * bz);
-    if (S.close_on) {
-        // CloseRange::force (close_range.rs:314-376), every pair seen from both sides.
-        float fx = 0.0f, fy = 0.0f;
-        const int R = S.reach;
-        const int c_lo = max(hc - R, 0), c_hi = min(hc + R, S.cols - 1);
-        const int r_lo = max(hr - R, 0), r_hi = min(hr + R, S.rows - 1);
-        const bool uniform = (info.flags & F_ION_UNIFORM) != 0u;
-        if (uniform) {
-            // Every neighbour cell carries the same ion density and all share the
-            // same fractional table offset, so the sum of their bilinear lookups
-            // is one lookup in the table summed over the offsets.
-            const float u = (px - 0.5f + S.half) * S.square_samples, w = (py - 0.5f + S.half) * S.square_samples;
-            const int a = (int)u, bb = (int)w;
-            const float fu = u - (float)a, fv = w - (float)bb;
-            const float *tab = S.ion_sum + ((size_t)(a - S.ion_a0) * ION_SUM + (bb - S.ion_a0)) * 2;
-            const float coef = S.k_ion_q * S.density[home];
-            const float w00 = (1.0f - fu) * (1.0f - fv), w10 = fu * (1.0f - fv), w01 = (1.0f - fu) * fv, w11 = fu * fv;
-            fx += coef * (tab[0] * w00 + tab[ION_SUM * 2] * w10 + tab[2] * w01 + tab[ION_SUM * 2 + 2] * w11);
-            fy += coef * (tab[1] * w00 + tab[ION_SUM * 2 + 1] * w10 + tab[3] * w01 + tab[ION_SUM * 2 + 3] * w11);
-        }
-        // Pairs: sum g(d^2) d over the neighbours within the cutoff, times k q q at
-        // the end. The electron itself contributes g(0) * 0 = 0. Neighbour cells
-        // entirely beyond the cutoff from this electron are skipped.
-        float sx = 0.0f, sy = 0.0f;
-        const float cut2 = S.cutoff2, gsc = S.g_scale;
-        for (int cc = c_lo; cc <= c_hi; cc++) {
-            const float ox = (float)(hc - cc);
-            const float gapx = ox > 0.0f ? ox + px - 1.0f : (ox < 0.0f ? -ox - px : 0.0f);
-            for (int rr = r_lo; rr <= r_hi; rr++) {
-                const float oy = (float)(hr - rr);
-                if (!uniform) {
-                    const float dens = S.density[cc * S.rows + rr];
-                    if (dens > 0.0f) {
-                        const float u = (ox + (px - 0.5f) + S.half) * S.square_samples;
-                        const float w = (oy + (py - 0.5f) + S.half) * S.square_samples;
-                        const float coef = S.k_ion_q * dens;
-                        fx += coef * square_trunc(S, 1, u, w);
-                        fy += coef * square_trunc(S, 2, u, w);
-                    }
-                }
-                const float gapy = oy > 0.0f ? oy + py - 1.0f : (oy < 0.0f ? -oy - py : 0.0f);
-                if (gapx * gapx + gapy * gapy >= cut2 * 1.0001f) continue;
-                const float bx = ox + px, by = oy + py;
-                int j, e;
-                nb.range(cc, rr, j, e);
-                for (; j < e; j++) {
-                    const float2 q = nb.p(j);
-                    const float dx = bx - q.x, dy = by - q.y;
-                    const float d2 = dx * dx + dy * dy;
-                    const float tt = fminf(d2 * gsc, (float)PAIR_SAMPLES);
-                    const int si = (int)tt;
-                    const float g0 = gt[si], g1 = gt[si + 1];
-                    float l = g0 + (g1 - g0) * (tt - (float)si);
-                    l = d2 < cut2 ? l : 0.0f;
-                    sx += l * dx;
-                    sy += l * dy;
-                }
-            }
-        }
-        fx += S.k_pair * sx;
-        fy += S.k_pair * sy;
-        ax += fx * S.inv_mass;
-        ay += fy * S.inv_mass;
-    }
-    // kick (physics.rs:57-60), then at most max_step (physics.rs:151-155).
-    const float scale = 1.0f / (1.0f + info.damping * h);
+    ax = S.qm * (ex + vy * bz);
+    ay = S.qm * (ey - vx * bz);
+}
+
+// Close-range force of the ions of one conductive cell at offset (ox, oy) =
+// home - cell on an electron at fractions (px, py) (close_range.rs:335-350).
+__device__ __forceinline__ void ion_cell(const DevScene &S, float ox, float oy, float px, float py, float dens,
+                                         float &fx, float &fy) {
+    const float u = (ox + (px - 0.5f) + S.half) * S.square_samples;
+    const float w = (oy + (py - 0.5f) + S.half) * S.square_samples;
+    const float coef = S.k_ion_q * dens;
+    fx += coef * square_trunc(S, 1, u, w);
+    fy += coef * square_trunc(S, 2, u, w);
+}
+
+// The same summed over a whole window of equal-density cells: every neighbour
+// cell shares the electron's fractional table offset, so the sum of their
+// bilinear lookups is one lookup in the table summed over the cell offsets.
+__device__ __forceinline__ void ion_uniform(const DevScene &S, int home, float px, float py, float &fx, float &fy) {
+    const float u = (px - 0.5f + S.half) * S.square_samples, w = (py - 0.5f + S.half) * S.square_samples;
+    const int a = (int)u, bb = (int)w;
+    const float fu = u - (float)a, fv = w - (float)bb;
+    const float *tab = S.ion_sum + ((size_t)(a - S.ion_a0) * ION_SUM + (bb - S.ion_a0)) * 2;
+    const float coef = S.k_ion_q * S.density[home];
+    const float w00 = (1.0f - fu) * (1.0f - fv), w10 = fu * (1.0f - fv), w01 = (1.0f - fu) * fv, w11 = fu * fv;
+    fx += coef * (tab[0] * w00 + tab[ION_SUM * 2] * w10 + tab[2] * w01 + tab[ION_SUM * 2 + 2] * w11);
+    fy += coef * (tab[1] * w00 + tab[ION_SUM * 2 + 1] * w10 + tab[3] * w01 + tab[ION_SUM * 2 + 3] * w11);
+}
+
+// One pair term g(r^2) d, g lerped from the r^2-uniform table
+// (close_range.rs:359-372). The offset (dx, dy) is in pair units, so its square
+// is the table coordinate r^2 * PAIR_SAMPLES / cutoff^2. A pair at or beyond the
+// cutoff reads the two zero samples past the table's end, and the electron
+// itself (d = 0) adds g(0) * 0: both contribute exactly zero.
+__device__ __forceinline__ void pair_term(const float *gt, float dx, float dy, float &sx, float &sy) {
+    const float d2 = dx * dx + dy * dy;
+    const float tt = d2 < (float)PAIR_SAMPLES ? d2 : (float)(PAIR_SAMPLES + 2);
+    const int si = (int)tt;
+    const float g0 = gt[si], g1 = gt[si + 1];
+    const float l = g0 + (g1 - g0) * __builtin_amdgcn_fractf(tt);
+    sx += l * dx;
+    sy += l * dy;
+}
+
+// kick (physics.rs:57-60), the max_step clamp (physics.rs:151-155) and
+// confinement (physics.rs:66-79: x first, then y from the resulting column);
+// writes the electron's new cell, fractions and velocity.
+__device__ __forceinline__ void finish(const DevScene &S, int id, int hc, int hr, float px, float py, float vx, float vy,
+                                       float damping, float ax, float ay, float h) {
+    const float scale = 1.0f / (1.0f + damping * h);
     float nvx = (vx + ax * h) * scale, nvy = (vy + ay * h) * scale;
     float mx = nvx * h, my = nvy * h;
     const float len = sqrtf(mx * mx + my * my);
     const float limit = fminf(S.max_step / fmaxf(len, 1e-30f), 1.0f);
     mx *= limit;
     my *= limit;
-    // confine (physics.rs:66-79): x first, then y from the resulting column.
     float tx = px + mx, ty = py + my;
     int cx = hc, cy = hr;
     resolve(S, 0, hc, hr, tx, nvx);
@@ -521,25 +506,109 @@ __device__ void push_core(const DevScene &S, const NB &nb, const float *gt, int 
     S.vy[id] = nvy;
 }
 
+// One electron's substep (physics.rs:139-158), one thread per electron: `slot`
+// is its place in the sorted snapshot.
+template <class NB>
+__device__ void push_core(const DevScene &S, const NB &nb, const float *gt, int slot, int k, float h, uint32_t t) {
+    const int id = S.order[slot];
+    const int home = S.scell[slot];
+    const int hc = home / S.rows, hr = home - (home / S.rows) * S.rows;
+    const float px = S.sfx[slot], py = S.sfy[slot];   // fractions within the home cell
+    const float vx = S.svx[slot], vy = S.svy[slot];
+    const CellInfo info = S.info[home];
+    float ax, ay;
+    field_accel(S, hc, hr, px, py, vx, vy, info.flags, k, h, t, ax, ay);
+    if (S.close_on) {
+        // CloseRange::force (close_range.rs:314-376), every pair seen from both sides.
+        float fx = 0.0f, fy = 0.0f;
+        const int R = S.reach;
+        const int c_lo = max(hc - R, 0), c_hi = min(hc + R, S.cols - 1);
+        const int r_lo = max(hr - R, 0), r_hi = min(hr + R, S.rows - 1);
+        const bool uniform = (info.flags & F_ION_UNIFORM) != 0u;
+        if (uniform) ion_uniform(S, home, px, py, fx, fy);
+        // Neighbour cells entirely beyond the cutoff from this electron are skipped.
+        float sx = 0.0f, sy = 0.0f;
+        const float cut2 = S.cutoff2;
+        for (int cc = c_lo; cc <= c_hi; cc++) {
+            const float ox = (float)(hc - cc);
+            const float gapx = ox > 0.0f ? ox + px - 1.0f : (ox < 0.0f ? -ox - px : 0.0f);
+            for (int rr = r_lo; rr <= r_hi; rr++) {
+                const float oy = (float)(hr - rr);
+                if (!uniform) {
+                    const float dens = S.density[cc * S.rows + rr];
+                    if (dens > 0.0f) ion_cell(S, ox, oy, px, py, dens, fx, fy);
+                }
+                const float gapy = oy > 0.0f ? oy + py - 1.0f : (oy < 0.0f ? -oy - py : 0.0f);
+                if (gapx * gapx + gapy * gapy >= cut2 * 1.0001f) continue;
+                const float2 o = nb.origin(hc, hr, px, py, cc, rr);
+                int j, e;
+                nb.range(cc, rr, j, e);
+                for (; j < e; j++) {
+                    const float2 q = nb.p(j);
+                    pair_term(gt, o.x - q.x, o.y - q.y, sx, sy);
+                }
+            }
+        }
+        fx += S.k_pair_u * sx;
+        fy += S.k_pair_u * sy;
+        ax += fx * S.inv_mass;
+        ay += fy * S.inv_mass;
+    }
+    finish(S, id, hc, hr, px, py, vx, vy, info.damping, ax, ay, h);
+}
+
 __device__ void push_one(const DevScene &S, int slot, int k, float h, uint32_t t) {
     push_core(S, GlobalNeighbours{S}, S.g_table, slot, k, h, t);
 }
 
-// Substep k for every electron, per PUSH_TILE x PUSH_TILE tile of cells: the
+// Inclusive prefix sum over the 64 lanes of a wave.
+__device__ __forceinline__ int wave_scan_incl(int v) {
+    const int lane = threadIdx.x & 63;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int u = __shfl_up(v, o, 64);
+        if (lane >= o) v += u;
+    }
+    return v;
+}
+
+// Lanes per electron in the cell mode: the cell's count rounded up to a power of
+// two, at most a wave.
+__device__ __forceinline__ int lanes_per(int n) {
+    int i = 1;
+    while (i < n && i < 64) i <<= 1;
+    return i;
+}
+
+// Substep k for every electron, per PUSH_TILE x PUSH_TILE tile of cells. The
 // positions of the tile's neighbourhood (reach cells around it) are staged in
-// LDS, so the ~(2 reach + 1)^2 x carriers pair candidates of each electron are
-// read from LDS. A tile whose window holds more than PUSH_STAGE electrons reads
-// them from global memory instead.
+// LDS relative to the tile's centre. Two ways to run the pair sums:
+//   * cell mode (tiles with many electrons per occupied cell): a wave takes up
+//     to I electrons of one home cell, I a power of two, and splits the
+//     window's (2 reach + 1) column runs between 64 / I lanes per electron, so
+//     every lane of the wave walks the same neighbour list without divergence;
+//     the partial sums meet in a fixed butterfly;
+//   * electron mode (sparse tiles): one thread per electron with per-cell
+//     culling against the cutoff.
+// A tile whose window holds more than PUSH_STAGE electrons reads them from
+// global memory in electron mode. Both modes sum in a fixed order, so a scene
+// gives the same bits alone or batched.
 constexpr int PUSH_WIN = PUSH_TILE + 2 * 4;   // window side for reach <= 4
 constexpr int PUSH_STAGE = 3328;
+constexpr int PUSH_CELLS = PUSH_TILE * PUSH_TILE;
+constexpr int CELL_MODE_MIN = 6;              // electrons per occupied cell for the cell mode
+constexpr int PUSH_THREADS = 512;             // 8 waves: two blocks fill a CU's 16 wave slots at <= 128 VGPRs
+constexpr int PUSH_WAVES = PUSH_THREADS / 64;
 
-__global__ __launch_bounds__(256) void k_push_tiled(Batch b, int k) {
+__global__ __launch_bounds__(PUSH_THREADS) void k_push_tiled(Batch b, int k) {
     __shared__ float2 lp[PUSH_STAGE];
-    __shared__ float gt[PAIR_SAMPLES + 2];
-    bool table_loaded = false;
+    __shared__ float gt[PAIR_SAMPLES + 4];
     __shared__ int lb[PUSH_WIN * PUSH_WIN], le[PUSH_WIN * PUSH_WIN];
     __shared__ int col_base[PUSH_WIN + 1], col_g0[PUSH_WIN];   // LDS base and first global slot per column
     __shared__ int own_pref[PUSH_TILE + 1];                     // prefix of the tile's electrons per column
+    __shared__ int item_pref[PUSH_CELLS + 1];                   // cell mode: prefix of work items per tile cell
+    __shared__ int cell_mode;
+    bool table_loaded = false;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     for (int blk = blockIdx.x; blk < b.total[KC_PUSH]; blk += gridDim.x) {
         const int s = scene_of(b, KC_PUSH, blk);
         const DevScene &S = b.scenes[s];
@@ -547,45 +616,58 @@ __global__ __launch_bounds__(256) void k_push_tiled(Batch b, int k) {
         const int sub = substeps_of(S, t);
         if (k >= sub) continue;
         const float h = S.dt / (float)sub;
-        if (k == 0 && threadIdx.x == 0 && block_in_scene(b, KC_PUSH, s, blk) == 0) S.ctr->substeps = sub;
         const int local = block_in_scene(b, KC_PUSH, s, blk);
+        if (k == 0 && threadIdx.x == 0 && local == 0) S.ctr->substeps = sub;
         if (S.close_on && !table_loaded) {   // the pair table is shared by every scene
-            for (int q = threadIdx.x; q < PAIR_SAMPLES + 2; q += 256) gt[q] = S.g_table[q];
+            for (int q = threadIdx.x; q < PAIR_SAMPLES + 4; q += PUSH_THREADS) gt[q] = S.g_table[q];
             table_loaded = true;
         }
         const int tiles_r = (S.rows + PUSH_TILE - 1) / PUSH_TILE;
         const int tc0 = (local / tiles_r) * PUSH_TILE, tr0 = (local % tiles_r) * PUSH_TILE;
         const int tc1 = min(tc0 + PUSH_TILE, S.cols), tr1 = min(tr0 + PUSH_TILE, S.rows);
+        const int cmx = tc0 + PUSH_TILE / 2, cmy = tr0 + PUSH_TILE / 2;
         const int R = S.close_on ? S.reach : 0;
         const int W = PUSH_TILE + 2 * R;
         const int wc0 = tc0 - R, wr0 = tr0 - R;
         const int rlo = max(wr0, 0), rhi = min(wr0 + W, S.rows) - 1;
-        // Column ranges of the window (each column's cells are contiguous in the sorted order).
-        if (threadIdx.x == 0) {
-            int base = 0;
-            for (int q = 0; q < W; q++) {
-                const int cc = wc0 + q;
-                col_base[q] = base;
-                col_g0[q] = 0;
-                if (cc >= 0 && cc < S.cols && rlo <= rhi) {
-                    const int g0 = S.cstart[cc * S.rows + rlo], g1 = S.cstart[cc * S.rows + rhi + 1];
-                    col_g0[q] = g0;
-                    base += g1 - g0;
-                }
+        // Wave 0 lays out the window (column runs, lane = column) and the tile's
+        // work (lane = tile cell, column-major).
+        if (wave == 0) {
+            const int cc = wc0 + lane;
+            int g0 = 0, n = 0;
+            if (lane < W && cc >= 0 && cc < S.cols && rlo <= rhi) {
+                g0 = S.cstart[cc * S.rows + rlo];
+                n = S.cstart[cc * S.rows + rhi + 1] - g0;
             }
-            col_base[W] = base;
-            int own = 0;
-            for (int q = 0; q < PUSH_TILE; q++) {
-                own_pref[q] = own;
-                const int cc = tc0 + q;
-                if (cc < tc1) own += S.cstart[cc * S.rows + tr1 - 1 + 1] - S.cstart[cc * S.rows + tr0];
+            const int incl = wave_scan_incl(n);
+            if (lane < W) {
+                col_base[lane] = incl - n;
+                col_g0[lane] = g0;
             }
-            own_pref[PUSH_TILE] = own;
+            if (lane == W - 1) col_base[W] = incl;
+            const int hc = tc0 + lane / PUSH_TILE, hr = tr0 + lane % PUSH_TILE;
+            int m = 0;
+            if (hc < tc1 && hr < tr1) {
+                const int cell = hc * S.rows + hr;
+                m = S.cstart[cell + 1] - S.cstart[cell];
+            }
+            const int I = lanes_per(m);
+            const int items = m > 0 ? (m + I - 1) / I : 0;
+            const int item_incl = wave_scan_incl(items);
+            const int own_incl = wave_scan_incl(m);
+            const int occupied = wave_scan_incl(m > 0 ? 1 : 0);
+            item_pref[lane] = item_incl - items;
+            if (lane % PUSH_TILE == 0) own_pref[lane / PUSH_TILE] = own_incl - m;
+            if (lane == 63) {
+                item_pref[PUSH_CELLS] = item_incl;
+                own_pref[PUSH_TILE] = own_incl;
+                cell_mode = own_incl >= CELL_MODE_MIN * occupied ? 1 : 0;
+            }
         }
         __syncthreads();
         const bool staged = col_base[W] <= PUSH_STAGE;
         if (staged) {
-            for (int q = threadIdx.x; q < W * W; q += 256) {
+            for (int q = threadIdx.x; q < W * W; q += PUSH_THREADS) {
                 const int ci = q / W, ri = q - (q / W) * W;
                 const int cc = wc0 + ci, rr = wr0 + ri;
                 if (cc < 0 || cc >= S.cols || rr < 0 || rr >= S.rows) {
@@ -599,19 +681,99 @@ __global__ __launch_bounds__(256) void k_push_tiled(Batch b, int k) {
             }
             for (int q = 0; q < W; q++) {
                 const int n = col_base[q + 1] - col_base[q];
-                for (int j = threadIdx.x; j < n; j += 256)
-                    lp[col_base[q] + j] = make_float2(S.sfx[col_g0[q] + j], S.sfy[col_g0[q] + j]);
+                const float ox = (float)(wc0 + q - cmx), unit = S.pair_unit;
+                for (int j = threadIdx.x; j < n; j += PUSH_THREADS) {
+                    const int g = col_g0[q] + j;
+                    const int rr = S.scell[g] - (wc0 + q) * S.rows;
+                    lp[col_base[q] + j] = make_float2((ox + S.sfx[g]) * unit, ((float)(rr - cmy) + S.sfy[g]) * unit);
+                }
             }
         }
         __syncthreads();
-        const int n_own = own_pref[PUSH_TILE];
-        for (int e = threadIdx.x; e < n_own; e += 256) {
-            int q = 0;
-            while (own_pref[q + 1] <= e) q++;
-            const int cc = tc0 + q;
-            const int slot = S.cstart[cc * S.rows + tr0] + (e - own_pref[q]);
-            if (staged) push_core(S, LdsNeighbours{lp, lb, le, wc0, wr0, W}, gt, slot, k, h, t);
-            else push_core(S, GlobalNeighbours{S}, gt, slot, k, h, t);
+        if (staged && S.close_on && cell_mode) {
+            const float unit = S.pair_unit;
+            const int n_items = item_pref[PUSH_CELLS];
+            for (int it = wave; it < n_items; it += PUSH_WAVES) {
+                int lo = 0, hi = PUSH_CELLS;
+                while (hi - lo > 1) {
+                    const int mid = (lo + hi) >> 1;
+                    if (item_pref[mid] <= it) lo = mid;
+                    else hi = mid;
+                }
+                lo = __builtin_amdgcn_readfirstlane(lo);
+                const int hc = tc0 + lo / PUSH_TILE, hr = tr0 + lo % PUSH_TILE;
+                const int home = hc * S.rows + hr;
+                const int e0 = __builtin_amdgcn_readfirstlane(S.cstart[home]);
+                const int n = __builtin_amdgcn_readfirstlane(S.cstart[home + 1]) - e0;
+                const int I = lanes_per(n), shift = __builtin_ctz(I), SL = 64 >> shift;
+                const int i = lane & (I - 1), sl = lane >> shift;
+                const int ei = (it - item_pref[lo]) * I + i;
+                const bool live = ei < n;
+                const int slot = e0 + (live ? ei : n - 1);
+                const float px = S.sfx[slot], py = S.sfy[slot];
+                const float bx = ((float)(hc - cmx) + px) * unit, by = ((float)(hr - cmy) + py) * unit;
+                const int c_lo = max(hc - R, 0), c_hi = min(hc + R, S.cols - 1);
+                const int r_lo = max(hr - R, 0), r_hi = min(hr + R, S.rows - 1);
+                float sx = 0.0f, sy = 0.0f, fx = 0.0f, fy = 0.0f;
+                for (int cc = c_lo; cc <= c_hi; cc++) {
+                    const int wcol = (cc - wc0) * W;
+                    const int jb = __builtin_amdgcn_readfirstlane(lb[wcol + r_lo - wr0]);
+                    const int je = __builtin_amdgcn_readfirstlane(le[wcol + r_hi - wr0]);
+                    const float2 *lq = lp + sl;
+                    int jj = jb;
+                    for (; jj + 4 * SL <= je; jj += 4 * SL) {
+                        const float2 q0 = lq[jj], q1 = lq[jj + SL], q2 = lq[jj + 2 * SL], q3 = lq[jj + 3 * SL];
+                        pair_term(gt, bx - q0.x, by - q0.y, sx, sy);
+                        pair_term(gt, bx - q1.x, by - q1.y, sx, sy);
+                        pair_term(gt, bx - q2.x, by - q2.y, sx, sy);
+                        pair_term(gt, bx - q3.x, by - q3.y, sx, sy);
+                    }
+                    for (; jj < je; jj += SL) {
+                        // past the run's end a lane pairs with itself: d = 0 adds nothing
+                        const float2 q = lp[min(jj + sl, je - 1)];
+                        const bool in = jj + sl < je;
+                        pair_term(gt, in ? bx - q.x : 0.0f, in ? by - q.y : 0.0f, sx, sy);
+                    }
+                }
+                const CellInfo info = S.info[home];
+                const bool uniform = (info.flags & F_ION_UNIFORM) != 0u;
+                if (!uniform) {
+                    const int nr = r_hi - r_lo + 1, cells = (c_hi - c_lo + 1) * nr;
+                    for (int w = sl; w < cells; w += SL) {
+                        const int cc = c_lo + w / nr, rr = r_lo + w % nr;
+                        const float dens = S.density[cc * S.rows + rr];
+                        if (dens > 0.0f) ion_cell(S, (float)(hc - cc), (float)(hr - rr), px, py, dens, fx, fy);
+                    }
+                }
+                for (int off = I; off < 64; off <<= 1) {
+                    sx += __shfl_xor(sx, off, 64);
+                    sy += __shfl_xor(sy, off, 64);
+                    fx += __shfl_xor(fx, off, 64);
+                    fy += __shfl_xor(fy, off, 64);
+                }
+                if (sl == 0 && live) {
+                    const float vx = S.svx[slot], vy = S.svy[slot];
+                    if (uniform) ion_uniform(S, home, px, py, fx, fy);
+                    fx += S.k_pair_u * sx;
+                    fy += S.k_pair_u * sy;
+                    float ax, ay;
+                    field_accel(S, hc, hr, px, py, vx, vy, info.flags, k, h, t, ax, ay);
+                    ax += fx * S.inv_mass;
+                    ay += fy * S.inv_mass;
+                    finish(S, S.order[slot], hc, hr, px, py, vx, vy, info.damping, ax, ay, h);
+                }
+            }
+        } else {
+            const int n_own = own_pref[PUSH_TILE];
+            for (int e = threadIdx.x; e < n_own; e += PUSH_THREADS) {
+                int q = 0;
+                while (own_pref[q + 1] <= e) q++;
+                const int cc = tc0 + q;
+                const int slot = S.cstart[cc * S.rows + tr0] + (e - own_pref[q]);
+                if (staged)
+                    push_core(S, LdsNeighbours{lp, lb, le, wc0, wr0, W, cmx, cmy, S.pair_unit}, gt, slot, k, h, t);
+                else push_core(S, GlobalNeighbours{S}, gt, slot, k, h, t);
+            }
         }
         __syncthreads();
     }
@@ -1082,7 +1244,7 @@ hipError_t tick_kernels(const Batch &b, int fdtd_substeps, hipStream_t stream, h
     EMB_MARK(0);
     EMB_LAUNCH(k_trim, KC_STACKS, 1024, b);
     EMB_MARK(1);
-    EMB_LAUNCH(k_push_tiled, KC_PUSH, 256, b, 0);
+    EMB_LAUNCH(k_push_tiled, KC_PUSH, PUSH_THREADS, b, 0);
     EMB_MARK(2);
     if (b.total[KC_ELECTRONS] > 0) EMB_LAUNCH(k_rest, KC_SCENE, 1024, b);
     EMB_MARK(3);

@@ -1,0 +1,169 @@
+// poisson.hip - the DST-I Poisson solve on its own, for one nx x ny node grid.
+//
+// Maxwell::poisson (crates/embloks-core/src/maxwell.rs:139-168): with Dirichlet
+// boundaries (phi = 0 outside the grid), phi = P(r) solves the 5-point
+// Laplacian lap(phi) = r exactly, through
+//   P(r) = Sx ((Sx r Sy) * inv_lap) Sy,
+//   S[k][n] = sin(pi (k+1)(n+1) / (N+1)),
+//   inv_lap[i][j] = [4 / ((nx+1)(ny+1))] / (lx_i + ly_j),  l_k = 2 cos(pi (k+1) / (N+1)) - 2
+// (maxwell.rs:56-85). The tick runs the same products (gemm.hip) on its
+// scenes; this entry point runs them on a caller's right-hand side, in FP32 or
+// BF16x3 MFMA, and times them with HIP events on its own stream.
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "gemm.hpp"
+#include "runtime.hpp"
+
+struct EmbPoisson {
+    EmbDevice *dev = nullptr;
+    int nx = 0, ny = 0;
+    hipStream_t stream = nullptr;
+    float *rhs = nullptr, *w1 = nullptr, *w2 = nullptr, *phi = nullptr, *inv_lap = nullptr;
+    const float *sx = nullptr, *sy = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+};
+
+namespace emb {
+namespace {
+
+hipError_t enqueue_solve(const EmbPoisson &p, int precision, int tile) {
+    const int nx = p.nx, ny = p.ny;
+    GemmArgs g;
+    g.M = nx;
+    g.N = ny;
+    g.ldc = ny;
+    // stage 0: w1 = Sx . rhs
+    g.A = p.sx; g.lda = nx; g.B = p.rhs; g.ldb = ny; g.K = nx; g.C = p.w1; g.scale = nullptr;
+    hipError_t e = launch_gemm(g, precision, tile, p.stream);
+    if (e != hipSuccess) return e;
+    // stage 1: w2 = (w1 . Sy) * inv_lap
+    g.A = p.w1; g.lda = ny; g.B = p.sy; g.ldb = ny; g.K = ny; g.C = p.w2; g.scale = p.inv_lap;
+    if ((e = launch_gemm(g, precision, tile, p.stream)) != hipSuccess) return e;
+    // stage 2: w1 = Sx . w2
+    g.A = p.sx; g.lda = nx; g.B = p.w2; g.ldb = ny; g.K = nx; g.C = p.w1; g.scale = nullptr;
+    if ((e = launch_gemm(g, precision, tile, p.stream)) != hipSuccess) return e;
+    // stage 3: phi = w1 . Sy
+    g.A = p.w1; g.lda = ny; g.B = p.sy; g.ldb = ny; g.K = ny; g.C = p.phi; g.scale = nullptr;
+    return launch_gemm(g, precision, tile, p.stream);
+}
+
+}  // namespace
+}  // namespace emb
+
+using emb::fail;
+
+extern "C" {
+
+int emb_poisson_create(EmbDevice *dev, int32_t nx, int32_t ny, EmbPoisson **out) {
+    EMB_GUARD({
+        if (!dev || !out) return fail(EMB_E_INVALID, "null argument");
+        if (nx < 1 || ny < 1 || nx > 32768 || ny > 32768) return fail(EMB_E_INVALID, "grid must be 1..32768 nodes per side");
+        EMB_HIP(hipSetDevice(dev->ordinal));
+        auto *p = new EmbPoisson();
+        p->dev = dev;
+        p->nx = nx;
+        p->ny = ny;
+        auto bail = [&](int rc) {
+            (void)hipFree(p->rhs);
+            (void)hipFree(p->w1);
+            (void)hipFree(p->w2);
+            (void)hipFree(p->phi);
+            (void)hipFree(p->inv_lap);
+            if (p->e0) (void)hipEventDestroy(p->e0);
+            if (p->e1) (void)hipEventDestroy(p->e1);
+            if (p->stream) (void)hipStreamDestroy(p->stream);
+            delete p;
+            return rc;
+        };
+        const size_t nodes = (size_t)nx * ny;
+        for (float **a : {&p->rhs, &p->w1, &p->w2, &p->phi, &p->inv_lap})
+            if (hipMalloc(a, nodes * sizeof(float)) != hipSuccess) return bail(fail(EMB_E_NOMEM, "device memory"));
+        if (hipStreamCreate(&p->stream) != hipSuccess || hipEventCreate(&p->e0) != hipSuccess ||
+            hipEventCreate(&p->e1) != hipSuccess)
+            return bail(fail(EMB_E_DEVICE, "stream or event creation failed"));
+        int rc = emb::device_sine(dev, nx, &p->sx);
+        if (rc == EMB_OK) rc = emb::device_sine(dev, ny, &p->sy);
+        if (rc != EMB_OK) return bail(rc);
+        // Inverse Laplacian with the transforms' normalisation (maxwell.rs:60-67), FP64 -> FP32.
+        std::vector<double> kx(nx), ky(ny);
+        const double pi = 3.14159265358979323846;
+        for (int k = 1; k <= nx; k++) kx[k - 1] = 2.0 * std::cos(pi * (double)k / (double)(nx + 1)) - 2.0;
+        for (int k = 1; k <= ny; k++) ky[k - 1] = 2.0 * std::cos(pi * (double)k / (double)(ny + 1)) - 2.0;
+        const double norm = 4.0 / ((double)(nx + 1) * (double)(ny + 1));
+        std::vector<float> il(nodes);
+        for (int i = 0; i < nx; i++)
+            for (int j = 0; j < ny; j++) il[(size_t)i * ny + j] = (float)(norm / (kx[i] + ky[j]));
+        if (hipMemcpy(p->inv_lap, il.data(), nodes * sizeof(float), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemset(p->rhs, 0, nodes * sizeof(float)) != hipSuccess)
+            return bail(fail(EMB_E_DEVICE, "upload failed"));
+        *out = p;
+        return EMB_OK;
+    })
+}
+
+int emb_poisson_set_rhs(EmbPoisson *p, const double *rhs) {
+    EMB_GUARD({
+        if (!p || !rhs) return fail(EMB_E_INVALID, "null argument");
+        const size_t nodes = (size_t)p->nx * p->ny;
+        std::vector<float> h(nodes);
+        for (size_t i = 0; i < nodes; i++) h[i] = (float)rhs[i];
+        EMB_HIP(hipSetDevice(p->dev->ordinal));
+        EMB_HIP(hipMemcpyAsync(p->rhs, h.data(), nodes * sizeof(float), hipMemcpyHostToDevice, p->stream));
+        EMB_HIP(hipStreamSynchronize(p->stream));
+        return EMB_OK;
+    })
+}
+
+int emb_poisson_solve(EmbPoisson *p, int32_t precision, int32_t tile, int32_t repeats, double *avg_ms) {
+    EMB_GUARD({
+        if (!p) return fail(EMB_E_INVALID, "null argument");
+        if (precision != EMB_DST_FP32 && precision != EMB_DST_BF16X3) return fail(EMB_E_INVALID, "unknown precision");
+        if (tile != 0 && tile != 64 && tile != 128) return fail(EMB_E_INVALID, "tile must be 0, 64 or 128");
+        if (repeats < 1) return fail(EMB_E_INVALID, "repeats must be >= 1");
+        if (tile == 0) tile = (long long)p->nx * p->ny >= (1LL << 20) ? 128 : 64;
+        const int prec = precision == EMB_DST_BF16X3 ? emb::GEMM_BF16X3 : emb::GEMM_FP32;
+        EMB_HIP(hipSetDevice(p->dev->ordinal));
+        EMB_HIP(hipEventRecord(p->e0, p->stream));
+        for (int r = 0; r < repeats; r++) EMB_HIP(emb::enqueue_solve(*p, prec, tile));
+        EMB_HIP(hipEventRecord(p->e1, p->stream));
+        EMB_HIP(hipEventSynchronize(p->e1));
+        if (avg_ms) {
+            float ms = 0.0f;
+            EMB_HIP(hipEventElapsedTime(&ms, p->e0, p->e1));
+            *avg_ms = (double)ms / repeats;
+        }
+        return EMB_OK;
+    })
+}
+
+int emb_poisson_read(EmbPoisson *p, double *phi) {
+    EMB_GUARD({
+        if (!p || !phi) return fail(EMB_E_INVALID, "null argument");
+        const size_t nodes = (size_t)p->nx * p->ny;
+        std::vector<float> h(nodes);
+        EMB_HIP(hipSetDevice(p->dev->ordinal));
+        EMB_HIP(hipMemcpyAsync(h.data(), p->phi, nodes * sizeof(float), hipMemcpyDeviceToHost, p->stream));
+        EMB_HIP(hipStreamSynchronize(p->stream));
+        for (size_t i = 0; i < nodes; i++) phi[i] = (double)h[i];
+        return EMB_OK;
+    })
+}
+
+int emb_poisson_destroy(EmbPoisson *p) {
+    EMB_GUARD({
+        if (!p) return EMB_OK;
+        (void)hipSetDevice(p->dev->ordinal);
+        if (p->stream) (void)hipStreamSynchronize(p->stream);
+        for (float *a : {p->rhs, p->w1, p->w2, p->phi, p->inv_lap}) (void)hipFree(a);
+        if (p->e0) (void)hipEventDestroy(p->e0);
+        if (p->e1) (void)hipEventDestroy(p->e1);
+        if (p->stream) (void)hipStreamDestroy(p->stream);
+        delete p;
+        return EMB_OK;
+    })
+}
+
+}  // extern "C"

@@ -106,19 +106,27 @@ template <class T> hipError_t put(const T *dst, const std::vector<T> &src, hipSt
     return put(const_cast<T *>(dst), src, st);
 }
 
-int shared_tables(EmbDevice *dev, const EmbSettings &s, int nx, int ny, const float **sx, const float **sy,
-                  DeviceTables &tab) {
-    std::lock_guard<std::mutex> guard(dev->lock);
-    for (int n : {nx, ny}) {
-        if (dev->sine.count(n)) continue;
+// The DST-I matrix of length n on the device, built once per device; the
+// caller holds dev->lock.
+int sine_locked(EmbDevice *dev, int n, const float **out) {
+    auto it = dev->sine.find(n);
+    if (it == dev->sine.end()) {
         std::vector<float> m = sine_matrix(n);
         float *p = nullptr;
         EMB_HIP(hipMalloc(&p, m.size() * sizeof(float)));
         EMB_HIP(hipMemcpy(p, m.data(), m.size() * sizeof(float), hipMemcpyHostToDevice));
-        dev->sine[n] = p;
+        it = dev->sine.emplace(n, p).first;
     }
-    *sx = dev->sine[nx];
-    *sy = dev->sine[ny];
+    *out = it->second;
+    return EMB_OK;
+}
+
+int shared_tables(EmbDevice *dev, const EmbSettings &s, int nx, int ny, const float **sx, const float **sy,
+                  DeviceTables &tab) {
+    std::lock_guard<std::mutex> guard(dev->lock);
+    int rc = sine_locked(dev, nx, sx);
+    if (rc == EMB_OK) rc = sine_locked(dev, ny, sy);
+    if (rc != EMB_OK) return rc;
     const auto key = std::make_tuple(s.electron_softening, (int)s.smoothing_passes, s.electron_cutoff);
     auto it = dev->tables.find(key);
     if (it == dev->tables.end()) {
@@ -126,6 +134,8 @@ int shared_tables(EmbDevice *dev, const EmbSettings &s, int nx, int ny, const fl
         if (s.electron_cutoff > 0.0) {
             auto h = host_tables(s.electron_softening, s.smoothing_passes, s.electron_cutoff);
             std::vector<float> g(h->g.begin(), h->g.end()), sq(h->square.begin(), h->square.end());
+            g.push_back(0.0f);   // two zero samples past the end: where pairs beyond the cutoff read
+            g.push_back(0.0f);
             EMB_HIP(hipMalloc(&t.g, g.size() * sizeof(float)));
             EMB_HIP(hipMalloc(&t.square, sq.size() * sizeof(float)));
             EMB_HIP(hipMemcpy(t.g, g.data(), g.size() * sizeof(float), hipMemcpyHostToDevice));
@@ -149,6 +159,11 @@ int shared_tables(EmbDevice *dev, const EmbSettings &s, int nx, int ny, const fl
 }
 
 }  // namespace
+
+int device_sine(EmbDevice *dev, int n, const float **out) {
+    std::lock_guard<std::mutex> guard(dev->lock);
+    return sine_locked(dev, n, out);
+}
 
 size_t scene_bytes(const EmbSettings &s, int cols, int rows, int64_t n, int n_stacks, int n_stack_cells, int n_bats,
                    int n_ac) {
@@ -235,6 +250,8 @@ int make_resident(EmbDevice *dev, const EmbSettings &s, HostScene &&hs_in, hipSt
     d.k_q = (float)(k * q);
     d.cutoff2 = (float)(s.electron_cutoff * s.electron_cutoff);
     d.g_scale = s.electron_cutoff > 0.0 ? (float)(4096.0 / (s.electron_cutoff * s.electron_cutoff)) : 0.0f;
+    d.pair_unit = s.electron_cutoff > 0.0 ? (float)(64.0 / s.electron_cutoff) : 0.0f;
+    d.k_pair_u = s.electron_cutoff > 0.0 ? (float)(k * q * q * s.electron_cutoff / 64.0) : 0.0f;
     d.half = (float)tables.half;
     d.square_samples = 32.0f;
     d.c2h = (float)(hs.h_fdtd * s.speed_of_light * s.speed_of_light);
@@ -365,13 +382,6 @@ struct EmbSim {
     Resident r;
     double *diag = nullptr;
 };
-
-#define EMB_GUARD(...)                                                         \
-    try {                                                                      \
-        __VA_ARGS__                                                            \
-    } catch (const std::exception &e) {                                        \
-        return fail(EMB_E_INVALID, std::string("exception: ") + e.what());     \
-    }
 
 extern "C" {
 

@@ -25,6 +25,14 @@ int fail(int code, const std::string &msg);
         if (e_ != hipSuccess) return ::emb::fail(EMB_E_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
+// Body of a C ABI function: C++ exceptions become an error code.
+#define EMB_GUARD(...)                                                         \
+    try {                                                                      \
+        __VA_ARGS__                                                            \
+    } catch (const std::exception &e) {                                        \
+        return ::emb::fail(EMB_E_INVALID, std::string("exception: ") + e.what()); \
+    }
+
 struct DeviceTables {
     float *g = nullptr;
     float *square = nullptr;
@@ -67,6 +75,9 @@ struct Resident {
     int last_substeps = 0;
     long long *dep_dump = nullptr;
 };
+
+// The FP32 DST-I matrix of length n on the device (built once per device).
+int device_sine(EmbDevice *dev, int n, const float **out);
 
 // Bytes a scene of this shape needs (arena only; shared tables excluded).
 size_t scene_bytes(const EmbSettings &s, int cols, int rows, int64_t n, int n_stacks, int n_stack_cells, int n_bats, int n_ac);

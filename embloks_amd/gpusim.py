@@ -51,6 +51,11 @@ SIGNATURES = {
     "emb_server_stats": (C.c_int, [VP, C.POINTER(abi.EmbServerStats)]),
     "emb_server_destroy": (C.c_int, [VP]),
     "emb_scene_footprint": (C.c_int64, [C.POINTER(abi.EmbSettings), C.c_int32, C.c_int32, C.c_int64]),
+    "emb_poisson_create": (C.c_int, [VP, C.c_int32, C.c_int32, C.POINTER(VP)]),
+    "emb_poisson_set_rhs": (C.c_int, [VP, D]),
+    "emb_poisson_solve": (C.c_int, [VP, C.c_int32, C.c_int32, C.c_int32, D]),
+    "emb_poisson_read": (C.c_int, [VP, D]),
+    "emb_poisson_destroy": (C.c_int, [VP]),
 }
 
 _lib = None
@@ -276,3 +281,45 @@ class Server:
 
 def footprint(settings: abi.EmbSettings, cols: int, rows: int, n_electrons: int) -> int:
     return int(lib().emb_scene_footprint(C.byref(settings), cols, rows, n_electrons))
+
+
+class Poisson:
+    """The DST-I Poisson solve on its own (Maxwell::poisson,
+    crates/embloks-core/src/maxwell.rs:139-168) on an nx x ny node grid."""
+
+    FP32, BF16X3 = 0, 1
+
+    def __init__(self, nx: int, ny: int, device: "Device | None" = None):
+        self.device = device or Device.default()
+        self.nx, self.ny = nx, ny
+        h = VP()
+        check(lib().emb_poisson_create(self.device.h, nx, ny, C.byref(h)))
+        self.h = h
+
+    def set_rhs(self, rhs: np.ndarray) -> None:
+        rhs = np.ascontiguousarray(rhs, dtype=np.float64)
+        assert rhs.shape == (self.nx, self.ny)
+        check(lib().emb_poisson_set_rhs(self.h, _p(rhs)))
+
+    def solve(self, precision: int = 0, tile: int = 0, repeats: int = 1) -> float:
+        """Run `repeats` solves; returns the mean milliseconds per solve."""
+        ms = C.c_double()
+        check(lib().emb_poisson_solve(self.h, precision, tile, repeats, C.byref(ms)))
+        return ms.value
+
+    def read(self) -> np.ndarray:
+        out = np.zeros((self.nx, self.ny))
+        check(lib().emb_poisson_read(self.h, _p(out)))
+        return out
+
+    def destroy(self) -> None:
+        if getattr(self, "h", None):
+            check(lib().emb_poisson_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+

@@ -377,3 +377,22 @@ def serving_stream(n_requests: int = 256, mean_gap_s: float = 0.002, min_side: i
                                    seed=seed * 1000003 + k)
         out.append((t, steps, world))
     return out
+
+
+def gaussian_blobs(n: int = 2048, count: int = 200, min_sigma: float = 2.0, max_sigma: float = 40.0,
+                   seed: int = 3) -> np.ndarray:
+    """A charge density on an n x n node grid: `count` Gaussian blobs with sigma
+    uniform in [min_sigma, max_sigma] cells, amplitude normal(0, 1) and centres
+    uniform over the grid, summed (each blob is separable, so it is built as an
+    outer product)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    axis = np.arange(n, dtype=np.float64)
+    rho = np.zeros((n, n))
+    for _ in range(count):
+        sigma = float(rng.uniform(min_sigma, max_sigma))
+        amp = float(rng.normal(0.0, 1.0))
+        cx, cy = rng.uniform(0.0, n, size=2)
+        gx = np.exp(-0.5 * ((axis - cx) / sigma) ** 2)
+        gy = np.exp(-0.5 * ((axis - cy) / sigma) ** 2)
+        rho += amp * np.outer(gx, gy)
+    return rho

@@ -204,6 +204,25 @@ int emb_server_result(EmbServer *srv, int64_t id, double *pos, double *vel, doub
 int emb_server_stats(EmbServer *srv, EmbServerStats *out);
 int emb_server_destroy(EmbServer *srv);
 
+/* --- The DST-I Poisson solve on its own ----------------------------------------
+ * Replaces Maxwell::poisson (crates/embloks-core/src/maxwell.rs:139-168) for a
+ * caller's right-hand side on an nx x ny node grid (x-major, index i * ny + j,
+ * phi = 0 outside): phi = Sx ((Sx r Sy) .* inv_lap) Sy, as four dense products
+ * on MFMA, in FP32 or in BF16 with a 3-term split (hi.hi + hi.lo + lo.hi, FP32
+ * accumulation). The tick itself always solves in FP32. */
+enum { EMB_DST_FP32 = 0, EMB_DST_BF16X3 = 1 };
+typedef struct EmbPoisson EmbPoisson;
+int emb_poisson_create(EmbDevice *dev, int32_t nx, int32_t ny, EmbPoisson **out);
+/* Upload the right-hand side (nx * ny doubles, rounded to FP32 on the device). */
+int emb_poisson_set_rhs(EmbPoisson *p, const double *rhs);
+/* Run `repeats` solves back to back on the handle's stream and wait; *avg_ms
+ * (if not NULL) gets the mean time per solve from HIP events around them.
+ * tile: 64 or 128 (output tile side), 0 = by grid size. */
+int emb_poisson_solve(EmbPoisson *p, int32_t precision, int32_t tile, int32_t repeats, double *avg_ms);
+/* The last solve's potential (nx * ny doubles). */
+int emb_poisson_read(EmbPoisson *p, double *phi);
+int emb_poisson_destroy(EmbPoisson *p);
+
 /* Footprint in bytes that a scene of this shape would occupy on the device. */
 int64_t emb_scene_footprint(const EmbSettings *settings, int32_t cols, int32_t rows, int64_t n_electrons);
 

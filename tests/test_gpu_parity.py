@@ -199,3 +199,28 @@ def test_fixture_scene_thirty_ticks():
         assert relative_error(e, array(f["final_e"])) < 2e-2
         for g, want in zip(gains, f["gains"]):
             assert abs(g - want) < 1e-3
+
+
+@gpu
+def test_ring_circuit_two_thousand_ticks():
+    # The 40 x 32 ring (one DC battery, one carrier per cell) run 2000 ticks on
+    # both backends from the same scene: a coupled run, so the FP32/FP64
+    # difference is reported as it grows rather than held to the one-tick bar.
+    world = W.ring_circuit()
+    cpu = orc.Sim(world)
+    sim = gpusim()(world)
+    cpu.step(2000)
+    sim.run_ticks(2000)
+    pos, vel, gains = sim.read_state()
+    cpos, cvel, cgains = cpu.state()
+    ex, ey, bz = sim.read_yee()
+    cex, cey, cbz = cpu.yee()
+    err = {"pos": float(np.max(np.abs(pos - cpos))), "ex": relative_error(ex, cex), "ey": relative_error(ey, cey),
+           "bz": relative_error(bz, cbz), "phi": relative_error(sim.read_potential(), cpu.potential_nodes()),
+           "gain": abs(gains[0] - cgains[0])}
+    residual, _, _ = sim.diagnostics()
+    print("ring 2000 ticks", err, "gauss residual", residual)
+    assert err["pos"] < 1e-2, err
+    assert err["ex"] < 1e-2 and err["ey"] < 1e-2 and err["phi"] < 1e-2, err
+    assert err["gain"] < 1e-3, err
+    assert residual < 1e-4, residual
