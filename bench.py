@@ -165,6 +165,34 @@ def run_poisson(n: int = 2048, repeats: int = 20):
     return out
 
 
+def run_p3m(repeats: int = 10):
+    """The close-range force alone (CloseRange::force, close_range.rs:314-376)
+    on the 4096 x 4096 scene with 4M clustered electrons at cutoff 2: time per
+    evaluation from HIP events over `repeats` evaluations on resident data, and
+    the candidate pairs it walks (each electron against every electron of its
+    (2 reach + 1)^2 cell window)."""
+    from embloks_amd import world as W
+    from embloks_amd.gpusim import GpuSim
+
+    world = W.clustered_charges()
+    t0 = time.perf_counter()
+    sim = GpuSim(world)
+    setup_s = time.perf_counter() - t0
+    sim.close_force(2)
+    _, ms = sim.close_force(repeats)
+    reach = int(math.ceil(world.settings["electron_cutoff"]))
+    counts = world.density   # electrons per cell (one carrier per cell, neutral cells)
+    pad = np.pad(counts, reach)
+    window = sum(pad[reach + dc: reach + dc + counts.shape[0], reach + dr: reach + dr + counts.shape[1]]
+                 for dc in range(-reach, reach + 1) for dr in range(-reach, reach + 1))
+    candidates = float((counts * window).sum())
+    sim.destroy()
+    return {"grid": f"{world.cols}x{world.rows}", "electrons": world.n_electrons, "cutoff": world.settings["electron_cutoff"],
+            "placement": "70% in 1000 log-normal Gaussian clusters (spread 4-24 cells), 30% uniform, seed 4",
+            "ms_per_eval": round(ms, 4), "candidate_pairs": candidates,
+            "candidate_pairs_per_s": round(candidates / (ms * 1e-3), 1), "scene_setup_s": round(setup_s, 2)}
+
+
 def cpu_baseline(world, budget_s: float = 20.0):
     sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
     from oracle import orc
@@ -232,6 +260,7 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=48)
     ap.add_argument("--serve-scale", type=float, default=0.02, help="fraction of each request's 1000-20000 steps")
     ap.add_argument("--no-poisson", action="store_true", help="skip the isolated 2048^2 DST solve")
+    ap.add_argument("--no-p3m", action="store_true", help="skip the 4096^2 clustered close-range force")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a plain 1 GiB device-to-device copy 5 times (scale for HBM byte counters)")
     args = ap.parse_args()
@@ -268,6 +297,8 @@ def main():
     }
     if not args.no_poisson:
         out["poisson"] = run_poisson()
+    if not args.no_p3m:
+        out["p3m"] = run_p3m()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(world)
     if not args.no_serve:

@@ -108,7 +108,7 @@ struct DevScene {
     float inv_mass;
     float q;                  // electron charge
     float k_pair;             // k q q (close_range.rs:366)
-    float k_pair_u;           // k q q / pair_unit: the pair sum in table units back to a force
+    float k_pair_fx;          // k q q 2^-24: the pair sum over 24-bit fixed-point offsets back to a force
     float k_ion_q;            // k * cell_charge * q (close_range.rs:348)
     float k_q;                // k * q (charges_potential, close_range.rs:463)
     float cutoff2, g_scale, half, square_samples;
@@ -120,7 +120,7 @@ struct DevScene {
     float rho_from_fixed;     // |q_e| / 2^44 / 4^(2 passes): int64 charge -> FP32
     float j_from_fixed;       // q_e / 2^28 / 4^(2 passes)
     float i_from_fixed;       // q_e / 2^28 (cell-binned current)
-    float pair_unit;          // sqrt(g_scale): pair offsets in these units square to table coordinates
+    float pair_s2;            // g_scale 2^-48: a squared 24-bit fixed-point offset to a table coordinate
     int32_t passes;
     int32_t pad1;
     // --- fields (FP32)

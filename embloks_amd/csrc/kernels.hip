@@ -378,12 +378,20 @@ __device__ __forceinline__ int substeps_of(const DevScene &S, uint32_t t) {
     return sub < 1 ? 1 : sub;
 }
 
+// Positions in the close-range sums are 24-bit fixed point: cell offset times
+// 2^24 plus the fraction rounded to 2^-24 of a cell. Pair offsets are then exact
+// integer differences, so two electrons 0.01 cells apart keep their separation
+// to ~1e-5 of itself whatever frame the positions are staged in.
+__device__ __forceinline__ int fix24(float f) { return __float2int_rn(f * 16777216.0f); }
+__device__ __forceinline__ int2 fixed_pos(int dc, int dr, float fx, float fy) {
+    return make_int2(dc * (1 << 24) + fix24(fx), dr * (1 << 24) + fix24(fy));
+}
+
 // Neighbour access of the per-electron close-range loop: the whole sorted
-// snapshot in global memory (positions as fractions of their own cell), or a
-// window of it staged in LDS (positions relative to the tile's centre cell).
-// Positions are in pair units (cells times pair_unit, so that a squared offset
-// is a coordinate of the pair table); `origin` gives the electron's position in
-// the frame of neighbour cell (cc, rr)'s stored positions.
+// snapshot in global memory (positions relative to their own cell), or a
+// window of it staged in LDS (positions relative to the window's corner cell).
+// `origin` gives the electron's position in the frame of neighbour cell
+// (cc, rr)'s positions.
 struct GlobalNeighbours {
     const DevScene &S;
     __device__ void range(int cc, int rr, int &b, int &e) const {
@@ -391,25 +399,24 @@ struct GlobalNeighbours {
         b = S.cstart[cell];
         e = S.cstart[cell + 1];
     }
-    __device__ float2 p(int j) const { return make_float2(S.sfx[j] * S.pair_unit, S.sfy[j] * S.pair_unit); }
-    __device__ float2 origin(int hc, int hr, float px, float py, int cc, int rr) const {
-        return make_float2(((float)(hc - cc) + px) * S.pair_unit, ((float)(hr - rr) + py) * S.pair_unit);
+    __device__ int2 p(int j) const { return fixed_pos(0, 0, S.sfx[j], S.sfy[j]); }
+    __device__ int2 origin(int hc, int hr, float px, float py, int cc, int rr) const {
+        return fixed_pos(hc - cc, hr - rr, px, py);
     }
 };
 
 struct LdsNeighbours {
-    const float2 *lp;            // LDS: staged positions, tile frame, pair units
+    const int2 *lp;              // LDS: staged positions, window frame
     const int *lb, *le;          // LDS: per window cell, range in lp
-    int wc0, wr0, W, cmx, cmy;
-    float unit;
+    int wc0, wr0, W;
     __device__ void range(int cc, int rr, int &b, int &e) const {
         const int w = (cc - wc0) * W + (rr - wr0);
         b = lb[w];
         e = le[w];
     }
-    __device__ float2 p(int j) const { return lp[j]; }
-    __device__ float2 origin(int hc, int hr, float px, float py, int, int) const {
-        return make_float2(((float)(hc - cmx) + px) * unit, ((float)(hr - cmy) + py) * unit);
+    __device__ int2 p(int j) const { return lp[j]; }
+    __device__ int2 origin(int hc, int hr, float px, float py, int, int) const {
+        return fixed_pos(hc - wc0, hr - wr0, px, py);
     }
 };
 
@@ -467,12 +474,14 @@ __device__ __forceinline__ void ion_uniform(const DevScene &S, int home, float p
 }
 
 // One pair term g(r^2) d, g lerped from the r^2-uniform table
-// (close_range.rs:359-372). The offset (dx, dy) is in pair units, so its square
-// is the table coordinate r^2 * PAIR_SAMPLES / cutoff^2. A pair at or beyond the
-// cutoff reads the two zero samples past the table's end, and the electron
-// itself (d = 0) adds g(0) * 0: both contribute exactly zero.
-__device__ __forceinline__ void pair_term(const float *gt, float dx, float dy, float &sx, float &sy) {
-    const float d2 = dx * dx + dy * dy;
+// (close_range.rs:359-372). The offset (ix, iy) is in 2^-24 cells; s2 =
+// g_scale 2^-48 turns its square into the table coordinate r^2 PAIR_SAMPLES /
+// cutoff^2. A pair at or beyond the cutoff reads the two zero samples past the
+// table's end, and the electron itself (d = 0) adds g(0) * 0: both contribute
+// exactly zero. The sums are in 2^-24 cells (k_pair_fx scales them back).
+__device__ __forceinline__ void pair_term(const float *gt, float s2, int ix, int iy, float &sx, float &sy) {
+    const float dx = (float)ix, dy = (float)iy;
+    const float d2 = (dx * dx + dy * dy) * s2;
     const float tt = d2 < (float)PAIR_SAMPLES ? d2 : (float)(PAIR_SAMPLES + 2);
     const int si = (int)tt;
     const float g0 = gt[si], g1 = gt[si + 1];
@@ -540,17 +549,17 @@ __device__ void push_core(const DevScene &S, const NB &nb, const float *gt, int 
                 }
                 const float gapy = oy > 0.0f ? oy + py - 1.0f : (oy < 0.0f ? -oy - py : 0.0f);
                 if (gapx * gapx + gapy * gapy >= cut2 * 1.0001f) continue;
-                const float2 o = nb.origin(hc, hr, px, py, cc, rr);
+                const int2 o = nb.origin(hc, hr, px, py, cc, rr);
                 int j, e;
                 nb.range(cc, rr, j, e);
                 for (; j < e; j++) {
-                    const float2 q = nb.p(j);
-                    pair_term(gt, o.x - q.x, o.y - q.y, sx, sy);
+                    const int2 q = nb.p(j);
+                    pair_term(gt, S.pair_s2, o.x - q.x, o.y - q.y, sx, sy);
                 }
             }
         }
-        fx += S.k_pair_u * sx;
-        fy += S.k_pair_u * sy;
+        fx += S.k_pair_fx * sx;
+        fy += S.k_pair_fx * sy;
         ax += fx * S.inv_mass;
         ay += fy * S.inv_mass;
     }
@@ -579,9 +588,151 @@ __device__ __forceinline__ int lanes_per(int n) {
     return i;
 }
 
-// Substep k for every electron, per PUSH_TILE x PUSH_TILE tile of cells. The
+// Column runs of the cell mode: the electrons of rows r_lo..r_hi of column cc,
+// contiguous in the sorted order, with fixed-point positions relative to the
+// window's corner cell (wc0, wr0). Staged in LDS, or read from the snapshot.
+struct LdsRuns {
+    const int2 *lp;
+    const int *lb, *le;
+    int wc0, wr0, W;
+    __device__ void run(int cc, int r_lo, int r_hi, int &jb, int &je) const {
+        const int w = (cc - wc0) * W;
+        jb = lb[w + r_lo - wr0];
+        je = le[w + r_hi - wr0];
+    }
+    __device__ int2 at(int j, int) const { return lp[j]; }
+};
+
+struct GlobalRuns {
+    const DevScene &S;
+    int wc0, wr0;
+    __device__ void run(int cc, int r_lo, int r_hi, int &jb, int &je) const {
+        jb = S.cstart[cc * S.rows + r_lo];
+        je = S.cstart[cc * S.rows + r_hi + 1];
+    }
+    __device__ int2 at(int j, int cc) const {
+        const int rr = S.scell[j] - cc * S.rows;
+        return fixed_pos(cc - wc0, rr - wr0, S.sfx[j], S.sfy[j]);
+    }
+};
+
+// One work item of the cell mode: electrons [chunk I, chunk I + I) of home cell
+// (hc, hr), I = lanes_per(count), each spread over SL = 64 / I lanes that split
+// the neighbour runs; the partial sums meet in a fixed butterfly. FORCE: write
+// the close-range force to force_out[id] instead of pushing.
+template <bool FORCE, class RUNS>
+__device__ __forceinline__ void cell_item(const DevScene &S, const RUNS &runs, const float *gt, int hc, int hr, int chunk,
+                                          int wc0, int wr0, int k, float h, uint32_t t, float2 *force_out) {
+    const int lane = threadIdx.x & 63;
+    const int R = S.reach;
+    const float s2 = S.pair_s2;
+    const int home = hc * S.rows + hr;
+    const int e0 = __builtin_amdgcn_readfirstlane(S.cstart[home]);
+    const int n = __builtin_amdgcn_readfirstlane(S.cstart[home + 1]) - e0;
+    const int I = lanes_per(n), shift = __builtin_ctz(I), SL = 64 >> shift;
+    const int i = lane & (I - 1), sl = lane >> shift;
+    const int ei = chunk * I + i;
+    const bool live = ei < n;
+    const int slot = e0 + (live ? ei : n - 1);
+    const float px = S.sfx[slot], py = S.sfy[slot];
+    const int2 b = fixed_pos(hc - wc0, hr - wr0, px, py);
+    const int c_lo = max(hc - R, 0), c_hi = min(hc + R, S.cols - 1);
+    const int r_lo = max(hr - R, 0), r_hi = min(hr + R, S.rows - 1);
+    float sx = 0.0f, sy = 0.0f, fx = 0.0f, fy = 0.0f;
+    for (int cc = c_lo; cc <= c_hi; cc++) {
+        int jb, je;
+        runs.run(cc, r_lo, r_hi, jb, je);
+        jb = __builtin_amdgcn_readfirstlane(jb);
+        je = __builtin_amdgcn_readfirstlane(je);
+        int jj = jb;
+        for (; jj + 4 * SL <= je; jj += 4 * SL) {
+            const int2 q0 = runs.at(jj + sl, cc), q1 = runs.at(jj + SL + sl, cc);
+            const int2 q2 = runs.at(jj + 2 * SL + sl, cc), q3 = runs.at(jj + 3 * SL + sl, cc);
+            pair_term(gt, s2, b.x - q0.x, b.y - q0.y, sx, sy);
+            pair_term(gt, s2, b.x - q1.x, b.y - q1.y, sx, sy);
+            pair_term(gt, s2, b.x - q2.x, b.y - q2.y, sx, sy);
+            pair_term(gt, s2, b.x - q3.x, b.y - q3.y, sx, sy);
+        }
+        for (; jj < je; jj += SL) {
+            // past the run's end a lane contributes nothing
+            const int2 q = runs.at(min(jj + sl, je - 1), cc);
+            const bool in = jj + sl < je;
+            pair_term(gt, s2, in ? b.x - q.x : 0, in ? b.y - q.y : 0, sx, sy);
+        }
+    }
+    const CellInfo info = S.info[home];
+    const bool uniform = (info.flags & F_ION_UNIFORM) != 0u;
+    if (!uniform) {
+        const int nr = r_hi - r_lo + 1, cells = (c_hi - c_lo + 1) * nr;
+        for (int w = sl; w < cells; w += SL) {
+            const int cc = c_lo + w / nr, rr = r_lo + w % nr;
+            const float dens = S.density[cc * S.rows + rr];
+            if (dens > 0.0f) ion_cell(S, (float)(hc - cc), (float)(hr - rr), px, py, dens, fx, fy);
+        }
+    }
+    for (int off = I; off < 64; off <<= 1) {
+        sx += __shfl_xor(sx, off, 64);
+        sy += __shfl_xor(sy, off, 64);
+        fx += __shfl_xor(fx, off, 64);
+        fy += __shfl_xor(fy, off, 64);
+    }
+    if (sl == 0 && live) {
+        if (uniform) ion_uniform(S, home, px, py, fx, fy);
+        fx += S.k_pair_fx * sx;
+        fy += S.k_pair_fx * sy;
+        if (FORCE) {
+            force_out[S.order[slot]] = make_float2(fx, fy);
+        } else {
+            const float vx = S.svx[slot], vy = S.svy[slot];
+            float ax, ay;
+            field_accel(S, hc, hr, px, py, vx, vy, info.flags, k, h, t, ax, ay);
+            ax += fx * S.inv_mass;
+            ay += fy * S.inv_mass;
+            finish(S, S.order[slot], hc, hr, px, py, vx, vy, info.damping, ax, ay, h);
+        }
+    }
+}
+
+// The close-range force alone on one electron (electron mode of FORCE).
+template <class NB>
+__device__ void force_core(const DevScene &S, const NB &nb, const float *gt, int slot, float2 *force_out) {
+    const int home = S.scell[slot];
+    const int hc = home / S.rows, hr = home - (home / S.rows) * S.rows;
+    const float px = S.sfx[slot], py = S.sfy[slot];
+    const CellInfo info = S.info[home];
+    float fx = 0.0f, fy = 0.0f, sx = 0.0f, sy = 0.0f;
+    const int R = S.reach;
+    const int c_lo = max(hc - R, 0), c_hi = min(hc + R, S.cols - 1);
+    const int r_lo = max(hr - R, 0), r_hi = min(hr + R, S.rows - 1);
+    const bool uniform = (info.flags & F_ION_UNIFORM) != 0u;
+    if (uniform) ion_uniform(S, home, px, py, fx, fy);
+    for (int cc = c_lo; cc <= c_hi; cc++) {
+        const float ox = (float)(hc - cc);
+        const float gapx = ox > 0.0f ? ox + px - 1.0f : (ox < 0.0f ? -ox - px : 0.0f);
+        for (int rr = r_lo; rr <= r_hi; rr++) {
+            const float oy = (float)(hr - rr);
+            if (!uniform) {
+                const float dens = S.density[cc * S.rows + rr];
+                if (dens > 0.0f) ion_cell(S, ox, oy, px, py, dens, fx, fy);
+            }
+            const float gapy = oy > 0.0f ? oy + py - 1.0f : (oy < 0.0f ? -oy - py : 0.0f);
+            if (gapx * gapx + gapy * gapy >= S.cutoff2 * 1.0001f) continue;
+            const int2 o = nb.origin(hc, hr, px, py, cc, rr);
+            int j, e;
+            nb.range(cc, rr, j, e);
+            for (; j < e; j++) {
+                const int2 q = nb.p(j);
+                pair_term(gt, S.pair_s2, o.x - q.x, o.y - q.y, sx, sy);
+            }
+        }
+    }
+    force_out[S.order[slot]] = make_float2(fx + S.k_pair_fx * sx, fy + S.k_pair_fx * sy);
+}
+
+// Substep k for every electron, per PUSH_TILE x PUSH_TILE tile of cells (FORCE:
+// the close-range force alone, CloseRange::force, close_range.rs:314-376). The
 // positions of the tile's neighbourhood (reach cells around it) are staged in
-// LDS relative to the tile's centre. Two ways to run the pair sums:
+// LDS, in fixed point relative to the window's corner, when they fit. Two ways to run the pair sums:
 //   * cell mode (tiles with many electrons per occupied cell): a wave takes up
 //     to I electrons of one home cell, I a power of two, and splits the
 //     window's (2 reach + 1) column runs between 64 / I lanes per electron, so
@@ -589,9 +740,8 @@ __device__ __forceinline__ int lanes_per(int n) {
 //     the partial sums meet in a fixed butterfly;
 //   * electron mode (sparse tiles): one thread per electron with per-cell
 //     culling against the cutoff.
-// A tile whose window holds more than PUSH_STAGE electrons reads them from
-// global memory in electron mode. Both modes sum in a fixed order, so a scene
-// gives the same bits alone or batched.
+// A window holding more than PUSH_STAGE electrons is read from global memory.
+// Every sum runs in a fixed order, so a scene gives the same bits alone or batched.
 constexpr int PUSH_WIN = PUSH_TILE + 2 * 4;   // window side for reach <= 4
 constexpr int PUSH_STAGE = 3328;
 constexpr int PUSH_CELLS = PUSH_TILE * PUSH_TILE;
@@ -599,8 +749,9 @@ constexpr int CELL_MODE_MIN = 6;              // electrons per occupied cell for
 constexpr int PUSH_THREADS = 512;             // 8 waves: two blocks fill a CU's 16 wave slots at <= 128 VGPRs
 constexpr int PUSH_WAVES = PUSH_THREADS / 64;
 
-__global__ __launch_bounds__(PUSH_THREADS) void k_push_tiled(Batch b, int k) {
-    __shared__ float2 lp[PUSH_STAGE];
+template <bool FORCE>
+__global__ __launch_bounds__(PUSH_THREADS) void k_push_tiled(Batch b, int k, float2 *force_out) {
+    __shared__ int2 lp[PUSH_STAGE];
     __shared__ float gt[PAIR_SAMPLES + 4];
     __shared__ int lb[PUSH_WIN * PUSH_WIN], le[PUSH_WIN * PUSH_WIN];
     __shared__ int col_base[PUSH_WIN + 1], col_g0[PUSH_WIN];   // LDS base and first global slot per column
@@ -613,11 +764,16 @@ __global__ __launch_bounds__(PUSH_THREADS) void k_push_tiled(Batch b, int k) {
         const int s = scene_of(b, KC_PUSH, blk);
         const DevScene &S = b.scenes[s];
         const uint32_t t = S.ctr->tick;
-        const int sub = substeps_of(S, t);
-        if (k >= sub) continue;
+        int sub = 1;
+        if (!FORCE) {
+            sub = substeps_of(S, t);
+            if (k >= sub) continue;
+        } else if (!S.close_on) {
+            continue;
+        }
         const float h = S.dt / (float)sub;
         const int local = block_in_scene(b, KC_PUSH, s, blk);
-        if (k == 0 && threadIdx.x == 0 && local == 0) S.ctr->substeps = sub;
+        if (!FORCE && k == 0 && threadIdx.x == 0 && local == 0) S.ctr->substeps = sub;
         if (S.close_on && !table_loaded) {   // the pair table is shared by every scene
             for (int q = threadIdx.x; q < PAIR_SAMPLES + 4; q += PUSH_THREADS) gt[q] = S.g_table[q];
             table_loaded = true;
@@ -625,7 +781,6 @@ __global__ __launch_bounds__(PUSH_THREADS) void k_push_tiled(Batch b, int k) {
         const int tiles_r = (S.rows + PUSH_TILE - 1) / PUSH_TILE;
         const int tc0 = (local / tiles_r) * PUSH_TILE, tr0 = (local % tiles_r) * PUSH_TILE;
         const int tc1 = min(tc0 + PUSH_TILE, S.cols), tr1 = min(tr0 + PUSH_TILE, S.rows);
-        const int cmx = tc0 + PUSH_TILE / 2, cmy = tr0 + PUSH_TILE / 2;
         const int R = S.close_on ? S.reach : 0;
         const int W = PUSH_TILE + 2 * R;
         const int wc0 = tc0 - R, wr0 = tr0 - R;
@@ -681,17 +836,15 @@ __global__ __launch_bounds__(PUSH_THREADS) void k_push_tiled(Batch b, int k) {
             }
             for (int q = 0; q < W; q++) {
                 const int n = col_base[q + 1] - col_base[q];
-                const float ox = (float)(wc0 + q - cmx), unit = S.pair_unit;
                 for (int j = threadIdx.x; j < n; j += PUSH_THREADS) {
                     const int g = col_g0[q] + j;
                     const int rr = S.scell[g] - (wc0 + q) * S.rows;
-                    lp[col_base[q] + j] = make_float2((ox + S.sfx[g]) * unit, ((float)(rr - cmy) + S.sfy[g]) * unit);
+                    lp[col_base[q] + j] = fixed_pos(q, rr - wr0, S.sfx[g], S.sfy[g]);
                 }
             }
         }
         __syncthreads();
-        if (staged && S.close_on && cell_mode) {
-            const float unit = S.pair_unit;
+        if (S.close_on && cell_mode) {
             const int n_items = item_pref[PUSH_CELLS];
             for (int it = wave; it < n_items; it += PUSH_WAVES) {
                 int lo = 0, hi = PUSH_CELLS;
@@ -702,66 +855,11 @@ __global__ __launch_bounds__(PUSH_THREADS) void k_push_tiled(Batch b, int k) {
                 }
                 lo = __builtin_amdgcn_readfirstlane(lo);
                 const int hc = tc0 + lo / PUSH_TILE, hr = tr0 + lo % PUSH_TILE;
-                const int home = hc * S.rows + hr;
-                const int e0 = __builtin_amdgcn_readfirstlane(S.cstart[home]);
-                const int n = __builtin_amdgcn_readfirstlane(S.cstart[home + 1]) - e0;
-                const int I = lanes_per(n), shift = __builtin_ctz(I), SL = 64 >> shift;
-                const int i = lane & (I - 1), sl = lane >> shift;
-                const int ei = (it - item_pref[lo]) * I + i;
-                const bool live = ei < n;
-                const int slot = e0 + (live ? ei : n - 1);
-                const float px = S.sfx[slot], py = S.sfy[slot];
-                const float bx = ((float)(hc - cmx) + px) * unit, by = ((float)(hr - cmy) + py) * unit;
-                const int c_lo = max(hc - R, 0), c_hi = min(hc + R, S.cols - 1);
-                const int r_lo = max(hr - R, 0), r_hi = min(hr + R, S.rows - 1);
-                float sx = 0.0f, sy = 0.0f, fx = 0.0f, fy = 0.0f;
-                for (int cc = c_lo; cc <= c_hi; cc++) {
-                    const int wcol = (cc - wc0) * W;
-                    const int jb = __builtin_amdgcn_readfirstlane(lb[wcol + r_lo - wr0]);
-                    const int je = __builtin_amdgcn_readfirstlane(le[wcol + r_hi - wr0]);
-                    const float2 *lq = lp + sl;
-                    int jj = jb;
-                    for (; jj + 4 * SL <= je; jj += 4 * SL) {
-                        const float2 q0 = lq[jj], q1 = lq[jj + SL], q2 = lq[jj + 2 * SL], q3 = lq[jj + 3 * SL];
-                        pair_term(gt, bx - q0.x, by - q0.y, sx, sy);
-                        pair_term(gt, bx - q1.x, by - q1.y, sx, sy);
-                        pair_term(gt, bx - q2.x, by - q2.y, sx, sy);
-                        pair_term(gt, bx - q3.x, by - q3.y, sx, sy);
-                    }
-                    for (; jj < je; jj += SL) {
-                        // past the run's end a lane pairs with itself: d = 0 adds nothing
-                        const float2 q = lp[min(jj + sl, je - 1)];
-                        const bool in = jj + sl < je;
-                        pair_term(gt, in ? bx - q.x : 0.0f, in ? by - q.y : 0.0f, sx, sy);
-                    }
-                }
-                const CellInfo info = S.info[home];
-                const bool uniform = (info.flags & F_ION_UNIFORM) != 0u;
-                if (!uniform) {
-                    const int nr = r_hi - r_lo + 1, cells = (c_hi - c_lo + 1) * nr;
-                    for (int w = sl; w < cells; w += SL) {
-                        const int cc = c_lo + w / nr, rr = r_lo + w % nr;
-                        const float dens = S.density[cc * S.rows + rr];
-                        if (dens > 0.0f) ion_cell(S, (float)(hc - cc), (float)(hr - rr), px, py, dens, fx, fy);
-                    }
-                }
-                for (int off = I; off < 64; off <<= 1) {
-                    sx += __shfl_xor(sx, off, 64);
-                    sy += __shfl_xor(sy, off, 64);
-                    fx += __shfl_xor(fx, off, 64);
-                    fy += __shfl_xor(fy, off, 64);
-                }
-                if (sl == 0 && live) {
-                    const float vx = S.svx[slot], vy = S.svy[slot];
-                    if (uniform) ion_uniform(S, home, px, py, fx, fy);
-                    fx += S.k_pair_u * sx;
-                    fy += S.k_pair_u * sy;
-                    float ax, ay;
-                    field_accel(S, hc, hr, px, py, vx, vy, info.flags, k, h, t, ax, ay);
-                    ax += fx * S.inv_mass;
-                    ay += fy * S.inv_mass;
-                    finish(S, S.order[slot], hc, hr, px, py, vx, vy, info.damping, ax, ay, h);
-                }
+                const int chunk = it - item_pref[lo];
+                if (staged)
+                    cell_item<FORCE>(S, LdsRuns{lp, lb, le, wc0, wr0, W}, gt, hc, hr, chunk, wc0, wr0, k, h, t, force_out);
+                else
+                    cell_item<FORCE>(S, GlobalRuns{S, wc0, wr0}, gt, hc, hr, chunk, wc0, wr0, k, h, t, force_out);
             }
         } else {
             const int n_own = own_pref[PUSH_TILE];
@@ -770,14 +868,25 @@ __global__ __launch_bounds__(PUSH_THREADS) void k_push_tiled(Batch b, int k) {
                 while (own_pref[q + 1] <= e) q++;
                 const int cc = tc0 + q;
                 const int slot = S.cstart[cc * S.rows + tr0] + (e - own_pref[q]);
-                if (staged)
-                    push_core(S, LdsNeighbours{lp, lb, le, wc0, wr0, W, cmx, cmy, S.pair_unit}, gt, slot, k, h, t);
-                else push_core(S, GlobalNeighbours{S}, gt, slot, k, h, t);
+                if (FORCE) {
+                    if (staged)
+                        force_core(S, LdsNeighbours{lp, lb, le, wc0, wr0, W}, gt, slot, force_out);
+                    else
+                        force_core(S, GlobalNeighbours{S}, gt, slot, force_out);
+                } else {
+                    if (staged)
+                        push_core(S, LdsNeighbours{lp, lb, le, wc0, wr0, W}, gt, slot, k, h, t);
+                    else
+                        push_core(S, GlobalNeighbours{S}, gt, slot, k, h, t);
+                }
             }
         }
         __syncthreads();
     }
 }
+
+template __global__ void k_push_tiled<false>(Batch, int, float2 *);
+template __global__ void k_push_tiled<true>(Batch, int, float2 *);
 
 __global__ __launch_bounds__(256) void k_push(Batch b, int k) {
     for (int blk = blockIdx.x; blk < b.total[KC_ELECTRONS]; blk += gridDim.x) {
@@ -1244,7 +1353,7 @@ hipError_t tick_kernels(const Batch &b, int fdtd_substeps, hipStream_t stream, h
     EMB_MARK(0);
     EMB_LAUNCH(k_trim, KC_STACKS, 1024, b);
     EMB_MARK(1);
-    EMB_LAUNCH(k_push_tiled, KC_PUSH, PUSH_THREADS, b, 0);
+    EMB_LAUNCH(k_push_tiled<false>, KC_PUSH, PUSH_THREADS, b, 0, nullptr);
     EMB_MARK(2);
     if (b.total[KC_ELECTRONS] > 0) EMB_LAUNCH(k_rest, KC_SCENE, 1024, b);
     EMB_MARK(3);
@@ -1278,6 +1387,13 @@ hipError_t deposit_kernel(const Batch &b, hipStream_t stream) {
 
 hipError_t diagnostics_kernel(const DevScene *scene, double c2, double *out, int blocks, hipStream_t stream) {
     k_diagnostics<<<blocks, 256, 0, stream>>>(scene, c2, out, blocks);
+    return hipGetLastError();
+}
+
+// The close-range force on every electron of the batch's scenes at their
+// current (sorted) positions, written by electron index.
+hipError_t close_force_kernel(const Batch &b, float2 *out, hipStream_t stream) {
+    EMB_LAUNCH(k_push_tiled<true>, KC_PUSH, PUSH_THREADS, b, 0, out);
     return hipGetLastError();
 }
 

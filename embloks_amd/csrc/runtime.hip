@@ -250,8 +250,8 @@ int make_resident(EmbDevice *dev, const EmbSettings &s, HostScene &&hs_in, hipSt
     d.k_q = (float)(k * q);
     d.cutoff2 = (float)(s.electron_cutoff * s.electron_cutoff);
     d.g_scale = s.electron_cutoff > 0.0 ? (float)(4096.0 / (s.electron_cutoff * s.electron_cutoff)) : 0.0f;
-    d.pair_unit = s.electron_cutoff > 0.0 ? (float)(64.0 / s.electron_cutoff) : 0.0f;
-    d.k_pair_u = s.electron_cutoff > 0.0 ? (float)(k * q * q * s.electron_cutoff / 64.0) : 0.0f;
+    d.pair_s2 = s.electron_cutoff > 0.0 ? (float)(4096.0 / (s.electron_cutoff * s.electron_cutoff) / 281474976710656.0) : 0.0f;
+    d.k_pair_fx = (float)(k * q * q / 16777216.0);
     d.half = (float)tables.half;
     d.square_samples = 32.0f;
     d.c2h = (float)(hs.h_fdtd * s.speed_of_light * s.speed_of_light);
@@ -381,6 +381,7 @@ struct EmbSim {
     hipStream_t stream = nullptr;
     Resident r;
     double *diag = nullptr;
+    float2 *force = nullptr;   // emb_sim_close_force output, allocated on first use
 };
 
 extern "C" {
@@ -742,12 +743,46 @@ const char *emb_tick_phase_name(int phase) {
     return (phase >= 0 && phase < TICK_PHASES) ? TICK_PHASE_NAMES[phase] : nullptr;
 }
 
+int emb_sim_close_force(EmbSim *sim, int32_t repeats, double *force, double *avg_ms) {
+    EMB_GUARD({
+        if (!sim) return fail(EMB_E_INVALID, "null argument");
+        if (repeats < 1) return fail(EMB_E_INVALID, "repeats must be >= 1");
+        if (sim->r.ds.close_on == 0) return fail(EMB_E_STATE, "the close-range force is off (electron_cutoff = 0)");
+        const size_t n = (size_t)sim->r.ds.n;
+        EMB_HIP(hipSetDevice(sim->dev->ordinal));
+        if (!sim->force && n > 0) EMB_HIP(hipMalloc(&sim->force, n * sizeof(float2)));
+        hipEvent_t e0, e1;
+        EMB_HIP(hipEventCreate(&e0));
+        EMB_HIP(hipEventCreate(&e1));
+        hipError_t err = hipEventRecord(e0, sim->stream);
+        for (int r = 0; r < repeats && err == hipSuccess; r++) err = close_force_kernel(sim->r.batch, sim->force, sim->stream);
+        if (err == hipSuccess) err = hipEventRecord(e1, sim->stream);
+        if (err == hipSuccess) err = hipEventSynchronize(e1);
+        float ms = 0.0f;
+        if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        EMB_HIP(err);
+        if (avg_ms) *avg_ms = (double)ms / repeats;
+        if (force && n > 0) {
+            std::vector<float2> h(n);
+            EMB_HIP(hipMemcpy(h.data(), sim->force, n * sizeof(float2), hipMemcpyDeviceToHost));
+            for (size_t i = 0; i < n; i++) {
+                force[2 * i] = h[i].x;
+                force[2 * i + 1] = h[i].y;
+            }
+        }
+        return EMB_OK;
+    })
+}
+
 int emb_sim_destroy(EmbSim *sim) {
     if (!sim) return EMB_OK;
     (void)hipSetDevice(sim->dev->ordinal);
     (void)hipStreamSynchronize(sim->stream);
     free_resident(sim->r);
     if (sim->diag) (void)hipFree(sim->diag);
+    if (sim->force) (void)hipFree(sim->force);
     (void)hipStreamDestroy(sim->stream);
     delete sim;
     return EMB_OK;

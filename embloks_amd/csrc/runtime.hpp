@@ -99,6 +99,7 @@ constexpr int TICK_PHASES = 11;
 extern const char *const TICK_PHASE_NAMES[];
 hipError_t tick_kernels(const Batch &b, int fdtd_substeps, hipStream_t stream, hipEvent_t *marks = nullptr);
 hipError_t deposit_kernel(const Batch &b, hipStream_t stream);
+hipError_t close_force_kernel(const Batch &b, float2 *out, hipStream_t stream);
 hipError_t diagnostics_kernel(const DevScene *scene, double c2, double *out, int blocks, hipStream_t stream);
 
 // Write the per-tick AC phases and EMF targets of ticks [r.tick, r.tick + count)

@@ -40,6 +40,7 @@ SIGNATURES = {
     "emb_sim_read_charge_fixed": (C.c_int, [VP, I64]),
     "emb_sim_diagnostics": (C.c_int, [VP, D, D, D]),
     "emb_sim_kernel_times": (C.c_int, [VP, C.c_int, D, C.c_int]),
+    "emb_sim_close_force": (C.c_int, [VP, C.c_int32, D, D]),
     "emb_tick_phase_name": (C.c_char_p, [C.c_int]),
     "emb_sim_destroy": (C.c_int, [VP]),
     "emb_server_create": (C.c_int, [VP, C.POINTER(abi.EmbSettings), C.POINTER(abi.EmbServerConfig), C.POINTER(VP)]),
@@ -198,6 +199,14 @@ class GpuSim:
         r, q, e = C.c_double(), C.c_double(), C.c_double()
         check(lib().emb_sim_diagnostics(self.h, C.byref(r), C.byref(q), C.byref(e)))
         return r.value, q.value, e.value
+
+    def close_force(self, repeats: int = 1):
+        """CloseRange::force (close_range.rs:314-376) on every electron at its
+        current position: (forces n x 2, mean ms per evaluation)."""
+        out = np.zeros((self.n, 2))
+        ms = C.c_double()
+        check(lib().emb_sim_close_force(self.h, repeats, _p(out), C.byref(ms)))
+        return out, ms.value
 
     def kernel_times(self, ticks: int) -> dict:
         """Run `ticks` more ticks with HIP events between the kernel phases; average ms per phase."""

@@ -396,3 +396,36 @@ def gaussian_blobs(n: int = 2048, count: int = 200, min_sigma: float = 2.0, max_
         gy = np.exp(-0.5 * ((axis - cy) / sigma) ** 2)
         rho += amp * np.outer(gx, gy)
     return rho
+
+
+def clustered_charges(side: int = 4096, n_electrons: int = 4_000_000, cluster_frac: float = 0.7,
+                      n_clusters: int = 1000, min_spread: float = 4.0, max_spread: float = 24.0,
+                      cutoff: float = 2.0, seed: int = 4) -> World:
+    """Electrons for the close-range force alone: `cluster_frac` of them in
+    Gaussian clusters (cluster sizes log-normal, centres uniform, spread
+    uniform in [min_spread, max_spread] cells), the rest uniform. Every cell
+    that holds electrons is a conductor whose ion density equals its electron
+    count (one carrier per cell), so each cell is neutral and the reference's
+    per-cell ion squares (close_range.rs:335-350) come into play everywhere."""
+    s = dict(abi.DEFAULTS)
+    s["carriers_per_cell"] = 1
+    s["electron_cutoff"] = cutoff
+    w = World(side, side, s, seed)
+    rng = np.random.Generator(np.random.PCG64(seed))
+    n_cl = int(round(cluster_frac * n_electrons))
+    weights = rng.lognormal(0.0, 1.0, n_clusters)
+    counts = rng.multinomial(n_cl, weights / weights.sum())
+    centres = rng.uniform(0.0, side, size=(n_clusters, 2))
+    spreads = rng.uniform(min_spread, max_spread, n_clusters)
+    clustered = np.repeat(centres, counts, axis=0) + rng.normal(size=(n_cl, 2)) * np.repeat(spreads, counts)[:, None]
+    background = rng.uniform(0.0, side, size=(n_electrons - n_cl, 2))
+    pos = np.clip(np.concatenate([clustered, background]), 1e-3, side - 1e-3)
+    cell = np.floor(pos).astype(np.int64)
+    per_cell = np.bincount(cell[:, 0] * side + cell[:, 1], minlength=side * side).reshape(side, side)
+    occupied = per_cell > 0
+    w.kind[occupied] = CONDUCTOR
+    w.resistance[occupied] = 1.0
+    w.density[occupied] = per_cell[occupied].astype(np.float64)
+    w.pos = pos
+    w.vel = np.zeros_like(pos)
+    return w

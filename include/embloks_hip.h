@@ -152,6 +152,11 @@ int emb_sim_diagnostics(EmbSim *sim, double *gauss_residual, double *electron_ch
  * emb_tick_phase_name. No counterpart in the reference (which times nothing
  * but the wall clock, crates/embloks-gpu/examples/bench.rs:47-54). */
 int emb_sim_kernel_times(EmbSim *sim, int ticks, double *avg_ms, int capacity);
+/* CloseRange::force (crates/embloks-core/src/close_range.rs:314-376): the
+ * close-range pair and ion force on every electron at its current position
+ * (n_electrons x 2 doubles, world order), computed `repeats` times on the sim's
+ * stream; *avg_ms (if not NULL) gets the mean time per evaluation. */
+int emb_sim_close_force(EmbSim *sim, int32_t repeats, double *force, double *avg_ms);
 const char *emb_tick_phase_name(int phase);
 int emb_sim_destroy(EmbSim *sim);
 

@@ -1034,6 +1034,57 @@ struct CloseRange {
         return force;
     }
 
+    // close_range.rs:314-376 for electron i alone: every pair seen from i's side
+    // (the force the Newton-3 loop above accumulates on i), plus the ions.
+    void force_on(const World &w, const CellList &list, size_t i, double out[2]) const {
+        auto tab = t();
+        const double k = s.coulomb_k, k_ion = s.coulomb_k * s.cell_charge;
+        const double cutoff = s.electron_cutoff;
+        const int reach = (int)std::ceil(cutoff);
+        const double cutoff2 = cutoff * cutoff;
+        const double scale = (double)(tab->g.size() - 2) / cutoff2;
+        const int m = tab->square.w;
+        const double samples = (double)(m - 1) / (2.0 * tab->half);
+        const int cols = w.cols, rows = w.rows;
+        const double x = w.pos[2 * i], y = w.pos[2 * i + 1];
+        const int ci = (int)(list.cell[i] / rows), ri = (int)(list.cell[i] % rows);
+        out[0] = out[1] = 0.0;
+        for (int c = std::max(ci - reach, 0); c < std::min(ci + reach + 1, cols); c++) {
+            for (int r = std::max(ri - reach, 0); r < std::min(ri + reach + 1, rows); r++) {
+                const double ion = w.density[(size_t)c * rows + r];
+                if (ion > 0.0) {
+                    const double u = (x - (double)c - 0.5 + tab->half) * samples;
+                    const double v = (y - (double)r - 0.5 + tab->half) * samples;
+                    const long long a = std::clamp((long long)u, 0LL, (long long)m - 2);
+                    const long long b = std::clamp((long long)v, 0LL, (long long)m - 2);
+                    const double fu = u - (double)a, fv = v - (double)b;
+                    for (int axis = 0; axis < 2; axis++) {
+                        const Grid3 &sq = tab->square;
+                        const double value = sq.at(a, b, axis + 1) * (1.0 - fu) * (1.0 - fv) +
+                                             sq.at(a + 1, b, axis + 1) * fu * (1.0 - fv) +
+                                             sq.at(a, b + 1, axis + 1) * (1.0 - fu) * fv +
+                                             sq.at(a + 1, b + 1, axis + 1) * fu * fv;
+                        out[axis] += k_ion * ion * w.charge[i] * value;
+                    }
+                }
+                const size_t other = (size_t)c * rows + r;
+                for (size_t s2 = list.start[other]; s2 < list.start[other + 1]; s2++) {
+                    const size_t j = list.order[s2];
+                    if (j == i) continue;
+                    const double dx = x - w.pos[2 * j], dy = y - w.pos[2 * j + 1];
+                    const double d2 = dx * dx + dy * dy;
+                    if (d2 >= cutoff2) continue;
+                    const double tt = d2 * scale;
+                    const size_t si = (size_t)tt;
+                    const double f = k * w.charge[i] * w.charge[j] *
+                                     (tab->g[si] + (tab->g[si + 1] - tab->g[si]) * (tt - (double)si));
+                    out[0] += f * dx;
+                    out[1] += f * dy;
+                }
+            }
+        }
+    }
+
     // close_range.rs:379-403
     Grid ion_potential(const World &w) const {
         auto tab = t();
@@ -1492,6 +1543,24 @@ int orc_sim_stacks(const OrcSim *sim, int *low_high, int *n_cells) {
         if (n_cells) n_cells[k] = (int)st[k].cells.size();
     }
     return (int)st.size();
+}
+
+void orc_close_force_subset(const EmbSettings *s, const EmbScene *sc, int64_t m, const int64_t *idx, double *out) {
+    World w;
+    w.s = *s;
+    w.cols = sc->cols;
+    w.rows = sc->rows;
+    const size_t cells = (size_t)w.cols * w.rows;
+    w.kind.assign(sc->kind, sc->kind + cells);
+    w.density.assign(sc->density, sc->density + cells);
+    const size_t n = (size_t)sc->n_electrons;
+    w.pos.assign(sc->pos, sc->pos + 2 * n);
+    w.charge.assign(n, -s->cell_charge / (double)s->carriers_per_cell);   // world.rs:59-66
+    CloseRange close;
+    close.s = *s;
+    const CellList list(w);
+#pragma omp parallel for schedule(dynamic, 64) num_threads(g_threads)
+    for (int64_t q = 0; q < m; q++) close.force_on(w, list, (size_t)idx[q], out + 2 * q);
 }
 
 void orc_sim_confine(const OrcSim *sim, const double *old_pos, const double *new_pos, const double *vel,

@@ -72,6 +72,10 @@ void orc_sim_node_fields(const OrcSim *sim, double *e, double *b);
 /* Simulation::potential(None): visible potential plus the close-range part. */
 void orc_sim_potential(OrcSim *sim, double *potential);
 void orc_sim_close_force(const OrcSim *sim, double *force);
+/* CloseRange::force for the electrons listed in idx only (m of them), on a
+ * scene built from `scene` without a field solve: the one-sided sum over every
+ * neighbour within the cutoff plus the ions, in FP64 (OpenMP over the list). */
+void orc_close_force_subset(const EmbSettings *s, const EmbScene *scene, int64_t m, const int64_t *idx, double *out);
 /* Stacks found in the scene: low/high terminals (c, r) and cell counts. */
 int orc_sim_stacks(const OrcSim *sim, int *low_high, int *n_cells);
 void orc_sim_labels(const OrcSim *sim, int64_t *labels);

@@ -74,6 +74,7 @@ def lib():
             "orc_sim_node_fields": (None, [vp, D, D]),
             "orc_sim_potential": (None, [vp, D]),
             "orc_sim_close_force": (None, [vp, D]),
+            "orc_close_force_subset": (None, [C.POINTER(abi.EmbSettings), C.POINTER(abi.EmbScene), C.c_int64, I64, D]),
             "orc_sim_stacks": (C.c_int, [vp, I32, I32]),
             "orc_sim_labels": (None, [vp, I64]),
             "orc_sim_confine": (None, [vp, D, D, D, C.c_double, D, D]),
@@ -226,6 +227,17 @@ def kick(vel, acc, damping, dt):
 
 def compact_g(r2, softening=0.25):
     return lib().orc_compact_g(float(r2), float(softening))
+
+
+def close_force_subset(world, idx) -> np.ndarray:
+    """CloseRange::force (close_range.rs:314-376) on the electrons `idx` of
+    `world` only, FP64, without building the field solver."""
+    idx = np.ascontiguousarray(idx, dtype=np.int64)
+    s = world.emb_settings()
+    holder = world.scene()
+    out = np.zeros((len(idx), 2))
+    lib().orc_close_force_subset(C.byref(s), holder.ref, len(idx), idx.ctypes.data_as(I64), ptr(out))
+    return out
 
 
 class Sim:

@@ -1,0 +1,92 @@
+"""The close-range (P3M particle-particle) force on the GPU against the FP64
+oracle's CloseRange::force (crates/embloks-core/src/close_range.rs:314-376):
+whole scenes at small size, and a sample of electrons of the 4096 x 4096,
+4M-electron clustered scene at cutoff 2."""
+import numpy as np
+import pytest
+
+from embloks_amd import world as W
+from oracle import orc
+
+gpu = pytest.mark.gpu
+
+
+def gpusim():
+    from embloks_amd.gpusim import GpuSim
+    return GpuSim
+
+
+def force_error(got, want):
+    """max |F - F_ref| / max |F_ref| over the electrons compared."""
+    return float(np.abs(got - want).max() / np.abs(want).max())
+
+
+# In neutral conductors an electron's close-range force is the small remainder
+# of its ions' attraction and its neighbours' repulsion (each ~10-100x larger),
+# summed over ~100-2000 FP32 pair terms: FP32 rounding of the parts, not of the
+# remainder, sets the error relative to max |F|.
+FORCE_TOL = 5e-5
+
+
+def check_forces(world, got, want, idx=None):
+    """Per-electron error against the oracle. The reference's pair force stops
+    at r = cutoff with g(cutoff^2) != 0 (close_range.rs:359-372), so a pair
+    within FP32 rounding of the cutoff is in the sum on one side and not the
+    other, a jump of one whole pair term: each electron over the tolerance
+    must own such a pair, and every other electron is held to FP32 rounding.
+    Returns (max error over the others, number of boundary outliers)."""
+    idx = np.arange(world.n_electrons) if idx is None else idx
+    err = np.abs(got - want).max(axis=1) / np.abs(want).max()
+    cut2 = world.settings["electron_cutoff"] ** 2
+    outliers = np.flatnonzero(err > FORCE_TOL)
+    for k in outliers:
+        d2 = ((world.pos - world.pos[idx[k]]) ** 2).sum(axis=1)
+        assert np.any(np.abs(d2 - cut2) < 1e-6 * cut2), (idx[k], err[k])
+    assert len(outliers) <= max(2, len(idx) // 1000), len(outliers)
+    rest = np.delete(err, outliers)
+    return float(rest.max()), len(outliers)
+
+
+@gpu
+@pytest.mark.parametrize("index", [0, 2, 5])
+def test_fixture_scenes_cutoff_3(index):
+    world = W.scene(index)
+    want = orc.Sim(world).close_force()
+    got, _ = gpusim()(world).close_force()
+    err, edge = check_forces(world, got, want)
+    print("scene", index, "force error", force_error(got, want), "without cutoff-boundary pairs", err, "boundary", edge)
+    assert err < FORCE_TOL, err
+
+
+@gpu
+@pytest.mark.parametrize("cutoff", [2.0, 3.0])
+def test_clustered_scene_small(cutoff):
+    # Dense clusters: cell mode from LDS and, where a window overflows the
+    # staging, from global memory; sparse background: electron mode.
+    world = W.clustered_charges(side=192, n_electrons=60000, n_clusters=12, min_spread=2.0, max_spread=8.0,
+                                cutoff=cutoff, seed=7)
+    want = orc.close_force_subset(world, np.arange(world.n_electrons))
+    got, _ = gpusim()(world).close_force()
+    err, edge = check_forces(world, got, want)
+    per_cell = world.density.max()
+    print("clustered 192^2 cutoff", cutoff, "max electrons per cell", per_cell, "force error", force_error(got, want),
+          "without cutoff-boundary pairs", err, "boundary", edge)
+    assert per_cell > 60   # some windows exceed the LDS staging
+    assert err < FORCE_TOL, err
+
+
+@gpu
+def test_clustered_4096_sampled():
+    world = W.clustered_charges()
+    sim = gpusim()(world)
+    got, ms = sim.close_force(repeats=3)
+    rng = np.random.default_rng(11)
+    cell = np.floor(world.pos).astype(np.int64)
+    counts = world.density[cell[:, 0], cell[:, 1]]
+    dense = np.flatnonzero(counts >= np.sort(counts)[-2000])[:2000]   # the densest cells' electrons
+    idx = np.unique(np.concatenate([rng.choice(world.n_electrons, 8000, replace=False), dense]))
+    want = orc.close_force_subset(world, idx)
+    err, edge = check_forces(world, got[idx], want, idx)
+    print("4096^2 clustered, 4M electrons: force eval", ms, "ms; sampled force error", force_error(got[idx], want),
+          "without cutoff-boundary pairs", err, "boundary", edge, "over", len(idx))
+    assert err < FORCE_TOL, err

@@ -364,3 +364,24 @@ def test_loop_current_follows_ohms_law():
         sim.step(15)
         samples.append(sim.current()[6:22, 3, 0].mean())
     assert abs(-np.mean(samples) / expected - 1) < 0.25
+
+
+@pytest.mark.parametrize("index", [0, 5])
+def test_subset_force_equals_the_full_force(index):
+    # The sampled one-sided sum (used against the 4M-electron GPU run) against
+    # CloseRange::force's Newton-3 loop over the whole scene.
+    w = W.scene(index)
+    full = orc.Sim(w).close_force()
+    sub = orc.close_force_subset(w, np.arange(w.n_electrons))
+    assert np.abs(full - sub).max() < 1e-12 * np.abs(full).max()
+    pick = np.array([3, 0, 17])
+    assert np.array_equal(orc.close_force_subset(w, pick), sub[pick])
+
+
+def test_clustered_scene_is_neutral_per_cell():
+    w = W.clustered_charges(side=128, n_electrons=20000, n_clusters=5, seed=2)
+    cell = np.floor(w.pos).astype(np.int64)
+    counts = np.zeros((128, 128))
+    np.add.at(counts, (cell[:, 0], cell[:, 1]), 1.0)
+    assert np.array_equal(counts, w.density)
+    assert np.array_equal(w.kind > 0, counts > 0)
