@@ -48,6 +48,7 @@ constexpr int MAX_STACKS_PER_BLOCK = 64;
 constexpr int SCAN_TILE = 1024;   // cells per block of the cell-list scan
 constexpr int TICK_TABLE = 64;    // ticks of host-computed AC phases / EMF targets
 constexpr int PUSH_TILE = 8;      // cells per side of a push tile
+constexpr int CELL_MODE_MIN = 6;  // a push tile with a cell this crowded runs the cell mode
 constexpr int FDTD_TI = 16, FDTD_TJ = 32;   // corners per FDTD tile (x, y)
 constexpr int ION_SUM = 33;       // samples per axis of the summed ion table
 constexpr int PAIR_SAMPLES = 4096;   // samples of the radial pair table (close_range.rs:30)
@@ -85,7 +86,7 @@ struct StackInfo {
 struct SceneCounters {
     uint32_t tick;            // ticks completed (parity selects ping-pong buffers)
     uint32_t vmax_bits[2];    // max |v| (float bits) of the electrons, by tick parity
-    uint32_t scan_ticket;     // last-block-done counter of the cell-list scan
+    uint32_t reserved_ctr;
     int32_t substeps;         // electron substeps of the current tick
     int32_t rest_error;       // set when the serial substep path rejected a scene
     uint32_t pad[2];
@@ -143,6 +144,7 @@ struct DevScene {
     int32_t *lstart;          // per cell: start within its scan tile
     int32_t *toff;            // per scan tile: offset of the tile
     int32_t *cstart;          // per cell + 1: absolute start in the sorted order
+    uint8_t *tile_dense;      // per push tile: a cell holds >= CELL_MODE_MIN electrons
     // --- deposit
     long long *cur;           // [2 * cells]: cell-binned sum of v, units 2^-28 (x q_e)
     const long long *static_rho;   // ions + fixed charges per cell, units 2^-44 electron charges

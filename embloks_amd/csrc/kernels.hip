@@ -34,22 +34,24 @@ __global__ __launch_bounds__(256) void k_count(Batch b) {
         const int s = scene_of(b, KC_ELECTRONS, blk);
         const DevScene &S = b.scenes[s];
         const int i = block_in_scene(b, KC_ELECTRONS, s, blk) * 256 + (int)threadIdx.x;
-        const int cell = i < S.n ? cell_of(S, S.cxy[i]) : -1;
-        // One atomic per distinct cell of the wave: electrons mostly arrive grouped
-        // by cell, so lanes of a cell share one counter update.
-        unsigned long long pending = __ballot(i < S.n);
-        int rank = 0;
-        while (pending) {
-            const int leader = __ffsll((long long)pending) - 1;
-            const int key = __shfl(cell, leader, 64);
-            const unsigned long long same = __ballot(cell == key) & pending;
-            int base = 0;
-            if (lane == leader) base = atomicAdd(&S.count[key], __popcll(same));
-            base = __shfl(base, leader, 64);
-            if ((same >> lane) & 1ull) rank = base + __popcll(same & ((1ull << lane) - 1ull));
-            pending &= ~same;
+        const bool valid = i < S.n;   // a prefix of the wave's lanes
+        const int cell = valid ? cell_of(S, S.cxy[i]) : -1;
+        // One atomic per run of consecutive lanes in the same cell (electrons mostly
+        // arrive grouped by cell), all runs of the wave at once.
+        const int prev = __shfl_up(cell, 1, 64);
+        const unsigned long long heads = __ballot(valid && (lane == 0 || prev != cell));
+        const int n_valid = __popcll(__ballot(valid));
+        const unsigned long long upto = (2ull << lane) - 1ull;   // lanes 0..lane
+        const int head = 63 - __clzll(heads & upto);
+        int base = 0;
+        if (valid && head == lane) {
+            const unsigned long long later = heads & ~upto;
+            const int next = later ? __ffsll((long long)later) - 1 : n_valid;
+            base = atomicAdd(&S.count[cell], next - lane);
         }
-        if (i < S.n) {
+        base = __shfl(base, valid ? head : 0, 64);
+        const int rank = base + (lane - head);
+        if (valid) {
             S.erank[i] = rank;
             S.ecell[i] = cell;
         }
@@ -79,13 +81,11 @@ __device__ __forceinline__ int block_scan_256(int v, int *wsum, int &total) {
 
 __global__ __launch_bounds__(256) void k_scan(Batch b) {
     __shared__ int wsum[4];
-    __shared__ int last;
     for (int blk = blockIdx.x; blk < b.total[KC_SCAN]; blk += gridDim.x) {
         const int s = scene_of(b, KC_SCAN, blk);
         const DevScene &S = b.scenes[s];
         const int tile = block_in_scene(b, KC_SCAN, s, blk);
         const int cells = S.cols * S.rows;
-        const int ntiles = (cells + SCAN_TILE - 1) / SCAN_TILE;
         const int base = tile * SCAN_TILE + (int)threadIdx.x * 4;
         int v[4], sum = 0;
         for (int q = 0; q < 4; q++) {
@@ -102,24 +102,29 @@ __global__ __launch_bounds__(256) void k_scan(Batch b) {
             run += v[q];
         }
         if (threadIdx.x == 0) S.toff[tile] = total;
-        __threadfence();
         __syncthreads();
-        if (threadIdx.x == 0) last = atomicAdd(&S.ctr->scan_ticket, 1u) == (unsigned)(ntiles - 1);
-        __syncthreads();
-        if (last) {
-            // The last tile of this scene turns the tile totals into offsets.
-            __threadfence();
-            int carry = 0;
-            for (int c0 = 0; c0 < ntiles; c0 += 256) {
-                const int k = c0 + (int)threadIdx.x;
-                const int t = k < ntiles ? __hip_atomic_load(&S.toff[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
-                int chunk_total;
-                const int ex = block_scan_256(t, wsum, chunk_total);
-                if (k < ntiles) S.toff[k] = carry + ex;
-                carry += chunk_total;
-            }
-            if (threadIdx.x == 0) S.ctr->scan_ticket = 0;
+    }
+}
+
+// One block per scene: the scan tiles' totals become their offsets, and the
+// dense-tile flags of the push are cleared for k_canon to set.
+__global__ __launch_bounds__(256) void k_scan_top(Batch b) {
+    __shared__ int wsum[4];
+    for (int blk = blockIdx.x; blk < b.total[KC_SCENE]; blk += gridDim.x) {
+        const int s = scene_of(b, KC_SCENE, blk);
+        const DevScene &S = b.scenes[s];
+        const int ntiles = (S.cols * S.rows + SCAN_TILE - 1) / SCAN_TILE;
+        int carry = 0;
+        for (int c0 = 0; c0 < ntiles; c0 += 256) {
+            const int k = c0 + (int)threadIdx.x;
+            const int t = k < ntiles ? S.toff[k] : 0;
+            int chunk_total;
+            const int ex = block_scan_256(t, wsum, chunk_total);
+            if (k < ntiles) S.toff[k] = carry + ex;
+            carry += chunk_total;
         }
+        const int ptiles = ((S.cols + PUSH_TILE - 1) / PUSH_TILE) * ((S.rows + PUSH_TILE - 1) / PUSH_TILE);
+        for (int q = threadIdx.x; q < ptiles; q += 256) S.tile_dense[q] = 0;
         __syncthreads();
     }
 }
@@ -177,6 +182,12 @@ __global__ __launch_bounds__(256) void k_canon(Batch b) {
         if (mine) {
             S.cstart[c] = s0;
             if (c == cells - 1) S.cstart[cells] = S.n;
+            // A push tile with a crowded cell runs the cell mode (k_push_tiled);
+            // the others go electron by electron (k_push_sparse).
+            if (s1 - s0 >= CELL_MODE_MIN) {
+                const int cc = c / S.rows, rr = c - cc * S.rows;
+                S.tile_dense[(cc / PUSH_TILE) * ((S.rows + PUSH_TILE - 1) / PUSH_TILE) + rr / PUSH_TILE] = 1;
+            }
         }
         if (S1 - S0 > CANON_CAP) {
             if (mine) canon_cell(S, c, s0, s1);
@@ -729,23 +740,20 @@ __device__ void force_core(const DevScene &S, const NB &nb, const float *gt, int
     force_out[S.order[slot]] = make_float2(fx + S.k_pair_fx * sx, fy + S.k_pair_fx * sy);
 }
 
-// Substep k for every electron, per PUSH_TILE x PUSH_TILE tile of cells (FORCE:
-// the close-range force alone, CloseRange::force, close_range.rs:314-376). The
-// positions of the tile's neighbourhood (reach cells around it) are staged in
-// LDS, in fixed point relative to the window's corner, when they fit. Two ways to run the pair sums:
-//   * cell mode (tiles with many electrons per occupied cell): a wave takes up
-//     to I electrons of one home cell, I a power of two, and splits the
-//     window's (2 reach + 1) column runs between 64 / I lanes per electron, so
-//     every lane of the wave walks the same neighbour list without divergence;
-//     the partial sums meet in a fixed butterfly;
-//   * electron mode (sparse tiles): one thread per electron with per-cell
-//     culling against the cutoff.
-// A window holding more than PUSH_STAGE electrons is read from global memory.
-// Every sum runs in a fixed order, so a scene gives the same bits alone or batched.
+// Substep k for the electrons of the crowded push tiles (tile_dense), per
+// PUSH_TILE x PUSH_TILE tile of cells (FORCE: the close-range force alone,
+// CloseRange::force, close_range.rs:314-376). The positions of the tile's
+// neighbourhood (reach cells around it) are staged in LDS, in fixed point
+// relative to the window's corner, when they fit (else read from the snapshot).
+// The pair sums run in cell mode: a wave takes up to I electrons of one home
+// cell, I a power of two, and splits the window's (2 reach + 1) column runs
+// between 64 / I lanes per electron, so every lane of the wave walks the same
+// neighbour list without divergence; the partial sums meet in a fixed
+// butterfly. The other tiles' electrons go through k_push_sparse. Every sum
+// runs in a fixed order, so a scene gives the same bits alone or batched.
 constexpr int PUSH_WIN = PUSH_TILE + 2 * 4;   // window side for reach <= 4
 constexpr int PUSH_STAGE = 3328;
 constexpr int PUSH_CELLS = PUSH_TILE * PUSH_TILE;
-constexpr int CELL_MODE_MIN = 6;              // electrons per occupied cell for the cell mode
 constexpr int PUSH_THREADS = 512;             // 8 waves: two blocks fill a CU's 16 wave slots at <= 128 VGPRs
 constexpr int PUSH_WAVES = PUSH_THREADS / 64;
 
@@ -755,9 +763,7 @@ __global__ __launch_bounds__(PUSH_THREADS) void k_push_tiled(Batch b, int k, flo
     __shared__ float gt[PAIR_SAMPLES + 4];
     __shared__ int lb[PUSH_WIN * PUSH_WIN], le[PUSH_WIN * PUSH_WIN];
     __shared__ int col_base[PUSH_WIN + 1], col_g0[PUSH_WIN];   // LDS base and first global slot per column
-    __shared__ int own_pref[PUSH_TILE + 1];                     // prefix of the tile's electrons per column
     __shared__ int item_pref[PUSH_CELLS + 1];                   // cell mode: prefix of work items per tile cell
-    __shared__ int cell_mode;
     bool table_loaded = false;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     for (int blk = blockIdx.x; blk < b.total[KC_PUSH]; blk += gridDim.x) {
@@ -774,7 +780,8 @@ __global__ __launch_bounds__(PUSH_THREADS) void k_push_tiled(Batch b, int k, flo
         const float h = S.dt / (float)sub;
         const int local = block_in_scene(b, KC_PUSH, s, blk);
         if (!FORCE && k == 0 && threadIdx.x == 0 && local == 0) S.ctr->substeps = sub;
-        if (S.close_on && !table_loaded) {   // the pair table is shared by every scene
+        if (!S.close_on || !S.tile_dense[local]) continue;
+        if (!table_loaded) {   // the pair table is shared by every scene
             for (int q = threadIdx.x; q < PAIR_SAMPLES + 4; q += PUSH_THREADS) gt[q] = S.g_table[q];
             table_loaded = true;
         }
@@ -809,15 +816,8 @@ __global__ __launch_bounds__(PUSH_THREADS) void k_push_tiled(Batch b, int k, flo
             const int I = lanes_per(m);
             const int items = m > 0 ? (m + I - 1) / I : 0;
             const int item_incl = wave_scan_incl(items);
-            const int own_incl = wave_scan_incl(m);
-            const int occupied = wave_scan_incl(m > 0 ? 1 : 0);
             item_pref[lane] = item_incl - items;
-            if (lane % PUSH_TILE == 0) own_pref[lane / PUSH_TILE] = own_incl - m;
-            if (lane == 63) {
-                item_pref[PUSH_CELLS] = item_incl;
-                own_pref[PUSH_TILE] = own_incl;
-                cell_mode = own_incl >= CELL_MODE_MIN * occupied ? 1 : 0;
-            }
+            if (lane == 63) item_pref[PUSH_CELLS] = item_incl;
         }
         __syncthreads();
         const bool staged = col_base[W] <= PUSH_STAGE;
@@ -834,73 +834,76 @@ __global__ __launch_bounds__(PUSH_THREADS) void k_push_tiled(Batch b, int k, flo
                     le[q] = col_base[ci] + (S.cstart[cell + 1] - col_g0[ci]);
                 }
             }
-            for (int q = 0; q < W; q++) {
-                const int n = col_base[q + 1] - col_base[q];
-                for (int j = threadIdx.x; j < n; j += PUSH_THREADS) {
-                    const int g = col_g0[q] + j;
-                    const int rr = S.scell[g] - (wc0 + q) * S.rows;
-                    lp[col_base[q] + j] = fixed_pos(q, rr - wr0, S.sfx[g], S.sfy[g]);
-                }
+            // One pass over every staged electron (column found by search), so a
+            // sparse window costs one round of loads rather than one per column.
+            const int staged_n = col_base[W];
+            for (int j = threadIdx.x; j < staged_n; j += PUSH_THREADS) {
+                int q = 0;
+                for (int step = PUSH_WIN / 2; step > 0; step >>= 1)
+                    if (q + step < W && col_base[q + step] <= j) q += step;
+                const int g = col_g0[q] + (j - col_base[q]);
+                const int rr = S.scell[g] - (wc0 + q) * S.rows;
+                lp[j] = fixed_pos(q, rr - wr0, S.sfx[g], S.sfy[g]);
             }
         }
         __syncthreads();
-        if (S.close_on && cell_mode) {
-            const int n_items = item_pref[PUSH_CELLS];
-            for (int it = wave; it < n_items; it += PUSH_WAVES) {
-                int lo = 0, hi = PUSH_CELLS;
-                while (hi - lo > 1) {
-                    const int mid = (lo + hi) >> 1;
-                    if (item_pref[mid] <= it) lo = mid;
-                    else hi = mid;
-                }
-                lo = __builtin_amdgcn_readfirstlane(lo);
-                const int hc = tc0 + lo / PUSH_TILE, hr = tr0 + lo % PUSH_TILE;
-                const int chunk = it - item_pref[lo];
-                if (staged)
-                    cell_item<FORCE>(S, LdsRuns{lp, lb, le, wc0, wr0, W}, gt, hc, hr, chunk, wc0, wr0, k, h, t, force_out);
-                else
-                    cell_item<FORCE>(S, GlobalRuns{S, wc0, wr0}, gt, hc, hr, chunk, wc0, wr0, k, h, t, force_out);
+        const int n_items = item_pref[PUSH_CELLS];
+        for (int it = wave; it < n_items; it += PUSH_WAVES) {
+            int lo = 0, hi = PUSH_CELLS;
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (item_pref[mid] <= it) lo = mid;
+                else hi = mid;
             }
-        } else {
-            const int n_own = own_pref[PUSH_TILE];
-            for (int e = threadIdx.x; e < n_own; e += PUSH_THREADS) {
-                int q = 0;
-                while (own_pref[q + 1] <= e) q++;
-                const int cc = tc0 + q;
-                const int slot = S.cstart[cc * S.rows + tr0] + (e - own_pref[q]);
-                if (FORCE) {
-                    if (staged)
-                        force_core(S, LdsNeighbours{lp, lb, le, wc0, wr0, W}, gt, slot, force_out);
-                    else
-                        force_core(S, GlobalNeighbours{S}, gt, slot, force_out);
-                } else {
-                    if (staged)
-                        push_core(S, LdsNeighbours{lp, lb, le, wc0, wr0, W}, gt, slot, k, h, t);
-                    else
-                        push_core(S, GlobalNeighbours{S}, gt, slot, k, h, t);
-                }
-            }
+            lo = __builtin_amdgcn_readfirstlane(lo);
+            const int hc = tc0 + lo / PUSH_TILE, hr = tr0 + lo % PUSH_TILE;
+            const int chunk = it - item_pref[lo];
+            if (staged)
+                cell_item<FORCE>(S, LdsRuns{lp, lb, le, wc0, wr0, W}, gt, hc, hr, chunk, wc0, wr0, k, h, t, force_out);
+            else
+                cell_item<FORCE>(S, GlobalRuns{S, wc0, wr0}, gt, hc, hr, chunk, wc0, wr0, k, h, t, force_out);
         }
         __syncthreads();
     }
 }
 
-template __global__ void k_push_tiled<false>(Batch, int, float2 *);
-template __global__ void k_push_tiled<true>(Batch, int, float2 *);
-
-__global__ __launch_bounds__(256) void k_push(Batch b, int k) {
+// Substep k (FORCE: the close-range force) for the electrons of the other push
+// tiles, one thread per electron in sorted order, neighbours read from the
+// snapshot in global memory with per-cell culling against the cutoff. With few
+// electrons per cell there is little to share between neighbours, and a tile
+// would be a chain of dependent loads for a handful of threads.
+template <bool FORCE>
+__global__ __launch_bounds__(256) void k_push_sparse(Batch b, int k, float2 *force_out) {
     for (int blk = blockIdx.x; blk < b.total[KC_ELECTRONS]; blk += gridDim.x) {
         const int s = scene_of(b, KC_ELECTRONS, blk);
         const DevScene &S = b.scenes[s];
         const uint32_t t = S.ctr->tick;
-        const int sub = substeps_of(S, t);
-        if (k >= sub) continue;
+        int sub = 1;
+        if (!FORCE) {
+            sub = substeps_of(S, t);
+            if (k >= sub) continue;
+        } else if (!S.close_on) {
+            continue;
+        }
         const float h = S.dt / (float)sub;
         const int slot = block_in_scene(b, KC_ELECTRONS, s, blk) * 256 + (int)threadIdx.x;
-        if (slot == 0 && k == 0) S.ctr->substeps = sub;
-        if (slot < S.n) push_one(S, slot, k, h, t);
+        if (!FORCE && k == 0 && slot == 0) S.ctr->substeps = sub;
+        if (slot >= S.n) continue;
+        const int cell = S.scell[slot];
+        const int cc = cell / S.rows, rr = cell - (cell / S.rows) * S.rows;
+        if (S.close_on && S.tile_dense[(cc / PUSH_TILE) * ((S.rows + PUSH_TILE - 1) / PUSH_TILE) + rr / PUSH_TILE]) continue;
+        if (FORCE)
+            force_core(S, GlobalNeighbours{S}, S.g_table, slot, force_out);
+        else
+            push_core(S, GlobalNeighbours{S}, S.g_table, slot, k, h, t);
     }
 }
+
+template __global__ void k_push_sparse<false>(Batch, int, float2 *);
+template __global__ void k_push_sparse<true>(Batch, int, float2 *);
+
+template __global__ void k_push_tiled<false>(Batch, int, float2 *);
+template __global__ void k_push_tiled<true>(Batch, int, float2 *);
 
 // Substeps 1.. of a tick that needs several (fast electrons): one 1024-thread
 // block per scene re-sorts and pushes serially. Rare (electrons drift far below
@@ -1325,6 +1328,7 @@ static inline int grid_of(int total) { return total < 8192 ? total : 8192; }
 hipError_t sort_kernels(const Batch &b, hipStream_t stream) {
     EMB_LAUNCH(k_count, KC_ELECTRONS, 256, b);
     EMB_LAUNCH(k_scan, KC_SCAN, 256, b);
+    EMB_LAUNCH(k_scan_top, KC_SCENE, 256, b);
     EMB_LAUNCH(k_scatter, KC_ELECTRONS, 256, b);
     EMB_LAUNCH(k_canon, KC_CELLS, 256, b);
     return hipGetLastError();
@@ -1354,12 +1358,14 @@ hipError_t tick_kernels(const Batch &b, int fdtd_substeps, hipStream_t stream, h
     EMB_LAUNCH(k_trim, KC_STACKS, 1024, b);
     EMB_MARK(1);
     EMB_LAUNCH(k_push_tiled<false>, KC_PUSH, PUSH_THREADS, b, 0, nullptr);
+    EMB_LAUNCH(k_push_sparse<false>, KC_ELECTRONS, 256, b, 0, nullptr);
     EMB_MARK(2);
     if (b.total[KC_ELECTRONS] > 0) EMB_LAUNCH(k_rest, KC_SCENE, 1024, b);
     EMB_MARK(3);
     EMB_LAUNCH(k_count, KC_ELECTRONS, 256, b);
     EMB_MARK(4);
     EMB_LAUNCH(k_scan, KC_SCAN, 256, b);
+    EMB_LAUNCH(k_scan_top, KC_SCENE, 256, b);
     EMB_MARK(5);
     EMB_LAUNCH(k_scatter, KC_ELECTRONS, 256, b);
     EMB_MARK(6);
@@ -1394,6 +1400,7 @@ hipError_t diagnostics_kernel(const DevScene *scene, double c2, double *out, int
 // current (sorted) positions, written by electron index.
 hipError_t close_force_kernel(const Batch &b, float2 *out, hipStream_t stream) {
     EMB_LAUNCH(k_push_tiled<true>, KC_PUSH, PUSH_THREADS, b, 0, out);
+    EMB_LAUNCH(k_push_sparse<true>, KC_ELECTRONS, 256, b, 0, out);
     return hipGetLastError();
 }
 

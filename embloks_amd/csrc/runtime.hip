@@ -79,6 +79,7 @@ void layout(Bump &a, const Counts &c, DevScene &d, DevScene **d_scene, int32_t *
     d.lstart = a.take<int32_t>(cells);
     d.toff = a.take<int32_t>((cells + SCAN_TILE - 1) / SCAN_TILE);
     d.cstart = a.take<int32_t>(cells + 1);
+    d.tile_dense = a.take<uint8_t>((size_t)((c.cols + PUSH_TILE - 1) / PUSH_TILE) * ((c.rows + PUSH_TILE - 1) / PUSH_TILE));
     d.cur = a.take<long long>(2 * cells);
     d.static_rho = a.take<long long>(cells);
     d.info = a.take<CellInfo>(cells);
