@@ -8,6 +8,12 @@
 // batch is captured once as a HIP graph and replayed until the batch changes.
 // In plain mode each scene's tick is launched on its own, one scene per launch:
 // the same kernels, the comparison the batched path is measured against.
+//
+// Rounds are pipelined: a poll launches its round and returns without waiting;
+// the next poll first admits new requests (uploads and set-up on a second
+// stream, overlapping the running round), then waits for the round, accounts
+// and retires, admits into the freed slots and launches the next round. Scene
+// preparation in submit (host FP64 work) thus overlaps the GPU too.
 
 #include <algorithm>
 #include <chrono>
@@ -64,6 +70,12 @@ struct EmbServer {
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipStream_t up_stream = nullptr;           // admissions: uploads and set-up kernels
+    hipEvent_t up_event = nullptr;
+    std::vector<Request *> inflight;           // the launched round's scenes
+    int64_t inflight_k = 0;                    // and its tick count
+    std::vector<DevScene> host_scenes;         // sources of the scene-table copies
+    std::vector<int32_t> host_prefix;
     // stats
     std::vector<float> latencies_ms;   // one sample per scene-step
     int64_t cell_steps = 0, completed = 0;
@@ -84,8 +96,10 @@ int drop_graph(EmbServer *sv) {
 // Scene table and block prefixes of the resident scenes.
 int rebuild_batch(EmbServer *sv) {
     const int n = (int)sv->active.size();
-    std::vector<DevScene> scenes;
-    std::vector<int32_t> prefix((size_t)KC_COUNT * (n + 1), 0);
+    std::vector<DevScene> &scenes = sv->host_scenes;
+    std::vector<int32_t> &prefix = sv->host_prefix;
+    scenes.clear();
+    prefix.assign((size_t)KC_COUNT * (n + 1), 0);
     sv->fdtd_max = 1;
     for (int k = 0; k < KC_COUNT; k++) sv->batch.total[k] = 0;
     for (int i = 0; i < n; i++) {
@@ -98,9 +112,10 @@ int rebuild_batch(EmbServer *sv) {
     }
     for (int k = 0; k < KC_COUNT; k++) sv->batch.total[k] = prefix[(size_t)k * (n + 1) + n];
     if (n > 0) {
+        // Stream-ordered after every earlier round; the sources live in the server
+        // until the next rebuild, which comes after this round has been waited for.
         EMB_HIP(hipMemcpyAsync(sv->d_scenes, scenes.data(), scenes.size() * sizeof(DevScene), hipMemcpyHostToDevice, sv->stream));
         EMB_HIP(hipMemcpyAsync(sv->d_prefix, prefix.data(), prefix.size() * 4, hipMemcpyHostToDevice, sv->stream));
-        EMB_HIP(hipStreamSynchronize(sv->stream));
     }
     sv->batch.scenes = sv->d_scenes;
     sv->batch.prefix = sv->d_prefix;
@@ -169,6 +184,8 @@ int emb_server_create(EmbDevice *dev, const EmbSettings *settings, const EmbServ
         sv->s = *settings;
         sv->cfg = *config;
         EMB_HIP(hipStreamCreateWithFlags(&sv->stream, hipStreamNonBlocking));
+        EMB_HIP(hipStreamCreateWithFlags(&sv->up_stream, hipStreamNonBlocking));
+        EMB_HIP(hipEventCreateWithFlags(&sv->up_event, hipEventDisableTiming));
         EMB_HIP(hipMalloc(&sv->d_scenes, sizeof(DevScene) * config->max_slots));
         EMB_HIP(hipMalloc(&sv->d_prefix, sizeof(int32_t) * KC_COUNT * (config->max_slots + 1)));
         EMB_HIP(hipEventCreate(&sv->ev0));
@@ -203,109 +220,154 @@ int emb_server_submit(EmbServer *sv, const EmbScene *scene, int64_t steps, int64
     }
 }
 
-int emb_server_cancel(EmbServer *sv, int64_t id) {
-    if (!sv) return fail(EMB_E_INVALID, "null server");
-    auto it = sv->all.find(id);
-    if (it == sv->all.end()) return fail(EMB_E_INVALID, "no such request");
-    Request *q = it->second.get();
-    if (q->state == QUEUED) {
-        sv->queue.erase(std::find(sv->queue.begin(), sv->queue.end(), q));
-    } else if (q->state == RUNNING) {
-        EMB_HIP(hipStreamSynchronize(sv->stream));
-        sv->active.erase(std::find(sv->active.begin(), sv->active.end(), q));
-        q->state = CANCELLED;
-        int rc = retire(sv, q);
-        if (rc != EMB_OK) return rc;
+}  // extern "C"
+
+namespace {
+
+// Admit queued requests, in arrival order, while a slot and the budget allow:
+// upload and set up on the admission stream; the main stream waits for it.
+int admit(EmbServer *sv) {
+    bool any = false;
+    while (!sv->queue.empty() && (int)sv->active.size() < sv->cfg.max_slots) {
+        Request *q = sv->queue.front();
+        if ((int64_t)(sv->used_bytes + q->footprint) > sv->cfg.hbm_budget_bytes) break;
+        sv->queue.pop_front();
+        int rc = make_resident(sv->dev, sv->s, std::move(q->hs), sv->up_stream, q->r);
+        if (rc != EMB_OK) {
+            free_resident(q->r);
+            q->state = CANCELLED;
+            return rc;
+        }
+        q->r.time = q->time0;
+        EMB_HIP(setup_kernels(q->r.batch, sv->up_stream));
+        sv->used_bytes += q->footprint;
+        q->state = RUNNING;
+        any = true;
+        if (q->steps == 0) {   // nothing to run: finished on admission
+            EMB_HIP(hipStreamSynchronize(sv->up_stream));
+            int rc2 = retire(sv, q);
+            q->state = DONE;
+            sv->completed++;
+            if (rc2 != EMB_OK) return rc2;
+            continue;
+        }
+        sv->active.push_back(q);
         sv->dirty = true;
     }
-    q->state = CANCELLED;
+    if (any) {
+        EMB_HIP(hipEventRecord(sv->up_event, sv->up_stream));
+        EMB_HIP(hipStreamWaitEvent(sv->stream, sv->up_event, 0));
+    }
+    sv->peak = std::max(sv->peak, (int)sv->active.size());
     return EMB_OK;
+}
+
+// Wait for the launched round, account its steps and latency, retire what finished.
+int finish_round(EmbServer *sv) {
+    if (sv->inflight.empty()) return EMB_OK;
+    EMB_HIP(hipEventSynchronize(sv->ev1));
+    float ms = 0.0f;
+    EMB_HIP(hipEventElapsedTime(&ms, sv->ev0, sv->ev1));
+    const int64_t k = sv->inflight_k;
+    const float per_step = ms / (float)k;
+    for (Request *q : sv->inflight) {
+        q->done += k;
+        sv->cell_steps += (int64_t)q->r.ds.nx * q->r.ds.ny * k;
+        for (int64_t t = 0; t < k; t++) sv->latencies_ms.push_back(per_step);
+    }
+    sv->inflight.clear();
+    std::vector<Request *> keep;
+    for (Request *q : sv->active) {
+        if (q->done >= q->steps) {
+            int rc = retire(sv, q);
+            q->state = DONE;
+            sv->completed++;
+            sv->dirty = true;
+            if (rc != EMB_OK) return rc;
+        } else {
+            keep.push_back(q);
+        }
+    }
+    sv->active.swap(keep);
+    sv->t_last = now_s();
+    return EMB_OK;
+}
+
+// Launch k ticks of every resident scene, without waiting.
+int launch_round(EmbServer *sv) {
+    if (sv->active.empty()) return EMB_OK;
+    if (sv->dirty) {
+        int rc = rebuild_batch(sv);
+        if (rc != EMB_OK) return rc;
+    }
+    int64_t k = sv->cfg.ticks_per_round;
+    for (Request *q : sv->active) k = std::min(k, q->steps - q->done);
+    for (Request *q : sv->active) {
+        int rc = enqueue_tick_tables(q->r, (int)k, sv->stream);
+        if (rc != EMB_OK) return rc;
+    }
+    const bool graphs = sv->cfg.batched && sv->cfg.use_graph;
+    if (graphs && !sv->graph_exec) {
+        int rc = capture_graph(sv);
+        if (rc != EMB_OK) return rc;
+    }
+    EMB_HIP(hipEventRecord(sv->ev0, sv->stream));
+    for (int64_t t = 0; t < k; t++) {
+        if (graphs) {
+            EMB_HIP(hipGraphLaunch(sv->graph_exec, sv->stream));
+        } else if (sv->cfg.batched) {
+            EMB_HIP(tick_kernels(sv->batch, sv->fdtd_max, sv->stream));
+        } else {
+            for (Request *q : sv->active) EMB_HIP(tick_kernels(q->r.batch, q->r.ds.fdtd_substeps, sv->stream));
+        }
+    }
+    EMB_HIP(hipEventRecord(sv->ev1, sv->stream));
+    sv->inflight = sv->active;
+    sv->inflight_k = k;
+    return EMB_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int emb_server_cancel(EmbServer *sv, int64_t id) {
+    try {
+        if (!sv) return fail(EMB_E_INVALID, "null server");
+        auto it = sv->all.find(id);
+        if (it == sv->all.end()) return fail(EMB_E_INVALID, "no such request");
+        Request *q = it->second.get();
+        if (q->state == QUEUED) {
+            sv->queue.erase(std::find(sv->queue.begin(), sv->queue.end(), q));
+        } else if (q->state == RUNNING) {
+            EMB_HIP(hipSetDevice(sv->dev->ordinal));
+            int rc = finish_round(sv);   // settle the running round first
+            if (rc != EMB_OK) return rc;
+            if (q->state == RUNNING) {
+                EMB_HIP(hipStreamSynchronize(sv->up_stream));
+                sv->active.erase(std::find(sv->active.begin(), sv->active.end(), q));
+                q->state = CANCELLED;
+                rc = retire(sv, q);
+                if (rc != EMB_OK) return rc;
+                sv->dirty = true;
+            }
+        }
+        if (q->state != DONE) q->state = CANCELLED;
+        return EMB_OK;
+    } catch (const std::exception &e) {
+        return fail(EMB_E_INVALID, e.what());
+    }
 }
 
 int emb_server_poll(EmbServer *sv, int32_t *active_out) {
     try {
         if (!sv) return fail(EMB_E_INVALID, "null server");
         EMB_HIP(hipSetDevice(sv->dev->ordinal));
-        // Admission by footprint: in arrival order, while a slot and the budget allow.
-        while (!sv->queue.empty() && (int)sv->active.size() < sv->cfg.max_slots) {
-            Request *q = sv->queue.front();
-            if ((int64_t)(sv->used_bytes + q->footprint) > sv->cfg.hbm_budget_bytes) break;
-            sv->queue.pop_front();
-            int rc = make_resident(sv->dev, sv->s, std::move(q->hs), sv->stream, q->r);
-            if (rc != EMB_OK) {
-                free_resident(q->r);
-                q->state = CANCELLED;
-                return rc;
-            }
-            q->r.time = q->time0;
-            EMB_HIP(setup_kernels(q->r.batch, sv->stream));
-            sv->used_bytes += q->footprint;
-            q->state = RUNNING;
-            if (q->steps == 0) {   // nothing to run: finished on admission
-                EMB_HIP(hipStreamSynchronize(sv->stream));
-                q->state = RUNNING;
-                int rc2 = retire(sv, q);
-                q->state = DONE;
-                sv->completed++;
-                if (rc2 != EMB_OK) return rc2;
-                continue;
-            }
-            sv->active.push_back(q);
-            sv->dirty = true;
-        }
-        sv->peak = std::max(sv->peak, (int)sv->active.size());
-        if (!sv->active.empty()) {
-            if (sv->dirty) {
-                int rc = rebuild_batch(sv);
-                if (rc != EMB_OK) return rc;
-            }
-            int64_t k = sv->cfg.ticks_per_round;
-            for (Request *q : sv->active) k = std::min(k, q->steps - q->done);
-            for (Request *q : sv->active) {
-                int rc = enqueue_tick_tables(q->r, (int)k, sv->stream);
-                if (rc != EMB_OK) return rc;
-            }
-            const bool graphs = sv->cfg.batched && sv->cfg.use_graph;
-            if (graphs && !sv->graph_exec) {
-                int rc = capture_graph(sv);
-                if (rc != EMB_OK) return rc;
-            }
-            EMB_HIP(hipEventRecord(sv->ev0, sv->stream));
-            for (int64_t t = 0; t < k; t++) {
-                if (graphs) {
-                    EMB_HIP(hipGraphLaunch(sv->graph_exec, sv->stream));
-                } else if (sv->cfg.batched) {
-                    EMB_HIP(tick_kernels(sv->batch, sv->fdtd_max, sv->stream));
-                } else {
-                    for (Request *q : sv->active) EMB_HIP(tick_kernels(q->r.batch, q->r.ds.fdtd_substeps, sv->stream));
-                }
-            }
-            EMB_HIP(hipEventRecord(sv->ev1, sv->stream));
-            EMB_HIP(hipEventSynchronize(sv->ev1));
-            float ms = 0.0f;
-            EMB_HIP(hipEventElapsedTime(&ms, sv->ev0, sv->ev1));
-            const float per_step = ms / (float)k;
-            for (Request *q : sv->active) {
-                q->done += k;
-                sv->cell_steps += (int64_t)q->r.ds.nx * q->r.ds.ny * k;
-                for (int64_t t = 0; t < k; t++) sv->latencies_ms.push_back(per_step);
-            }
-            // Retire finished requests.
-            std::vector<Request *> keep;
-            for (Request *q : sv->active) {
-                if (q->done >= q->steps) {
-                    int rc = retire(sv, q);
-                    q->state = DONE;
-                    sv->completed++;
-                    sv->dirty = true;
-                    if (rc != EMB_OK) return rc;
-                } else {
-                    keep.push_back(q);
-                }
-            }
-            sv->active.swap(keep);
-            sv->t_last = now_s();
-        }
+        int rc = admit(sv);                  // overlaps the running round
+        if (rc == EMB_OK) rc = finish_round(sv);
+        if (rc == EMB_OK) rc = admit(sv);    // into the slots just freed
+        if (rc == EMB_OK) rc = launch_round(sv);
+        if (rc != EMB_OK) return rc;
         if (active_out) *active_out = (int32_t)(sv->active.size() + sv->queue.size());
         return EMB_OK;
     } catch (const std::exception &e) {
@@ -369,12 +431,15 @@ int emb_server_destroy(EmbServer *sv) {
     if (!sv) return EMB_OK;
     (void)hipSetDevice(sv->dev->ordinal);
     (void)hipStreamSynchronize(sv->stream);
+    (void)hipStreamSynchronize(sv->up_stream);
     for (Request *q : sv->active) free_resident(q->r);
     drop_graph(sv);
     if (sv->d_scenes) (void)hipFree(sv->d_scenes);
     if (sv->d_prefix) (void)hipFree(sv->d_prefix);
     if (sv->ev0) (void)hipEventDestroy(sv->ev0);
     if (sv->ev1) (void)hipEventDestroy(sv->ev1);
+    if (sv->up_event) (void)hipEventDestroy(sv->up_event);
+    (void)hipStreamDestroy(sv->up_stream);
     (void)hipStreamDestroy(sv->stream);
     delete sv;
     return EMB_OK;
