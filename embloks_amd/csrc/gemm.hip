@@ -15,7 +15,9 @@
 //   * BF16x3: each operand split as x = hi + lo (hi = bf16(x), lo = bf16(x - hi)),
 //     A.B ~ lo.hi + hi.lo + hi.hi with v_mfma_f32_16x16x32_bf16 and FP32
 //     accumulation (the lo.lo term is dropped). Offered for the isolated solve;
-//     its error is measured against FP32 and FP64.
+//     its error is measured against FP32 and FP64. The operands are split (and
+//     transposed where the contraction runs over rows) by an O(n^2) pass before
+//     each product, so the O(n^3) kernel only loads bf16 and multiplies.
 // Both use 256 threads as a 2 x 2 grid of waves, operands staged through LDS
 // in K slices with the next slice prefetched into registers.
 
@@ -131,105 +133,6 @@ __device__ __forceinline__ void tile_f32(const GemmArgs &g, int m0, int n0, floa
 }
 
 // ---------------------------------------------------------------------------------------
-// BF16x3 tile: BM x BN output, K slices of 32 (one MFMA), LDS rows K-contiguous:
-// A as [m][k], B transposed to [n][k], hi and lo halves side by side.
-
-template <int BM, int BN>
-__device__ __forceinline__ void tile_bf16x3(const GemmArgs &g, int m0, int n0, __bf16 *lds) {
-    constexpr int GK = 32, P = GK + 8;   // row pitch in bf16: 80 bytes keeps 16-byte reads aligned
-    constexpr int WM = BM / 2, WN = BN / 2, TM = WM / 16, TN = WN / 16;
-    constexpr int AV = BM * GK / 4 / 256, BV = BN * GK / 4 / 256;
-    static_assert(AV >= 1 && BV >= 1, "tile too small for 256 threads");
-    __bf16 *Ah = lds, *Al = Ah + BM * P, *Bh = Al + BM * P, *Bl = Bh + BN * P;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wm = (wave >> 1) * WM, wn = (wave & 1) * WN;
-    const int l16 = lane & 15, kq = lane >> 4;
-    const bool avec = (g.lda & 3) == 0 && (reinterpret_cast<uintptr_t>(g.A) & 15) == 0;
-    const bool bvec = (g.ldb & 3) == 0 && (reinterpret_cast<uintptr_t>(g.B) & 15) == 0;
-    floatx4 acc[TM][TN];
-#pragma unroll
-    for (int x = 0; x < TM; x++)
-#pragma unroll
-        for (int y = 0; y < TN; y++) acc[x][y] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
-    float4 ra[AV], rb[BV];
-    auto load = [&](int k0) {
-#pragma unroll
-        for (int v = 0; v < AV; v++) {
-            const int idx = tid + 256 * v, row = idx >> 3, kk = (idx & 7) * 4;
-            const int gr = m0 + row, gk = k0 + kk;
-            ra[v] = gr < g.M ? load4(g.A + (size_t)gr * g.lda + gk, g.K - gk, avec) : float4{0.0f, 0.0f, 0.0f, 0.0f};
-        }
-#pragma unroll
-        for (int v = 0; v < BV; v++) {
-            const int idx = tid + 256 * v, kr = idx / (BN / 4), col = (idx % (BN / 4)) * 4;
-            const int gk = k0 + kr, gc = n0 + col;
-            rb[v] = gk < g.K ? load4(g.B + (size_t)gk * g.ldb + gc, g.N - gc, bvec) : float4{0.0f, 0.0f, 0.0f, 0.0f};
-        }
-    };
-    auto split = [](float x, __bf16 &h, __bf16 &l) {
-        h = (__bf16)x;
-        l = (__bf16)(x - (float)h);
-    };
-    auto store = [&]() {
-#pragma unroll
-        for (int v = 0; v < AV; v++) {
-            const int idx = tid + 256 * v, row = idx >> 3, kk = (idx & 7) * 4;
-            const float e[4] = {ra[v].x, ra[v].y, ra[v].z, ra[v].w};
-#pragma unroll
-            for (int q = 0; q < 4; q++) split(e[q], Ah[row * P + kk + q], Al[row * P + kk + q]);
-        }
-#pragma unroll
-        for (int v = 0; v < BV; v++) {
-            const int idx = tid + 256 * v, kr = idx / (BN / 4), col = (idx % (BN / 4)) * 4;
-            const float e[4] = {rb[v].x, rb[v].y, rb[v].z, rb[v].w};
-#pragma unroll
-            for (int q = 0; q < 4; q++) split(e[q], Bh[(col + q) * P + kr], Bl[(col + q) * P + kr]);
-        }
-    };
-    load(0);
-    for (int k0 = 0; k0 < g.K; k0 += GK) {
-        store();
-        __syncthreads();
-        if (k0 + GK < g.K) load(k0 + GK);
-        bf16x8 ah[TM], al[TM], bh[TN], bl[TN];
-#pragma unroll
-        for (int x = 0; x < TM; x++) {
-            const int r = (wm + x * 16 + l16) * P + kq * 8;
-            ah[x] = *reinterpret_cast<const bf16x8 *>(Ah + r);
-            al[x] = *reinterpret_cast<const bf16x8 *>(Al + r);
-        }
-#pragma unroll
-        for (int y = 0; y < TN; y++) {
-            const int r = (wn + y * 16 + l16) * P + kq * 8;
-            bh[y] = *reinterpret_cast<const bf16x8 *>(Bh + r);
-            bl[y] = *reinterpret_cast<const bf16x8 *>(Bl + r);
-        }
-#pragma unroll
-        for (int x = 0; x < TM; x++)
-#pragma unroll
-            for (int y = 0; y < TN; y++) {
-                acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[x], bh[y], acc[x][y], 0, 0, 0);
-                acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[x], bl[y], acc[x][y], 0, 0, 0);
-                acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[x], bh[y], acc[x][y], 0, 0, 0);
-            }
-        __syncthreads();
-    }
-#pragma unroll
-    for (int x = 0; x < TM; x++)
-#pragma unroll
-        for (int y = 0; y < TN; y++)
-#pragma unroll
-            for (int i = 0; i < 4; i++) {
-                const int row = m0 + wm + x * 16 + kq * 4 + i, col = n0 + wn + y * 16 + l16;
-                if (row < g.M && col < g.N) {
-                    float v = acc[x][y][i];
-                    if (g.scale) v *= g.scale[(size_t)row * g.ldc + col];
-                    g.C[(size_t)row * g.ldc + col] = v;
-                }
-            }
-}
-
-// ---------------------------------------------------------------------------------------
 // The tick's solve over a batch of scenes: blocks walk (scene, channel, tile).
 
 constexpr int TICK_BM = 64, TICK_BN = 64;
@@ -295,28 +198,184 @@ __global__ __launch_bounds__(256) void k_gemm_f32(GemmArgs g) {
     }
 }
 
-template <int BM, int BN>
-__global__ __launch_bounds__(256) void k_gemm_bf16x3(GemmArgs g) {
-    __shared__ __align__(16) __bf16 lds[2 * (BM + BN) * 40];
-    const int tn = (g.N + BN - 1) / BN, tiles = ((g.M + BM - 1) / BM) * tn;
-    for (int blk = blockIdx.x; blk < tiles; blk += gridDim.x) {
-        tile_bf16x3<BM, BN>(g, (blk / tn) * BM, (blk % tn) * BN, lds);
-        __syncthreads();
-    }
-}
-
-hipError_t launch_gemm(const GemmArgs &g, int precision, int tile, hipStream_t stream) {
+hipError_t launch_gemm(const GemmArgs &g, int tile, hipStream_t stream) {
     const int t = tile == 128 ? 128 : 64;
     const long long tiles = (long long)((g.M + t - 1) / t) * ((g.N + t - 1) / t);
     const int grid = (int)(tiles < 8192 ? tiles : 8192);
     if (grid == 0) return hipSuccess;
-    if (precision == GEMM_BF16X3) {
-        if (t == 128) k_gemm_bf16x3<128, 128><<<grid, 256, 0, stream>>>(g);
-        else k_gemm_bf16x3<64, 64><<<grid, 256, 0, stream>>>(g);
-    } else {
-        if (t == 128) k_gemm_f32<128, 128><<<grid, 256, 0, stream>>>(g);
-        else k_gemm_f32<64, 64><<<grid, 256, 0, stream>>>(g);
+    if (t == 128) k_gemm_f32<128, 128><<<grid, 256, 0, stream>>>(g);
+    else k_gemm_f32<64, 64><<<grid, 256, 0, stream>>>(g);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------------------
+// BF16x3: x = hi + lo with hi = bf16(x) and lo = bf16(x - hi); a product keeps
+// hi.hi + hi.lo + lo.hi (the lo.lo term, ~2^-16 of it, is dropped).
+
+// Split an FP32 rows x cols matrix (pitch ld) into bf16 halves with rows of
+// pitch ldd (a multiple of 32, zero past the data): straight, dst[r][c], or
+// TRANSPOSE, dst[c][r]. 32 x 32 tiles through LDS keep both sides coalesced.
+template <bool TRANSPOSE>
+__global__ __launch_bounds__(256) void k_split(const float *src, int rows, int cols, int ld, __bf16 *hi, __bf16 *lo,
+                                               int ldd) {
+    __shared__ float tile[32][33];
+    const int out_rows = TRANSPOSE ? cols : rows, out_cols = ldd;
+    const int tr = (out_rows + 31) / 32, tc = (out_cols + 31) / 32;
+    for (int blk = blockIdx.x; blk < tr * tc; blk += gridDim.x) {
+        const int r0 = (blk / tc) * 32, c0 = (blk % tc) * 32;   // output tile
+        for (int e = threadIdx.x; e < 1024; e += 256) {
+            const int i = e >> 5, j = e & 31;
+            // source element for output (r0 + i, c0 + j); read along the source's rows
+            const int sr = TRANSPOSE ? c0 + i : r0 + i, sc = TRANSPOSE ? r0 + j : c0 + j;
+            tile[i][j] = (sr < rows && sc < cols) ? src[(size_t)sr * ld + sc] : 0.0f;
+        }
+        __syncthreads();
+        for (int e = threadIdx.x; e < 1024; e += 256) {
+            const int i = e >> 5, j = e & 31;
+            const int r = r0 + i, c = c0 + j;
+            if (r < out_rows && c < out_cols) {
+                const float x = TRANSPOSE ? tile[j][i] : tile[i][j];
+                const __bf16 h = (__bf16)x;
+                hi[(size_t)r * ldd + c] = h;
+                lo[(size_t)r * ldd + c] = (__bf16)(x - (float)h);
+            }
+        }
+        __syncthreads();
     }
+}
+
+// C = A . B^T with A (M x K) and B (N x K) given as K-contiguous bf16 halves of
+// pitch lda / ldb (multiples of 32, zero past K), FP32 accumulation, optional
+// elementwise scale. BM x BN tiles, K slices of 32 (one v_mfma_f32_16x16x32_bf16
+// per 16 x 16 x 32 block and half pairing), the next slice prefetched into
+// registers while the current one is multiplied.
+struct NtArgs {
+    const __bf16 *ah = nullptr, *al = nullptr, *bh = nullptr, *bl = nullptr;
+    int lda = 0, ldb = 0;
+    float *C = nullptr;
+    const float *scale = nullptr;
+    int ldc = 0, M = 0, N = 0, K = 0;
+};
+
+template <int BM, int BN>
+__global__ __launch_bounds__(256) void k_gemm_nt_bf16x3(NtArgs g) {
+    constexpr int GK = 32, P = GK + 8;   // LDS row pitch in bf16 (80 bytes: 16-byte reads stay aligned)
+    constexpr int WM = BM / 2, WN = BN / 2, TM = WM / 16, TN = WN / 16;
+    constexpr int CA = BM * (GK / 8) / 256, CB = BN * (GK / 8) / 256;   // 16-byte chunks per thread per half
+    static_assert(CA >= 1 && CB >= 1, "tile too small for 256 threads");
+    __shared__ __align__(16) __bf16 lds[2 * (BM + BN) * P];
+    __bf16 *Ah = lds, *Al = Ah + BM * P, *Bh = Al + BM * P, *Bl = Bh + BN * P;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = (wave >> 1) * WM, wn = (wave & 1) * WN;
+    const int l16 = lane & 15, kq = lane >> 4;
+    const int tn = (g.N + BN - 1) / BN, tiles = ((g.M + BM - 1) / BM) * tn;
+    for (int blk = blockIdx.x; blk < tiles; blk += gridDim.x) {
+        const int m0 = (blk / tn) * BM, n0 = (blk % tn) * BN;
+        floatx4 acc[TM][TN];
+#pragma unroll
+        for (int x = 0; x < TM; x++)
+#pragma unroll
+            for (int y = 0; y < TN; y++) acc[x][y] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+        uint4 rah[CA], ral[CA], rbh[CB], rbl[CB];
+        auto load = [&](int k0) {
+#pragma unroll
+            for (int v = 0; v < CA; v++) {
+                const int c = tid + 256 * v, row = c / (GK / 8), part = c % (GK / 8);
+                const bool ok = m0 + row < g.M;
+                const size_t off = (size_t)(m0 + row) * g.lda + k0 + part * 8;
+                rah[v] = ok ? *reinterpret_cast<const uint4 *>(g.ah + off) : uint4{0u, 0u, 0u, 0u};
+                ral[v] = ok ? *reinterpret_cast<const uint4 *>(g.al + off) : uint4{0u, 0u, 0u, 0u};
+            }
+#pragma unroll
+            for (int v = 0; v < CB; v++) {
+                const int c = tid + 256 * v, row = c / (GK / 8), part = c % (GK / 8);
+                const bool ok = n0 + row < g.N;
+                const size_t off = (size_t)(n0 + row) * g.ldb + k0 + part * 8;
+                rbh[v] = ok ? *reinterpret_cast<const uint4 *>(g.bh + off) : uint4{0u, 0u, 0u, 0u};
+                rbl[v] = ok ? *reinterpret_cast<const uint4 *>(g.bl + off) : uint4{0u, 0u, 0u, 0u};
+            }
+        };
+        auto store = [&]() {
+#pragma unroll
+            for (int v = 0; v < CA; v++) {
+                const int c = tid + 256 * v, row = c / (GK / 8), part = c % (GK / 8);
+                *reinterpret_cast<uint4 *>(Ah + row * P + part * 8) = rah[v];
+                *reinterpret_cast<uint4 *>(Al + row * P + part * 8) = ral[v];
+            }
+#pragma unroll
+            for (int v = 0; v < CB; v++) {
+                const int c = tid + 256 * v, row = c / (GK / 8), part = c % (GK / 8);
+                *reinterpret_cast<uint4 *>(Bh + row * P + part * 8) = rbh[v];
+                *reinterpret_cast<uint4 *>(Bl + row * P + part * 8) = rbl[v];
+            }
+        };
+        load(0);
+        for (int k0 = 0; k0 < g.K; k0 += GK) {
+            __syncthreads();
+            store();
+            __syncthreads();
+            if (k0 + GK < g.K) load(k0 + GK);
+            bf16x8 ah[TM], al[TM], bh[TN], bl[TN];
+#pragma unroll
+            for (int x = 0; x < TM; x++) {
+                const int r = (wm + x * 16 + l16) * P + kq * 8;
+                ah[x] = *reinterpret_cast<const bf16x8 *>(Ah + r);
+                al[x] = *reinterpret_cast<const bf16x8 *>(Al + r);
+            }
+#pragma unroll
+            for (int y = 0; y < TN; y++) {
+                const int r = (wn + y * 16 + l16) * P + kq * 8;
+                bh[y] = *reinterpret_cast<const bf16x8 *>(Bh + r);
+                bl[y] = *reinterpret_cast<const bf16x8 *>(Bl + r);
+            }
+#pragma unroll
+            for (int x = 0; x < TM; x++)
+#pragma unroll
+                for (int y = 0; y < TN; y++) {
+                    acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[x], bh[y], acc[x][y], 0, 0, 0);
+                    acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[x], bl[y], acc[x][y], 0, 0, 0);
+                    acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[x], bh[y], acc[x][y], 0, 0, 0);
+                }
+        }
+#pragma unroll
+        for (int x = 0; x < TM; x++)
+#pragma unroll
+            for (int y = 0; y < TN; y++)
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    const int row = m0 + wm + x * 16 + kq * 4 + i, col = n0 + wn + y * 16 + l16;
+                    if (row < g.M && col < g.N) {
+                        float v = acc[x][y][i];
+                        if (g.scale) v *= g.scale[(size_t)row * g.ldc + col];
+                        g.C[(size_t)row * g.ldc + col] = v;
+                    }
+                }
+        __syncthreads();
+    }
+}
+
+hipError_t launch_split(const float *src, int rows, int cols, int ld, bool transpose, __bf16 *hi, __bf16 *lo, int ldd,
+                        hipStream_t stream) {
+    const int out_rows = transpose ? cols : rows;
+    const long long tiles = (long long)((out_rows + 31) / 32) * ((ldd + 31) / 32);
+    const int grid = (int)(tiles < 8192 ? tiles : 8192);
+    if (transpose) k_split<true><<<grid, 256, 0, stream>>>(src, rows, cols, ld, hi, lo, ldd);
+    else k_split<false><<<grid, 256, 0, stream>>>(src, rows, cols, ld, hi, lo, ldd);
+    return hipGetLastError();
+}
+
+hipError_t launch_gemm_nt_bf16x3(const __bf16 *ah, const __bf16 *al, int lda, const __bf16 *bh, const __bf16 *bl,
+                                 int ldb, float *C, const float *scale, int ldc, int M, int N, int K, int tile,
+                                 hipStream_t stream) {
+    NtArgs g;
+    g.ah = ah; g.al = al; g.bh = bh; g.bl = bl; g.lda = lda; g.ldb = ldb;
+    g.C = C; g.scale = scale; g.ldc = ldc; g.M = M; g.N = N; g.K = K;
+    const int t = tile == 128 ? 128 : 64;
+    const long long tiles = (long long)((M + t - 1) / t) * ((N + t - 1) / t);
+    const int grid = (int)(tiles < 8192 ? tiles : 8192);
+    if (grid == 0) return hipSuccess;
+    if (t == 128) k_gemm_nt_bf16x3<128, 128><<<grid, 256, 0, stream>>>(g);
+    else k_gemm_nt_bf16x3<64, 64><<<grid, 256, 0, stream>>>(g);
     return hipGetLastError();
 }
 

@@ -17,10 +17,18 @@ struct GemmArgs {
     int M = 0, N = 0, K = 0, lda = 0, ldb = 0, ldc = 0;
 };
 
-enum { GEMM_FP32 = 0, GEMM_BF16X3 = 1 };
+// One FP32 product with 64 x 64 or 128 x 128 output tiles.
+hipError_t launch_gemm(const GemmArgs &g, int tile, hipStream_t stream);
 
-// One product with 64 x 64 or 128 x 128 output tiles.
-hipError_t launch_gemm(const GemmArgs &g, int precision, int tile, hipStream_t stream);
+// BF16x3: split an FP32 rows x cols matrix (pitch ld) into bf16 halves whose
+// rows have pitch ldd (a multiple of 32, zero-padded), transposed or not ...
+hipError_t launch_split(const float *src, int rows, int cols, int ld, bool transpose, __bf16 *hi, __bf16 *lo, int ldd,
+                        hipStream_t stream);
+// ... and C = A . B^T from K-contiguous halves (A: M x K, B: N x K, K a
+// multiple of 32), FP32 accumulation, optional elementwise scale.
+hipError_t launch_gemm_nt_bf16x3(const __bf16 *ah, const __bf16 *al, int lda, const __bf16 *bh, const __bf16 *bl,
+                                 int ldb, float *C, const float *scale, int ldc, int M, int N, int K, int tile,
+                                 hipStream_t stream);
 
 // The four products of the tick's solve for every scene of a batch.
 hipError_t launch_dst_gemms(const Batch &b, hipStream_t stream);

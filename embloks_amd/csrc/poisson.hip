@@ -10,6 +10,7 @@
 // scenes; this entry point runs them on a caller's right-hand side, in FP32 or
 // BF16x3 MFMA, and times them with HIP events on its own stream.
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -24,12 +25,15 @@ struct EmbPoisson {
     float *rhs = nullptr, *w1 = nullptr, *w2 = nullptr, *phi = nullptr, *inv_lap = nullptr;
     const float *sx = nullptr, *sy = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
+    // BF16x3: halves of Sx, Sy and of each product's data operand, rows padded to 32
+    int kx = 0, ky = 0;
+    __bf16 *sxh = nullptr, *sxl = nullptr, *syh = nullptr, *syl = nullptr, *dh = nullptr, *dl = nullptr;
 };
 
 namespace emb {
 namespace {
 
-hipError_t enqueue_solve(const EmbPoisson &p, int precision, int tile) {
+hipError_t enqueue_solve_fp32(const EmbPoisson &p, int tile) {
     const int nx = p.nx, ny = p.ny;
     GemmArgs g;
     g.M = nx;
@@ -37,17 +41,44 @@ hipError_t enqueue_solve(const EmbPoisson &p, int precision, int tile) {
     g.ldc = ny;
     // stage 0: w1 = Sx . rhs
     g.A = p.sx; g.lda = nx; g.B = p.rhs; g.ldb = ny; g.K = nx; g.C = p.w1; g.scale = nullptr;
-    hipError_t e = launch_gemm(g, precision, tile, p.stream);
+    hipError_t e = launch_gemm(g, tile, p.stream);
     if (e != hipSuccess) return e;
     // stage 1: w2 = (w1 . Sy) * inv_lap
     g.A = p.w1; g.lda = ny; g.B = p.sy; g.ldb = ny; g.K = ny; g.C = p.w2; g.scale = p.inv_lap;
-    if ((e = launch_gemm(g, precision, tile, p.stream)) != hipSuccess) return e;
+    if ((e = launch_gemm(g, tile, p.stream)) != hipSuccess) return e;
     // stage 2: w1 = Sx . w2
     g.A = p.sx; g.lda = nx; g.B = p.w2; g.ldb = ny; g.K = nx; g.C = p.w1; g.scale = nullptr;
-    if ((e = launch_gemm(g, precision, tile, p.stream)) != hipSuccess) return e;
+    if ((e = launch_gemm(g, tile, p.stream)) != hipSuccess) return e;
     // stage 3: phi = w1 . Sy
     g.A = p.w1; g.lda = ny; g.B = p.sy; g.ldb = ny; g.K = ny; g.C = p.phi; g.scale = nullptr;
-    return launch_gemm(g, precision, tile, p.stream);
+    return launch_gemm(g, tile, p.stream);
+}
+
+// The same four products as C = A . B^T on split operands: Sx is A as it is
+// (rows K-contiguous), Sy is B^T as it is (symmetric); the data operand is
+// split before each product, transposed where the contraction runs over its
+// rows (stages 0 and 2).
+hipError_t enqueue_solve_bf16x3(const EmbPoisson &p, int tile) {
+    const int nx = p.nx, ny = p.ny;
+    hipError_t e;
+    // stage 0: w1 = Sx . rhs = Sx . (rhs^T)^T
+    if ((e = launch_split(p.rhs, nx, ny, ny, true, p.dh, p.dl, p.kx, p.stream)) != hipSuccess) return e;
+    if ((e = launch_gemm_nt_bf16x3(p.sxh, p.sxl, p.kx, p.dh, p.dl, p.kx, p.w1, nullptr, ny, nx, ny, p.kx, tile,
+                                   p.stream)) != hipSuccess)
+        return e;
+    // stage 1: w2 = (w1 . Sy) * inv_lap
+    if ((e = launch_split(p.w1, nx, ny, ny, false, p.dh, p.dl, p.ky, p.stream)) != hipSuccess) return e;
+    if ((e = launch_gemm_nt_bf16x3(p.dh, p.dl, p.ky, p.syh, p.syl, p.ky, p.w2, p.inv_lap, ny, nx, ny, p.ky, tile,
+                                   p.stream)) != hipSuccess)
+        return e;
+    // stage 2: w1 = Sx . w2
+    if ((e = launch_split(p.w2, nx, ny, ny, true, p.dh, p.dl, p.kx, p.stream)) != hipSuccess) return e;
+    if ((e = launch_gemm_nt_bf16x3(p.sxh, p.sxl, p.kx, p.dh, p.dl, p.kx, p.w1, nullptr, ny, nx, ny, p.kx, tile,
+                                   p.stream)) != hipSuccess)
+        return e;
+    // stage 3: phi = w1 . Sy
+    if ((e = launch_split(p.w1, nx, ny, ny, false, p.dh, p.dl, p.ky, p.stream)) != hipSuccess) return e;
+    return launch_gemm_nt_bf16x3(p.dh, p.dl, p.ky, p.syh, p.syl, p.ky, p.phi, nullptr, ny, nx, ny, p.ky, tile, p.stream);
 }
 
 }  // namespace
@@ -72,6 +103,7 @@ int emb_poisson_create(EmbDevice *dev, int32_t nx, int32_t ny, EmbPoisson **out)
             (void)hipFree(p->w2);
             (void)hipFree(p->phi);
             (void)hipFree(p->inv_lap);
+            for (__bf16 *a : {p->sxh, p->sxl, p->syh, p->syl, p->dh, p->dl}) (void)hipFree(a);
             if (p->e0) (void)hipEventDestroy(p->e0);
             if (p->e1) (void)hipEventDestroy(p->e1);
             if (p->stream) (void)hipStreamDestroy(p->stream);
@@ -87,6 +119,17 @@ int emb_poisson_create(EmbDevice *dev, int32_t nx, int32_t ny, EmbPoisson **out)
         int rc = emb::device_sine(dev, nx, &p->sx);
         if (rc == EMB_OK) rc = emb::device_sine(dev, ny, &p->sy);
         if (rc != EMB_OK) return bail(rc);
+        p->kx = (nx + 31) / 32 * 32;
+        p->ky = (ny + 31) / 32 * 32;
+        const size_t halves[6] = {(size_t)nx * p->kx, (size_t)nx * p->kx, (size_t)ny * p->ky, (size_t)ny * p->ky,
+                                  std::max((size_t)ny * p->kx, (size_t)nx * p->ky), std::max((size_t)ny * p->kx, (size_t)nx * p->ky)};
+        __bf16 **hp[6] = {&p->sxh, &p->sxl, &p->syh, &p->syl, &p->dh, &p->dl};
+        for (int q = 0; q < 6; q++)
+            if (hipMalloc(hp[q], halves[q] * sizeof(__bf16)) != hipSuccess) return bail(fail(EMB_E_NOMEM, "device memory"));
+        if (emb::launch_split(p->sx, nx, nx, nx, false, p->sxh, p->sxl, p->kx, p->stream) != hipSuccess ||
+            emb::launch_split(p->sy, ny, ny, ny, false, p->syh, p->syl, p->ky, p->stream) != hipSuccess ||
+            hipStreamSynchronize(p->stream) != hipSuccess)
+            return bail(fail(EMB_E_DEVICE, "splitting the sine matrices failed"));
         // Inverse Laplacian with the transforms' normalisation (maxwell.rs:60-67), FP64 -> FP32.
         std::vector<double> kx(nx), ky(ny);
         const double pi = 3.14159265358979323846;
@@ -124,10 +167,10 @@ int emb_poisson_solve(EmbPoisson *p, int32_t precision, int32_t tile, int32_t re
         if (tile != 0 && tile != 64 && tile != 128) return fail(EMB_E_INVALID, "tile must be 0, 64 or 128");
         if (repeats < 1) return fail(EMB_E_INVALID, "repeats must be >= 1");
         if (tile == 0) tile = (long long)p->nx * p->ny >= (1LL << 20) ? 128 : 64;
-        const int prec = precision == EMB_DST_BF16X3 ? emb::GEMM_BF16X3 : emb::GEMM_FP32;
         EMB_HIP(hipSetDevice(p->dev->ordinal));
         EMB_HIP(hipEventRecord(p->e0, p->stream));
-        for (int r = 0; r < repeats; r++) EMB_HIP(emb::enqueue_solve(*p, prec, tile));
+        for (int r = 0; r < repeats; r++)
+            EMB_HIP(precision == EMB_DST_BF16X3 ? emb::enqueue_solve_bf16x3(*p, tile) : emb::enqueue_solve_fp32(*p, tile));
         EMB_HIP(hipEventRecord(p->e1, p->stream));
         EMB_HIP(hipEventSynchronize(p->e1));
         if (avg_ms) {
@@ -158,6 +201,7 @@ int emb_poisson_destroy(EmbPoisson *p) {
         (void)hipSetDevice(p->dev->ordinal);
         if (p->stream) (void)hipStreamSynchronize(p->stream);
         for (float *a : {p->rhs, p->w1, p->w2, p->phi, p->inv_lap}) (void)hipFree(a);
+        for (__bf16 *a : {p->sxh, p->sxl, p->syh, p->syl, p->dh, p->dl}) (void)hipFree(a);
         if (p->e0) (void)hipEventDestroy(p->e0);
         if (p->e1) (void)hipEventDestroy(p->e1);
         if (p->stream) (void)hipStreamDestroy(p->stream);
