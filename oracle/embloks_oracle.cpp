@@ -378,6 +378,7 @@ struct Maxwell {
     // maxwell.rs:336-350
     void gather(int64_t n, const double *pos, double *out) const {
         const double m = (double)margin;
+#pragma omp parallel for num_threads(g_threads) schedule(static) if (g_threads > 1)
         for (int64_t p = 0; p < n; p++) {
             const double x = pos[2 * p], y = pos[2 * p + 1];
             out[3 * p + 0] = bilinear(ex, x + m, y - 0.5 + m);
@@ -990,6 +991,13 @@ struct CloseRange {
         CellList list(w);
         const size_t n = w.n();
         std::vector<double> force(2 * n, 0.0);
+        if (g_threads > 1) {
+            // Timing only (the CPU baseline on all cores): each electron's force
+            // from its own side, equal to the loop below up to FP64 rounding.
+#pragma omp parallel for schedule(dynamic, 256) num_threads(g_threads)
+            for (long long i = 0; i < (long long)n; i++) force_on(w, list, (size_t)i, &force[2 * (size_t)i]);
+            return force;
+        }
         for (size_t i = 0; i < n; i++) {
             const double x = w.pos[2 * i], y = w.pos[2 * i + 1];
             const int ci = (int)(list.cell[i] / rows), ri = (int)(list.cell[i] % rows);
@@ -1265,6 +1273,8 @@ struct OrcSim {
             maxwell->gather((int64_t)w.n(), w.pos.data(), fields.data());
             std::vector<double> closef;
             if (close.enabled()) closef = close.force(w);
+            // Electrons are independent here: threads change the timing, not the result.
+#pragma omp parallel for num_threads(g_threads) schedule(static) if (g_threads > 1)
             for (size_t i = 0; i < w.n(); i++) {
                 int c, r;
                 grid_cell(&w.pos[2 * i], w.cols, w.rows, c, r);
