@@ -98,10 +98,11 @@ def run_single(args):
     return world, elapsed, phases, info
 
 
-# Kernel that carries each tick phase (the rocprofv3 name without the namespace).
-PHASE_KERNEL = {"trim": "k_trim", "push": "k_push_tiled", "rest": "k_rest", "count": "k_count", "scan": "k_scan",
-                "scatter": "k_scatter", "canon": "k_canon", "deposit_smooth": "k_deposit_smooth", "fdtd": "k_fdtd",
-                "correct": "k_correct"}
+# Kernels that carry each tick phase (the rocprofv3 names without the namespace).
+PHASE_KERNELS = {"trim": ["k_trim"], "push": ["k_push_tiled<false>", "k_push_sparse<false>"], "rest": ["k_rest"],
+                 "count": ["k_count"], "scan": ["k_scan", "k_scan_top"], "scatter": ["k_scatter"], "canon": ["k_canon"],
+                 "deposit_smooth": ["k_deposit_smooth"], "fdtd": ["k_fdtd"], "correct": ["k_correct"],
+                 "dst_gemm": [f"k_dst_gemm<{i}>" for i in range(4)]}
 
 
 def measured_traffic(phase: str):
@@ -116,10 +117,8 @@ def measured_traffic(phase: str):
             continue
         name, rd, wr = line.split()
         table[name] = int(rd) + int(wr)
-    if phase == "dst_gemm":
-        parts = [table.get(f"k_dst_gemm<{i}>") for i in range(4)]
-        return sum(parts) if all(p is not None for p in parts) else None
-    return table.get(PHASE_KERNEL.get(phase, ""))
+    parts = [table.get(k) for k in PHASE_KERNELS.get(phase, [])]
+    return sum(parts) if parts and all(p is not None for p in parts) else None
 
 
 def roofline(phases: dict, info: dict):
