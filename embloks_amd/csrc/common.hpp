@@ -49,6 +49,7 @@ constexpr int SCAN_TILE = 1024;   // cells per block of the cell-list scan
 constexpr int TICK_TABLE = 64;    // ticks of host-computed AC phases / EMF targets
 constexpr int PUSH_TILE = 8;      // cells per side of a push tile
 constexpr int CELL_MODE_MIN = 6;  // a push tile with a cell this crowded runs the cell mode
+constexpr int HEAVY_CELL = 64;    // a cell this crowded is pushed by k_push_heavy, spread over blocks
 constexpr int FDTD_TI = 16, FDTD_TJ = 32;   // corners per FDTD tile (x, y)
 constexpr int ION_SUM = 33;       // samples per axis of the summed ion table
 constexpr int PAIR_SAMPLES = 4096;   // samples of the radial pair table (close_range.rs:30)
@@ -86,7 +87,7 @@ struct StackInfo {
 struct SceneCounters {
     uint32_t tick;            // ticks completed (parity selects ping-pong buffers)
     uint32_t vmax_bits[2];    // max |v| (float bits) of the electrons, by tick parity
-    uint32_t reserved_ctr;
+    int32_t heavy_n;          // cells listed in heavy_cells by the last sort
     int32_t substeps;         // electron substeps of the current tick
     int32_t rest_error;       // set when the serial substep path rejected a scene
     uint32_t pad[2];
@@ -145,6 +146,7 @@ struct DevScene {
     int32_t *toff;            // per scan tile: offset of the tile
     int32_t *cstart;          // per cell + 1: absolute start in the sorted order
     uint8_t *tile_dense;      // per push tile: a cell holds >= CELL_MODE_MIN electrons
+    int32_t *heavy_cells;     // cells holding >= HEAVY_CELL electrons (ctr->heavy_n of them)
     // --- deposit
     long long *cur;           // [2 * cells]: cell-binned sum of v, units 2^-28 (x q_e)
     const long long *static_rho;   // ions + fixed charges per cell, units 2^-44 electron charges
@@ -156,6 +158,7 @@ struct DevScene {
     const float *g_table;     // PAIR_SAMPLES + 4: the reference's PAIR_SAMPLES + 2 samples, then two zeros
     const lerp2 *g_lerp;     // PAIR_SAMPLES + 1: (g[i], g[i+1] - g[i])
     const float *square;      // square_n^2 * 3
+    const lerp2 *square_f2;   // square_n^2: the force channels (1, 2) of the square table as pairs
     const float *ion_sum;     // ION_SUM^2 * 2: the square table's force channels summed over
                               // the (2 reach + 1)^2 cell offsets (uniform-density neighbourhoods)
     int32_t ion_a0;           // square-table index of the home cell's low corner offset
@@ -183,6 +186,7 @@ enum KernelClass {
     KC_STACKS,          // one block per battery stack (at least one per scene)
     KC_SCENE,           // one block per scene
     KC_PUSH,            // PUSH_TILE x PUSH_TILE cell tiles
+    KC_HEAVY,           // blocks sharing the scene's heavy cells (one per 16k electrons, at most 256)
     KC_COUNT
 };
 

@@ -125,6 +125,7 @@ __global__ __launch_bounds__(256) void k_scan_top(Batch b) {
         }
         const int ptiles = ((S.cols + PUSH_TILE - 1) / PUSH_TILE) * ((S.rows + PUSH_TILE - 1) / PUSH_TILE);
         for (int q = threadIdx.x; q < ptiles; q += 256) S.tile_dense[q] = 0;
+        if (threadIdx.x == 0) S.ctr->heavy_n = 0;
         __syncthreads();
     }
 }
@@ -188,6 +189,9 @@ __global__ __launch_bounds__(256) void k_canon(Batch b) {
                 const int cc = c / S.rows, rr = c - cc * S.rows;
                 S.tile_dense[(cc / PUSH_TILE) * ((S.rows + PUSH_TILE - 1) / PUSH_TILE) + rr / PUSH_TILE] = 1;
             }
+            // Crowded cells get their own blocks (k_push_heavy); the list's order
+            // does not matter, each cell's result depends on the cell alone.
+            if (s1 - s0 >= HEAVY_CELL) S.heavy_cells[atomicAdd(&S.ctr->heavy_n, 1)] = c;
         }
         if (S1 - S0 > CANON_CAP) {
             if (mine) canon_cell(S, c, s0, s1);
@@ -459,15 +463,33 @@ __device__ __forceinline__ void field_accel(const DevScene &S, int hc, int hr, f
     ay = S.qm * (ey - vx * bz);
 }
 
-// Close-range force of the ions of one conductive cell at offset (ox, oy) =
-// home - cell on an electron at fractions (px, py) (close_range.rs:335-350).
-__device__ __forceinline__ void ion_cell(const DevScene &S, float ox, float oy, float px, float py, float dens,
-                                         float &fx, float &fy) {
-    const float u = (ox + (px - 0.5f) + S.half) * S.square_samples;
-    const float w = (oy + (py - 0.5f) + S.half) * S.square_samples;
+// Close-range force of the ions of the conductive cells around an electron at
+// fractions (px, py) of its home cell (close_range.rs:335-350). Every cell of
+// the window sees the electron at the same fraction of a table sample (cell
+// offsets are whole multiples of square_samples), so the bilinear weights are
+// computed once; a cell at offset (ox, oy) = home - cell then reads its four
+// force samples (the window never reaches the table's edge, where the
+// reference clamps).
+struct IonWeights {
+    int a0, b0, step;
+    float w00, w10, w01, w11;
+};
+
+__device__ __forceinline__ IonWeights ion_weights(const DevScene &S, float px, float py) {
+    const float u = ((px - 0.5f) + S.half) * S.square_samples, v = ((py - 0.5f) + S.half) * S.square_samples;
+    const int a = (int)u, b = (int)v;
+    const float fu = u - (float)a, fv = v - (float)b;
+    return IonWeights{a, b, (int)S.square_samples, (1.0f - fu) * (1.0f - fv), fu * (1.0f - fv), (1.0f - fu) * fv, fu * fv};
+}
+
+__device__ __forceinline__ void ion_cell(const DevScene &S, const IonWeights &w, int ox, int oy, float dens, float &fx,
+                                         float &fy) {
+    const int n = S.square_n;
+    const lerp2 *t = S.square_f2 + (size_t)(w.a0 + ox * w.step) * n + (w.b0 + oy * w.step);
+    const lerp2 t00 = t[0], t10 = t[n], t01 = t[1], t11 = t[n + 1];
     const float coef = S.k_ion_q * dens;
-    fx += coef * square_trunc(S, 1, u, w);
-    fy += coef * square_trunc(S, 2, u, w);
+    fx += coef * (t00.x * w.w00 + t10.x * w.w10 + t01.x * w.w01 + t11.x * w.w11);
+    fy += coef * (t00.y * w.w00 + t10.y * w.w10 + t01.y * w.w01 + t11.y * w.w11);
 }
 
 // The same summed over a whole window of equal-density cells: every neighbour
@@ -546,6 +568,7 @@ __device__ void push_core(const DevScene &S, const NB &nb, const float *gt, int 
         const int r_lo = max(hr - R, 0), r_hi = min(hr + R, S.rows - 1);
         const bool uniform = (info.flags & F_ION_UNIFORM) != 0u;
         if (uniform) ion_uniform(S, home, px, py, fx, fy);
+        const IonWeights iw = ion_weights(S, px, py);
         // Neighbour cells entirely beyond the cutoff from this electron are skipped.
         float sx = 0.0f, sy = 0.0f;
         const float cut2 = S.cutoff2;
@@ -556,7 +579,7 @@ __device__ void push_core(const DevScene &S, const NB &nb, const float *gt, int 
                 const float oy = (float)(hr - rr);
                 if (!uniform) {
                     const float dens = S.density[cc * S.rows + rr];
-                    if (dens > 0.0f) ion_cell(S, ox, oy, px, py, dens, fx, fy);
+                    if (dens > 0.0f) ion_cell(S, iw, hc - cc, hr - rr, dens, fx, fy);
                 }
                 const float gapy = oy > 0.0f ? oy + py - 1.0f : (oy < 0.0f ? -oy - py : 0.0f);
                 if (gapx * gapx + gapy * gapy >= cut2 * 1.0001f) continue;
@@ -632,15 +655,14 @@ struct GlobalRuns {
 // the neighbour runs; the partial sums meet in a fixed butterfly. FORCE: write
 // the close-range force to force_out[id] instead of pushing.
 template <bool FORCE, class RUNS>
-__device__ __forceinline__ void cell_item(const DevScene &S, const RUNS &runs, const float *gt, int hc, int hr, int chunk,
-                                          int wc0, int wr0, int k, float h, uint32_t t, float2 *force_out) {
+__device__ __forceinline__ void cell_item(const DevScene &S, const RUNS &runs, const float *gt, const float *dens_w,
+                                          int hc, int hr, int e0, int n, int chunk, int wc0, int wr0, int W, int k,
+                                          float h, uint32_t t, float2 *force_out, int imax = 64) {
     const int lane = threadIdx.x & 63;
     const int R = S.reach;
     const float s2 = S.pair_s2;
     const int home = hc * S.rows + hr;
-    const int e0 = __builtin_amdgcn_readfirstlane(S.cstart[home]);
-    const int n = __builtin_amdgcn_readfirstlane(S.cstart[home + 1]) - e0;
-    const int I = lanes_per(n), shift = __builtin_ctz(I), SL = 64 >> shift;
+    const int I = min(lanes_per(n), imax), shift = __builtin_ctz(I), SL = 64 >> shift;
     const int i = lane & (I - 1), sl = lane >> shift;
     const int ei = chunk * I + i;
     const bool live = ei < n;
@@ -674,11 +696,12 @@ __device__ __forceinline__ void cell_item(const DevScene &S, const RUNS &runs, c
     const CellInfo info = S.info[home];
     const bool uniform = (info.flags & F_ION_UNIFORM) != 0u;
     if (!uniform) {
+        const IonWeights iw = ion_weights(S, px, py);
         const int nr = r_hi - r_lo + 1, cells = (c_hi - c_lo + 1) * nr;
         for (int w = sl; w < cells; w += SL) {
             const int cc = c_lo + w / nr, rr = r_lo + w % nr;
-            const float dens = S.density[cc * S.rows + rr];
-            if (dens > 0.0f) ion_cell(S, (float)(hc - cc), (float)(hr - rr), px, py, dens, fx, fy);
+            const float dens = dens_w[(cc - wc0) * W + (rr - wr0)];
+            if (dens > 0.0f) ion_cell(S, iw, hc - cc, hr - rr, dens, fx, fy);
         }
     }
     for (int off = I; off < 64; off <<= 1) {
@@ -717,6 +740,7 @@ __device__ void force_core(const DevScene &S, const NB &nb, const float *gt, int
     const int r_lo = max(hr - R, 0), r_hi = min(hr + R, S.rows - 1);
     const bool uniform = (info.flags & F_ION_UNIFORM) != 0u;
     if (uniform) ion_uniform(S, home, px, py, fx, fy);
+    const IonWeights iw = ion_weights(S, px, py);
     for (int cc = c_lo; cc <= c_hi; cc++) {
         const float ox = (float)(hc - cc);
         const float gapx = ox > 0.0f ? ox + px - 1.0f : (ox < 0.0f ? -ox - px : 0.0f);
@@ -724,7 +748,7 @@ __device__ void force_core(const DevScene &S, const NB &nb, const float *gt, int
             const float oy = (float)(hr - rr);
             if (!uniform) {
                 const float dens = S.density[cc * S.rows + rr];
-                if (dens > 0.0f) ion_cell(S, ox, oy, px, py, dens, fx, fy);
+                if (dens > 0.0f) ion_cell(S, iw, hc - cc, hr - rr, dens, fx, fy);
             }
             const float gapy = oy > 0.0f ? oy + py - 1.0f : (oy < 0.0f ? -oy - py : 0.0f);
             if (gapx * gapx + gapy * gapy >= S.cutoff2 * 1.0001f) continue;
@@ -752,7 +776,7 @@ __device__ void force_core(const DevScene &S, const NB &nb, const float *gt, int
 // butterfly. The other tiles' electrons go through k_push_sparse. Every sum
 // runs in a fixed order, so a scene gives the same bits alone or batched.
 constexpr int PUSH_WIN = PUSH_TILE + 2 * 4;   // window side for reach <= 4
-constexpr int PUSH_STAGE = 3328;
+constexpr int PUSH_STAGE = 7168;   // positions (56 KB): the block stays under 80 KB of LDS, two per CU
 constexpr int PUSH_CELLS = PUSH_TILE * PUSH_TILE;
 constexpr int PUSH_THREADS = 512;             // 8 waves: two blocks fill a CU's 16 wave slots at <= 128 VGPRs
 constexpr int PUSH_WAVES = PUSH_THREADS / 64;
@@ -762,6 +786,8 @@ __global__ __launch_bounds__(PUSH_THREADS) void k_push_tiled(Batch b, int k, flo
     __shared__ int2 lp[PUSH_STAGE];
     __shared__ float gt[PAIR_SAMPLES + 4];
     __shared__ int lb[PUSH_WIN * PUSH_WIN], le[PUSH_WIN * PUSH_WIN];
+    __shared__ float dens_w[PUSH_WIN * PUSH_WIN];               // ion density of the window's cells
+    __shared__ int cell_e0[PUSH_CELLS], cell_n[PUSH_CELLS];     // tile cells: first sorted slot, electrons
     __shared__ int col_base[PUSH_WIN + 1], col_g0[PUSH_WIN];   // LDS base and first global slot per column
     __shared__ int item_pref[PUSH_CELLS + 1];                   // cell mode: prefix of work items per tile cell
     bool table_loaded = false;
@@ -808,62 +834,82 @@ __global__ __launch_bounds__(PUSH_THREADS) void k_push_tiled(Batch b, int k, flo
             }
             if (lane == W - 1) col_base[W] = incl;
             const int hc = tc0 + lane / PUSH_TILE, hr = tr0 + lane % PUSH_TILE;
-            int m = 0;
+            int m = 0, first = 0;
             if (hc < tc1 && hr < tr1) {
                 const int cell = hc * S.rows + hr;
-                m = S.cstart[cell + 1] - S.cstart[cell];
+                first = S.cstart[cell];
+                m = S.cstart[cell + 1] - first;
             }
+            cell_e0[lane] = first;
+            cell_n[lane] = m;
             const int I = lanes_per(m);
-            const int items = m > 0 ? (m + I - 1) / I : 0;
+            const int items = (m > 0 && m < HEAVY_CELL) ? (m + I - 1) / I : 0;
             const int item_incl = wave_scan_incl(items);
             item_pref[lane] = item_incl - items;
             if (lane == 63) item_pref[PUSH_CELLS] = item_incl;
         }
         __syncthreads();
-        const bool staged = col_base[W] <= PUSH_STAGE;
-        if (staged) {
-            for (int q = threadIdx.x; q < W * W; q += PUSH_THREADS) {
-                const int ci = q / W, ri = q - (q / W) * W;
-                const int cc = wc0 + ci, rr = wr0 + ri;
-                if (cc < 0 || cc >= S.cols || rr < 0 || rr >= S.rows) {
-                    lb[q] = 0;
-                    le[q] = 0;
-                } else {
-                    const int cell = cc * S.rows + rr;
-                    lb[q] = col_base[ci] + (S.cstart[cell] - col_g0[ci]);
-                    le[q] = col_base[ci] + (S.cstart[cell + 1] - col_g0[ci]);
+        for (int q = threadIdx.x; q < W * W; q += PUSH_THREADS) {
+            const int ci = q / W, ri = q - (q / W) * W;
+            const int cc = wc0 + ci, rr = wr0 + ri;
+            const bool in = cc >= 0 && cc < S.cols && rr >= 0 && rr < S.rows;
+            dens_w[q] = in ? S.density[cc * S.rows + rr] : 0.0f;
+        }
+        // The whole window in one pass when it fits the staging; else one pass
+        // per column of home cells, whose window is only 2 reach + 1 columns wide
+        // (reading from global memory only if even that does not fit).
+        const int passes = col_base[W] <= PUSH_STAGE ? 1 : PUSH_TILE;
+        for (int pass = 0; pass < passes; pass++) {
+            const int ca = passes == 1 ? 0 : pass, cb = passes == 1 ? W : min(pass + 2 * R + 1, W);
+            const int la = passes == 1 ? 0 : pass * PUSH_TILE, lz = passes == 1 ? PUSH_CELLS : la + PUSH_TILE;
+            const int base = col_base[ca], staged_n = col_base[cb] - base;
+            const bool staged = staged_n <= PUSH_STAGE;
+            if (staged) {
+                for (int q = threadIdx.x; q < (cb - ca) * W; q += PUSH_THREADS) {
+                    const int ci = ca + q / W, ri = q - (q / W) * W;
+                    const int cc = wc0 + ci, rr = wr0 + ri;
+                    const int w = ci * W + ri;
+                    if (cc < 0 || cc >= S.cols || rr < 0 || rr >= S.rows) {
+                        lb[w] = 0;
+                        le[w] = 0;
+                    } else {
+                        const int cell = cc * S.rows + rr;
+                        lb[w] = col_base[ci] - base + (S.cstart[cell] - col_g0[ci]);
+                        le[w] = col_base[ci] - base + (S.cstart[cell + 1] - col_g0[ci]);
+                    }
+                }
+                // One pass over every staged electron (column found by search), so a
+                // sparse window costs one round of loads rather than one per column.
+                for (int j = threadIdx.x; j < staged_n; j += PUSH_THREADS) {
+                    int q = ca;
+                    for (int step = PUSH_WIN / 2; step > 0; step >>= 1)
+                        if (q + step < cb && col_base[q + step] - base <= j) q += step;
+                    const int g = col_g0[q] + (j - (col_base[q] - base));
+                    const int rr = S.scell[g] - (wc0 + q) * S.rows;
+                    lp[j] = fixed_pos(q, rr - wr0, S.sfx[g], S.sfy[g]);
                 }
             }
-            // One pass over every staged electron (column found by search), so a
-            // sparse window costs one round of loads rather than one per column.
-            const int staged_n = col_base[W];
-            for (int j = threadIdx.x; j < staged_n; j += PUSH_THREADS) {
-                int q = 0;
-                for (int step = PUSH_WIN / 2; step > 0; step >>= 1)
-                    if (q + step < W && col_base[q + step] <= j) q += step;
-                const int g = col_g0[q] + (j - col_base[q]);
-                const int rr = S.scell[g] - (wc0 + q) * S.rows;
-                lp[j] = fixed_pos(q, rr - wr0, S.sfx[g], S.sfy[g]);
+            __syncthreads();
+            for (int it = item_pref[la] + wave; it < item_pref[lz]; it += PUSH_WAVES) {
+                int lo = la, hi = lz;
+                while (hi - lo > 1) {
+                    const int mid = (lo + hi) >> 1;
+                    if (item_pref[mid] <= it) lo = mid;
+                    else hi = mid;
+                }
+                lo = __builtin_amdgcn_readfirstlane(lo);
+                const int hc = tc0 + lo / PUSH_TILE, hr = tr0 + lo % PUSH_TILE;
+                const int chunk = it - item_pref[lo];
+                const int e0 = __builtin_amdgcn_readfirstlane(cell_e0[lo]), n = __builtin_amdgcn_readfirstlane(cell_n[lo]);
+                if (staged)
+                    cell_item<FORCE>(S, LdsRuns{lp, lb, le, wc0, wr0, W}, gt, dens_w, hc, hr, e0, n, chunk, wc0, wr0, W,
+                                     k, h, t, force_out);
+                else
+                    cell_item<FORCE>(S, GlobalRuns{S, wc0, wr0}, gt, dens_w, hc, hr, e0, n, chunk, wc0, wr0, W, k, h, t,
+                                     force_out);
             }
+            __syncthreads();
         }
-        __syncthreads();
-        const int n_items = item_pref[PUSH_CELLS];
-        for (int it = wave; it < n_items; it += PUSH_WAVES) {
-            int lo = 0, hi = PUSH_CELLS;
-            while (hi - lo > 1) {
-                const int mid = (lo + hi) >> 1;
-                if (item_pref[mid] <= it) lo = mid;
-                else hi = mid;
-            }
-            lo = __builtin_amdgcn_readfirstlane(lo);
-            const int hc = tc0 + lo / PUSH_TILE, hr = tr0 + lo % PUSH_TILE;
-            const int chunk = it - item_pref[lo];
-            if (staged)
-                cell_item<FORCE>(S, LdsRuns{lp, lb, le, wc0, wr0, W}, gt, hc, hr, chunk, wc0, wr0, k, h, t, force_out);
-            else
-                cell_item<FORCE>(S, GlobalRuns{S, wc0, wr0}, gt, hc, hr, chunk, wc0, wr0, k, h, t, force_out);
-        }
-        __syncthreads();
     }
 }
 
@@ -901,6 +947,105 @@ __global__ __launch_bounds__(256) void k_push_sparse(Batch b, int k, float2 *for
 
 template __global__ void k_push_sparse<false>(Batch, int, float2 *);
 template __global__ void k_push_sparse<true>(Batch, int, float2 *);
+
+// Substep k (FORCE: the close-range force) for the electrons of crowded cells
+// (>= HEAVY_CELL, listed by the sort). A block takes whole cells from the list,
+// stages the cell's (2 reach + 1)^2 window and spreads the cell's 16-electron
+// chunks over its waves: a cluster's core is shared among many CUs instead of
+// weighing on its tile's one block.
+constexpr int HEAVY_I = 16;
+constexpr int HEAVY_WIN = 2 * 4 + 1;   // window side for reach <= 4
+
+template <bool FORCE>
+__global__ __launch_bounds__(PUSH_THREADS) void k_push_heavy(Batch b, int k, float2 *force_out) {
+    __shared__ int2 lp[PUSH_STAGE];
+    __shared__ float gt[PAIR_SAMPLES + 4];
+    __shared__ int lb[HEAVY_WIN * HEAVY_WIN], le[HEAVY_WIN * HEAVY_WIN];
+    __shared__ float dens_w[HEAVY_WIN * HEAVY_WIN];
+    __shared__ int col_base[HEAVY_WIN + 1], col_g0[HEAVY_WIN];
+    bool table_loaded = false;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int blk = blockIdx.x; blk < b.total[KC_HEAVY]; blk += gridDim.x) {
+        const int s = scene_of(b, KC_HEAVY, blk);
+        const DevScene &S = b.scenes[s];
+        if (!S.close_on) continue;
+        const uint32_t t = S.ctr->tick;
+        int sub = 1;
+        if (!FORCE) {
+            sub = substeps_of(S, t);
+            if (k >= sub) continue;
+        }
+        const float h = S.dt / (float)sub;
+        const int *pre = b.prefix + KC_HEAVY * (b.n_scenes + 1);
+        const int nblk = pre[s + 1] - pre[s];
+        const int heavy_n = S.ctr->heavy_n;
+        for (int hcell = block_in_scene(b, KC_HEAVY, s, blk); hcell < heavy_n; hcell += nblk) {
+            if (!table_loaded) {   // the pair table is shared by every scene
+                for (int q = threadIdx.x; q < PAIR_SAMPLES + 4; q += PUSH_THREADS) gt[q] = S.g_table[q];
+                table_loaded = true;
+            }
+            const int cell = S.heavy_cells[hcell];
+            const int hc = cell / S.rows, hr = cell - (cell / S.rows) * S.rows;
+            const int R = S.reach, W = 2 * R + 1;
+            const int wc0 = hc - R, wr0 = hr - R;
+            const int rlo = max(wr0, 0), rhi = min(wr0 + W, S.rows) - 1;
+            if (wave == 0) {
+                const int cc = wc0 + lane;
+                int g0 = 0, m = 0;
+                if (lane < W && cc >= 0 && cc < S.cols) {
+                    g0 = S.cstart[cc * S.rows + rlo];
+                    m = S.cstart[cc * S.rows + rhi + 1] - g0;
+                }
+                const int incl = wave_scan_incl(m);
+                if (lane < W) {
+                    col_base[lane] = incl - m;
+                    col_g0[lane] = g0;
+                }
+                if (lane == W - 1) col_base[W] = incl;
+            }
+            __syncthreads();
+            const bool staged = col_base[W] <= PUSH_STAGE;
+            for (int q = threadIdx.x; q < W * W; q += PUSH_THREADS) {
+                const int ci = q / W, ri = q - (q / W) * W;
+                const int cc = wc0 + ci, rr = wr0 + ri;
+                const bool in = cc >= 0 && cc < S.cols && rr >= 0 && rr < S.rows;
+                const int c2 = cc * S.rows + rr;
+                dens_w[q] = in ? S.density[c2] : 0.0f;
+                if (staged) {
+                    lb[q] = in ? col_base[ci] + (S.cstart[c2] - col_g0[ci]) : 0;
+                    le[q] = in ? col_base[ci] + (S.cstart[c2 + 1] - col_g0[ci]) : 0;
+                }
+            }
+            if (staged) {
+                const int staged_n = col_base[W];
+                for (int j = threadIdx.x; j < staged_n; j += PUSH_THREADS) {
+                    int q = 0;
+                    for (int step = 8; step > 0; step >>= 1)   // W <= HEAVY_WIN = 9
+                        if (q + step < W && col_base[q + step] <= j) q += step;
+                    const int g = col_g0[q] + (j - col_base[q]);
+                    const int rr = S.scell[g] - (wc0 + q) * S.rows;
+                    lp[j] = fixed_pos(q, rr - wr0, S.sfx[g], S.sfy[g]);
+                }
+            }
+            __syncthreads();
+            const int e0 = __builtin_amdgcn_readfirstlane(S.cstart[cell]);
+            const int n = __builtin_amdgcn_readfirstlane(S.cstart[cell + 1]) - e0;
+            const int chunks = (n + HEAVY_I - 1) / HEAVY_I;
+            for (int ch = wave; ch < chunks; ch += PUSH_WAVES) {
+                if (staged)
+                    cell_item<FORCE>(S, LdsRuns{lp, lb, le, wc0, wr0, W}, gt, dens_w, hc, hr, e0, n, ch, wc0, wr0, W, k, h,
+                                     t, force_out, HEAVY_I);
+                else
+                    cell_item<FORCE>(S, GlobalRuns{S, wc0, wr0}, gt, dens_w, hc, hr, e0, n, ch, wc0, wr0, W, k, h, t,
+                                     force_out, HEAVY_I);
+            }
+            __syncthreads();
+        }
+    }
+}
+
+template __global__ void k_push_heavy<false>(Batch, int, float2 *);
+template __global__ void k_push_heavy<true>(Batch, int, float2 *);
 
 template __global__ void k_push_tiled<false>(Batch, int, float2 *);
 template __global__ void k_push_tiled<true>(Batch, int, float2 *);
@@ -1359,6 +1504,7 @@ hipError_t tick_kernels(const Batch &b, int fdtd_substeps, hipStream_t stream, h
     EMB_MARK(1);
     EMB_LAUNCH(k_push_tiled<false>, KC_PUSH, PUSH_THREADS, b, 0, nullptr);
     EMB_LAUNCH(k_push_sparse<false>, KC_ELECTRONS, 256, b, 0, nullptr);
+    EMB_LAUNCH(k_push_heavy<false>, KC_HEAVY, PUSH_THREADS, b, 0, nullptr);
     EMB_MARK(2);
     if (b.total[KC_ELECTRONS] > 0) EMB_LAUNCH(k_rest, KC_SCENE, 1024, b);
     EMB_MARK(3);
@@ -1401,6 +1547,7 @@ hipError_t diagnostics_kernel(const DevScene *scene, double c2, double *out, int
 hipError_t close_force_kernel(const Batch &b, float2 *out, hipStream_t stream) {
     EMB_LAUNCH(k_push_tiled<true>, KC_PUSH, PUSH_THREADS, b, 0, out);
     EMB_LAUNCH(k_push_sparse<true>, KC_ELECTRONS, 256, b, 0, out);
+    EMB_LAUNCH(k_push_heavy<true>, KC_HEAVY, PUSH_THREADS, b, 0, out);
     return hipGetLastError();
 }
 

@@ -80,6 +80,7 @@ void layout(Bump &a, const Counts &c, DevScene &d, DevScene **d_scene, int32_t *
     d.toff = a.take<int32_t>((cells + SCAN_TILE - 1) / SCAN_TILE);
     d.cstart = a.take<int32_t>(cells + 1);
     d.tile_dense = a.take<uint8_t>((size_t)((c.cols + PUSH_TILE - 1) / PUSH_TILE) * ((c.rows + PUSH_TILE - 1) / PUSH_TILE));
+    d.heavy_cells = a.take<int32_t>(n / HEAVY_CELL + 1);
     d.cur = a.take<long long>(2 * cells);
     d.static_rho = a.take<long long>(cells);
     d.info = a.take<CellInfo>(cells);
@@ -145,6 +146,11 @@ int shared_tables(EmbDevice *dev, const EmbSettings &s, int nx, int ny, const fl
             for (size_t i = 0; i + 1 < h->g.size(); i++) gl[i] = make_float2((float)h->g[i], (float)(h->g[i + 1] - h->g[i]));
             EMB_HIP(hipMalloc(&t.g_lerp, gl.size() * sizeof(float2)));
             EMB_HIP(hipMemcpy(t.g_lerp, gl.data(), gl.size() * sizeof(float2), hipMemcpyHostToDevice));
+            std::vector<float2> sf(h->square.size() / 3);
+            for (size_t i = 0; i < sf.size(); i++)
+                sf[i] = make_float2((float)h->square[3 * i + 1], (float)h->square[3 * i + 2]);
+            EMB_HIP(hipMalloc(&t.square_f2, sf.size() * sizeof(float2)));
+            EMB_HIP(hipMemcpy(t.square_f2, sf.data(), sf.size() * sizeof(float2), hipMemcpyHostToDevice));
             std::vector<float> is(h->ion_sum.begin(), h->ion_sum.end());
             EMB_HIP(hipMalloc(&t.ion_sum, is.size() * sizeof(float)));
             EMB_HIP(hipMemcpy(t.ion_sum, is.data(), is.size() * sizeof(float), hipMemcpyHostToDevice));
@@ -192,6 +198,7 @@ void scene_blocks(const DevScene &d, int out[KC_COUNT]) {
     out[KC_STACKS] = std::max(1, d.n_stacks);
     out[KC_SCENE] = 1;
     out[KC_PUSH] = cdiv(d.cols, PUSH_TILE) * cdiv(d.rows, PUSH_TILE);
+    out[KC_HEAVY] = (int)std::min<long long>(256, std::max<long long>(1, d.n / 16384));
 }
 
 int make_resident(EmbDevice *dev, const EmbSettings &s, HostScene &&hs_in, hipStream_t st, Resident &r) {
@@ -236,6 +243,7 @@ int make_resident(EmbDevice *dev, const EmbSettings &s, HostScene &&hs_in, hipSt
     d.square_n = tables.square_n;
     d.g_table = tables.g;
     d.square = tables.square;
+    d.square_f2 = reinterpret_cast<const lerp2 *>(tables.square_f2);
     d.ion_sum = tables.ion_sum;
     d.g_lerp = tables.g_lerp;
     d.ion_a0 = tables.ion_a0;

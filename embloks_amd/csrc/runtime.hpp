@@ -42,6 +42,7 @@ struct DeviceTables {
     float *ion_sum = nullptr;
     int ion_a0 = 0;
     float2 *g_lerp = nullptr;
+    float2 *square_f2 = nullptr;
 };
 
 }  // namespace emb
