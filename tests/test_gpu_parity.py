@@ -224,3 +224,39 @@ def test_ring_circuit_two_thousand_ticks():
     assert err["ex"] < 1e-2 and err["ey"] < 1e-2 and err["phi"] < 1e-2, err
     assert err["gain"] < 1e-3, err
     assert residual < 1e-4, residual
+
+
+def crowded_world(carriers=80):
+    # A conductor block at 80 carriers per cell: every cell takes the crowded-cell
+    # path (k_push_heavy), its neighbours the cell mode, the vacuum edge nothing.
+    s = dict(W.abi.DEFAULTS)
+    s["carriers_per_cell"] = carriers
+    w = W.World(24, 20, s, seed=3)
+    w.paint(W.block(8, 16, 6, 14), W.CONDUCTOR, resistance=1.0)
+    w.paint([(8, 10)], W.BATTERY, emf=1.0, orientation=W.RIGHT)
+    return w
+
+
+@gpu
+def test_crowded_cells_one_tick_match_the_oracle():
+    world = crowded_world()
+    assert world.n_electrons > 64 * 60
+    cpu, sim = reset_and_compare(world)
+    pos, _, _ = sim.read_state()
+    cpos, _, _ = cpu.state()
+    ex, ey, _ = sim.read_yee()
+    cex, cey, _ = cpu.yee()
+    err = {"pos": float(np.max(np.abs(pos - cpos))), "ex": relative_error(ex, cex), "ey": relative_error(ey, cey),
+           "phi": relative_error(sim.read_potential(), cpu.potential_nodes())}
+    print("crowded cells one tick", err)
+    assert err["pos"] < 1e-4 and err["ex"] < 1e-5 and err["ey"] < 1e-5 and err["phi"] < 1e-5, err
+
+
+@gpu
+def test_crowded_cells_are_deterministic():
+    world = crowded_world()
+    a, b = gpusim()(world), gpusim()(world)
+    a.run_ticks(5)
+    b.run_ticks(5)
+    for x, y in zip(a.read_yee() + a.read_state()[:2], b.read_yee() + b.read_state()[:2]):
+        np.testing.assert_array_equal(x, y)
