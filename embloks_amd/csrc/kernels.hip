@@ -1122,11 +1122,20 @@ __global__ __launch_bounds__(1024) void k_rest(Batch b) {
 constexpr int ST = 32;          // output tile
 constexpr int MAX_PASSES = 4;
 constexpr int SW = ST + 2 * MAX_PASSES;   // LDS row pitch: window of the tile plus a halo of `passes`
+constexpr int DEP_THREADS = 1024;
+constexpr int DEP_SRC = (SW + 2) * (SW + 2);   // source cells of a window
 
-__global__ __launch_bounds__(256) void k_deposit_smooth(Batch b, int bias) {
+__device__ __forceinline__ int wave_max_int(int v) {
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+__global__ __launch_bounds__(DEP_THREADS) void k_deposit_smooth(Batch b, int bias) {
     __shared__ long long dep[3][SW * SW];
     __shared__ long long tmp[ST * SW];
-    __shared__ float wmax[4];
+    __shared__ float wmax[DEP_THREADS / 64];
+    __shared__ int cell_first[DEP_SRC], cell_count[DEP_SRC];
+    __shared__ int wmost[DEP_THREADS / 64];
     for (int blk = blockIdx.x; blk < b.total[KC_SMOOTH]; blk += gridDim.x) {
         const int s = scene_of(b, KC_SMOOTH, blk);
         const DevScene &S = b.scenes[s];
@@ -1140,7 +1149,7 @@ __global__ __launch_bounds__(256) void k_deposit_smooth(Batch b, int bias) {
         const int wc0 = R0x - 2 * P, wr0 = R0y - 2 * P;
         const int cells = S.cols * S.rows;
         const uint32_t t = S.ctr->tick + bias;
-        for (int q = threadIdx.x; q < W * W; q += 256) {
+        for (int q = threadIdx.x; q < W * W; q += DEP_THREADS) {
             const int i = q / W, j = q - (q / W) * W;
             const int c = wc0 + i, r = wr0 + j;
             const bool on = c >= 0 && r >= 0 && c < S.cols && r < S.rows;
@@ -1148,46 +1157,62 @@ __global__ __launch_bounds__(256) void k_deposit_smooth(Batch b, int bias) {
             dep[1][i * SW + j] = 0;
             dep[2][i * SW + j] = 0;
         }
-        __syncthreads();
         // Electrons whose CIC corners reach the window live in cells [w0 - 1, w0 + W].
         const int sc0 = max(wc0 - 1, 0), sc1 = min(wc0 + W, S.cols - 1);
         const int sr0 = max(wr0 - 1, 0), sr1 = min(wr0 + W, S.rows - 1);
-        for (int c = sc0; c <= sc1 && sr0 <= sr1; c++) {
-            const int e0 = S.cstart[c * S.rows + sr0], e1 = S.cstart[c * S.rows + sr1 + 1];
-            for (int k = e0 + (int)threadIdx.x; k < e1; k += 256) {
-                // CIC weights (fields.rs:27-40) from the cell + fraction position: the
-                // lower corner cell is floor(x - 1/2), with 22-bit fixed-point weights.
-                const int cell = S.scell[k];
-                const int ce = cell / S.rows, re = cell - (cell / S.rows) * S.rows;
-                const float x = S.sfx[k], y = S.sfy[k];
-                const int c0 = x >= 0.5f ? ce : ce - 1, r0 = y >= 0.5f ? re : re - 1;
-                const float ux = x >= 0.5f ? x - 0.5f : x + 0.5f, uy = y >= 0.5f ? y - 0.5f : y + 0.5f;
-                const int fu = (int)rintf(ux * 4194304.0f), fv = (int)rintf(uy * 4194304.0f);
-                const int wx[2] = {(1 << W_BITS) - fu, fu}, wy[2] = {(1 << W_BITS) - fv, fv};
-                const int ca = clampi(c0, 0, S.cols - 1), cb = clampi(c0 + 1, 0, S.cols - 1);
-                const int ra = clampi(r0, 0, S.rows - 1), rb = clampi(r0 + 1, 0, S.rows - 1);
-                const float vx = S.svx[k], vy = S.svy[k];
-                for (int a = 0; a < 2; a++) {
-                    const int li = (a ? cb : ca) - wc0;
-                    if (li < 0 || li >= W || wx[a] == 0) continue;
-                    for (int bb = 0; bb < 2; bb++) {
-                        const int lj = (bb ? rb : ra) - wr0;
-                        if (lj < 0 || lj >= W || wy[bb] == 0) continue;
-                        const long long w = (long long)wx[a] * (long long)wy[bb];
-                        const float wf = __fmul_rn((float)wx[a], (float)wy[bb]);
-                        const long long jx = __float2ll_rn(__fmul_rn(__fmul_rn(vx, wf), J_FROM_W));
-                        const long long jy = __float2ll_rn(__fmul_rn(__fmul_rn(vy, wf), J_FROM_W));
-                        const int at = li * SW + lj;
-                        atomicAdd((unsigned long long *)&dep[0][at], as_ull(-w));
-                        atomicAdd((unsigned long long *)&dep[1][at], as_ull(jx));
-                        atomicAdd((unsigned long long *)&dep[2][at], as_ull(jy));
-                    }
+        const int ncy = max(sr1 - sr0 + 1, 0), ncell = max(sc1 - sc0 + 1, 0) * ncy;
+        int most = 0;
+        for (int q = threadIdx.x; q < ncell; q += DEP_THREADS) {
+            const int cell = (sc0 + q / ncy) * S.rows + sr0 + q % ncy;
+            const int first = S.cstart[cell], n = S.cstart[cell + 1] - first;
+            cell_first[q] = first;
+            cell_count[q] = n;
+            most = max(most, n);
+        }
+        most = wave_max_int(most);
+        if ((threadIdx.x & 63) == 0) wmost[threadIdx.x >> 6] = most;
+        __syncthreads();
+        most = 0;
+        for (int w = 0; w < DEP_THREADS / 64; w++) most = max(most, wmost[w]);
+        // Lane q takes electron q / ncell of source cell q % ncell: neighbouring
+        // lanes work on neighbouring cells, so one atomic instruction mostly hits
+        // distinct addresses (in cell order, 16 lanes of a cell would share four).
+        // Integer sums: the same bits in any order.
+        for (int q = threadIdx.x; q < ncell * most; q += DEP_THREADS) {
+            const int ci = q % ncell, kk = q / ncell;
+            if (kk >= cell_count[ci]) continue;
+            const int k = cell_first[ci] + kk;
+            // CIC weights (fields.rs:27-40) from the cell + fraction position: the
+            // lower corner cell is floor(x - 1/2), with 22-bit fixed-point weights.
+            const int ce = sc0 + ci / ncy, re = sr0 + ci % ncy;
+            const float x = S.sfx[k], y = S.sfy[k];
+            const int c0 = x >= 0.5f ? ce : ce - 1, r0 = y >= 0.5f ? re : re - 1;
+            const float ux = x >= 0.5f ? x - 0.5f : x + 0.5f, uy = y >= 0.5f ? y - 0.5f : y + 0.5f;
+            const int fu = (int)rintf(ux * 4194304.0f), fv = (int)rintf(uy * 4194304.0f);
+            const int wx[2] = {(1 << W_BITS) - fu, fu}, wy[2] = {(1 << W_BITS) - fv, fv};
+            const int ca = clampi(c0, 0, S.cols - 1), cb = clampi(c0 + 1, 0, S.cols - 1);
+            const int ra = clampi(r0, 0, S.rows - 1), rb = clampi(r0 + 1, 0, S.rows - 1);
+            const float vx = S.svx[k], vy = S.svy[k];
+            for (int a = 0; a < 2; a++) {
+                const int li = (a ? cb : ca) - wc0;
+                if (li < 0 || li >= W || wx[a] == 0) continue;
+                for (int bb = 0; bb < 2; bb++) {
+                    const int lj = (bb ? rb : ra) - wr0;
+                    if (lj < 0 || lj >= W || wy[bb] == 0) continue;
+                    const long long w = (long long)wx[a] * (long long)wy[bb];
+                    const float wf = __fmul_rn((float)wx[a], (float)wy[bb]);
+                    const long long jx = __float2ll_rn(__fmul_rn(__fmul_rn(vx, wf), J_FROM_W));
+                    const long long jy = __float2ll_rn(__fmul_rn(__fmul_rn(vy, wf), J_FROM_W));
+                    const int at = li * SW + lj;
+                    atomicAdd((unsigned long long *)&dep[0][at], as_ull(-w));
+                    atomicAdd((unsigned long long *)&dep[1][at], as_ull(jx));
+                    atomicAdd((unsigned long long *)&dep[2][at], as_ull(jy));
                 }
             }
         }
         // Cells owned by this tile: current (physics.rs:34-43) and speed maximum.
         float vmax = 0.0f;
-        for (int q = threadIdx.x; q < ST * ST; q += 256) {
+        for (int q = threadIdx.x; q < ST * ST; q += DEP_THREADS) {
             const int c = R0x - P + q / ST, r = R0y - P + (q % ST);
             if (c < 0 || r < 0 || c >= S.cols || r >= S.rows) continue;
             const int cell = c * S.rows + r;
@@ -1205,11 +1230,12 @@ __global__ __launch_bounds__(256) void k_deposit_smooth(Batch b, int bias) {
         if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = vmax;
         __syncthreads();
         if (threadIdx.x == 0) {
-            const float m = fmaxf(fmaxf(wmax[0], wmax[1]), fmaxf(wmax[2], wmax[3]));
+            float m = wmax[0];
+            for (int w = 1; w < DEP_THREADS / 64; w++) m = fmaxf(m, wmax[w]);
             atomicMax(&S.ctr->vmax_bits[(t + 1) & 1], __float_as_uint(m));
         }
         if (S.dep_dump) {
-            for (int q = threadIdx.x; q < ST * ST; q += 256) {
+            for (int q = threadIdx.x; q < ST * ST; q += DEP_THREADS) {
                 const int c = R0x - P + q / ST, r = R0y - P + (q % ST);
                 if (c < 0 || r < 0 || c >= S.cols || r >= S.rows) continue;
                 const int at = (q / ST + P) * SW + (q % ST + P);
@@ -1231,14 +1257,14 @@ __global__ __launch_bounds__(256) void k_deposit_smooth(Batch b, int bias) {
             const float scale = ch == 0 ? S.rho_from_fixed : S.j_from_fixed;
             float *out = ch == 0 ? S.rho : (ch == 1 ? S.jx : S.jy);
             __syncthreads();
-            for (int q = threadIdx.x; q < ST * W; q += 256) {
+            for (int q = threadIdx.x; q < ST * W; q += DEP_THREADS) {
                 const int i = q / W, j = q - (q / W) * W;
                 long long acc = 0;
                 for (int d = 0; d <= 2 * P; d++) acc += bin[d] * dep[ch][(i + d) * SW + j];
                 tmp[i * SW + j] = acc;
             }
             __syncthreads();
-            for (int q = threadIdx.x; q < ST * ST; q += 256) {
+            for (int q = threadIdx.x; q < ST * ST; q += DEP_THREADS) {
                 const int i = q / ST, j = q % ST;
                 const int a = R0x + i, rr = R0y + j;
                 if (a >= region_c || rr >= region_r) continue;
@@ -1484,7 +1510,7 @@ hipError_t sort_kernels(const Batch &b, hipStream_t stream) {
 hipError_t setup_kernels(const Batch &b, hipStream_t stream) {
     hipError_t e = sort_kernels(b, stream);
     if (e != hipSuccess) return e;
-    EMB_LAUNCH(k_deposit_smooth, KC_SMOOTH, 256, b, -1);
+    EMB_LAUNCH(k_deposit_smooth, KC_SMOOTH, DEP_THREADS, b, -1);
     EMB_LAUNCH(k_rhs, KC_NODES, 256, b);
     e = launch_dst_gemms(b, stream);
     if (e != hipSuccess) return e;
@@ -1517,7 +1543,7 @@ hipError_t tick_kernels(const Batch &b, int fdtd_substeps, hipStream_t stream, h
     EMB_MARK(6);
     EMB_LAUNCH(k_canon, KC_CELLS, 256, b);
     EMB_MARK(7);
-    EMB_LAUNCH(k_deposit_smooth, KC_SMOOTH, 256, b, 0);
+    EMB_LAUNCH(k_deposit_smooth, KC_SMOOTH, DEP_THREADS, b, 0);
     EMB_MARK(8);
     for (int f = 0; f < fdtd_substeps; f++) EMB_LAUNCH(k_fdtd, KC_FDTD, 256, b, f);
     EMB_MARK(9);
@@ -1533,7 +1559,7 @@ extern const char *const TICK_PHASE_NAMES[] = {"trim", "push", "rest", "count", 
                                         "deposit_smooth", "fdtd", "dst_gemm", "correct"};
 
 hipError_t deposit_kernel(const Batch &b, hipStream_t stream) {
-    EMB_LAUNCH(k_deposit_smooth, KC_SMOOTH, 256, b, -1);
+    EMB_LAUNCH(k_deposit_smooth, KC_SMOOTH, DEP_THREADS, b, -1);
     return hipGetLastError();
 }
 
