@@ -213,14 +213,35 @@ __global__ __launch_bounds__(256) void k_canon(Batch b) {
             }
         }
         __syncthreads();
-        for (int q = S0 + (int)threadIdx.x; q < S1; q += 256) {
-            const int e = lorder[q - S0];
-            S.order[q] = e;
-            S.scell[q] = S.ecell[e];
-            S.sfx[q] = S.fx[e];
-            S.sfy[q] = S.fy[e];
-            S.svx[q] = S.vx[e];
-            S.svy[q] = S.vy[e];
+        // Four slots per thread per round: their gathers are all in flight at once.
+        for (int q0 = S0 + (int)threadIdx.x; q0 < S1; q0 += 4 * 256) {
+            int e[4];
+            int cel[4];
+            float fxv[4], fyv[4], vxv[4], vyv[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int q = q0 + u * 256;
+                e[u] = q < S1 ? lorder[q - S0] : 0;
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                cel[u] = S.ecell[e[u]];
+                fxv[u] = S.fx[e[u]];
+                fyv[u] = S.fy[e[u]];
+                vxv[u] = S.vx[e[u]];
+                vyv[u] = S.vy[e[u]];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int q = q0 + u * 256;
+                if (q >= S1) continue;
+                S.order[q] = e[u];
+                S.scell[q] = cel[u];
+                S.sfx[q] = fxv[u];
+                S.sfy[q] = fyv[u];
+                S.svx[q] = vxv[u];
+                S.svy[q] = vyv[u];
+            }
         }
         __syncthreads();
     }
@@ -278,22 +299,28 @@ __global__ __launch_bounds__(1024) void k_trim(Batch b) {
         const int per = 1 + (S.close_on ? side * side : 0);
         const int lists = I.low_n + I.high_n;
         float acc_lo = 0.0f, acc_hi = 0.0f;
-        for (int it = (int)threadIdx.x; it < lists * per; it += 1024) {
-            const int li = it / per, sub = it - li * per;
+        // Work item = (terminal cell, neighbour cell, one of TRIM_SPLIT lanes over
+        // that cell's electrons), so no thread walks a whole cell's electrons.
+        constexpr int TRIM_SPLIT = 4;
+        for (int it = (int)threadIdx.x; it < lists * per * TRIM_SPLIT; it += 1024) {
+            const int part = it % TRIM_SPLIT, item = it / TRIM_SPLIT;
+            const int li = item / per, sub = item - li * per;
             const bool high = li >= I.low_n;
             const int cell = S.stack_cells[high ? I.high_off + (li - I.low_n) : I.low_off + li];
             const int c = cell / S.rows, r = cell - c * S.rows;
             float v = 0.0f;
             if (sub == 0) {
-                v = S.phi[(size_t)(c + S.margin) * S.ny + (r + S.margin)];
-                if (S.close_on) v += S.ion_pot[cell];
+                if (part == 0) {
+                    v = S.phi[(size_t)(c + S.margin) * S.ny + (r + S.margin)];
+                    if (S.close_on) v += S.ion_pot[cell];
+                }
             } else {
                 const int cc = c + (sub - 1) / side - R, rr = r + (sub - 1) % side - R;
                 if (cc >= 0 && rr >= 0 && cc < S.cols && rr < S.rows) {
                     const int nb = cc * S.rows + rr;
                     // Offset of the electron from the centre of cell (c, r).
                     const float ox = (float)(cc - c) - 0.5f + S.half, oy = (float)(rr - r) - 0.5f + S.half;
-                    for (int k = S.cstart[nb]; k < S.cstart[nb + 1]; k++) {
+                    for (int k = S.cstart[nb] + part; k < S.cstart[nb + 1]; k += TRIM_SPLIT) {
                         const float u = (ox + S.sfx[k]) * S.square_samples;
                         const float w = (oy + S.sfy[k]) * S.square_samples;
                         v += square_floor(S, 0, u, w);
