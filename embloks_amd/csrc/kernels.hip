@@ -539,13 +539,12 @@ __device__ __forceinline__ void ion_uniform(const DevScene &S, int home, float p
 // cutoff^2. A pair at or beyond the cutoff reads the two zero samples past the
 // table's end, and the electron itself (d = 0) adds g(0) * 0: both contribute
 // exactly zero. The sums are in 2^-24 cells (k_pair_fx scales them back).
-__device__ __forceinline__ void pair_term(const float *gt, float s2, int ix, int iy, float &sx, float &sy) {
+__device__ __forceinline__ void pair_term(const lerp2 *gt, float s2, int ix, int iy, float &sx, float &sy) {
     const float dx = (float)ix, dy = (float)iy;
     const float d2 = (dx * dx + dy * dy) * s2;
     const float tt = d2 < (float)PAIR_SAMPLES ? d2 : (float)(PAIR_SAMPLES + 2);
-    const int si = (int)tt;
-    const float g0 = gt[si], g1 = gt[si + 1];
-    const float l = g0 + (g1 - g0) * __builtin_amdgcn_fractf(tt);
+    const lerp2 g = gt[(int)tt];
+    const float l = g.x + g.y * __builtin_amdgcn_fractf(tt);
     sx += l * dx;
     sy += l * dy;
 }
@@ -578,7 +577,7 @@ __device__ __forceinline__ void finish(const DevScene &S, int id, int hc, int hr
 // One electron's substep (physics.rs:139-158), one thread per electron: `slot`
 // is its place in the sorted snapshot.
 template <class NB>
-__device__ void push_core(const DevScene &S, const NB &nb, const float *gt, int slot, int k, float h, uint32_t t) {
+__device__ void push_core(const DevScene &S, const NB &nb, const lerp2 *gt, int slot, int k, float h, uint32_t t) {
     const int id = S.order[slot];
     const int home = S.scell[slot];
     const int hc = home / S.rows, hr = home - (home / S.rows) * S.rows;
@@ -628,7 +627,7 @@ __device__ void push_core(const DevScene &S, const NB &nb, const float *gt, int 
 }
 
 __device__ void push_one(const DevScene &S, int slot, int k, float h, uint32_t t) {
-    push_core(S, GlobalNeighbours{S}, S.g_table, slot, k, h, t);
+    push_core(S, GlobalNeighbours{S}, S.g_lerp, slot, k, h, t);
 }
 
 // Inclusive prefix sum over the 64 lanes of a wave.
@@ -682,7 +681,7 @@ struct GlobalRuns {
 // the neighbour runs; the partial sums meet in a fixed butterfly. FORCE: write
 // the close-range force to force_out[id] instead of pushing.
 template <bool FORCE, class RUNS>
-__device__ __forceinline__ void cell_item(const DevScene &S, const RUNS &runs, const float *gt, const float *dens_w,
+__device__ __forceinline__ void cell_item(const DevScene &S, const RUNS &runs, const lerp2 *gt, const float *dens_w,
                                           int hc, int hr, int e0, int n, int chunk, int wc0, int wr0, int W, int k,
                                           float h, uint32_t t, float2 *force_out, int imax = 64) {
     const int lane = threadIdx.x & 63;
@@ -756,7 +755,7 @@ __device__ __forceinline__ void cell_item(const DevScene &S, const RUNS &runs, c
 
 // The close-range force alone on one electron (electron mode of FORCE).
 template <class NB>
-__device__ void force_core(const DevScene &S, const NB &nb, const float *gt, int slot, float2 *force_out) {
+__device__ void force_core(const DevScene &S, const NB &nb, const lerp2 *gt, int slot, float2 *force_out) {
     const int home = S.scell[slot];
     const int hc = home / S.rows, hr = home - (home / S.rows) * S.rows;
     const float px = S.sfx[slot], py = S.sfy[slot];
@@ -803,7 +802,7 @@ __device__ void force_core(const DevScene &S, const NB &nb, const float *gt, int
 // butterfly. The other tiles' electrons go through k_push_sparse. Every sum
 // runs in a fixed order, so a scene gives the same bits alone or batched.
 constexpr int PUSH_WIN = PUSH_TILE + 2 * 4;   // window side for reach <= 4
-constexpr int PUSH_STAGE = 7168;   // positions (56 KB): the block stays under 80 KB of LDS, two per CU
+constexpr int PUSH_STAGE = 5120;   // positions (40 KB): with the 33 KB pair table, under 80 KB: two blocks per CU
 constexpr int PUSH_CELLS = PUSH_TILE * PUSH_TILE;
 constexpr int PUSH_THREADS = 512;             // 8 waves: two blocks fill a CU's 16 wave slots at <= 128 VGPRs
 constexpr int PUSH_WAVES = PUSH_THREADS / 64;
@@ -811,7 +810,7 @@ constexpr int PUSH_WAVES = PUSH_THREADS / 64;
 template <bool FORCE>
 __global__ __launch_bounds__(PUSH_THREADS) void k_push_tiled(Batch b, int k, float2 *force_out) {
     __shared__ int2 lp[PUSH_STAGE];
-    __shared__ float gt[PAIR_SAMPLES + 4];
+    __shared__ lerp2 gt[PAIR_SAMPLES + 4];
     __shared__ int lb[PUSH_WIN * PUSH_WIN], le[PUSH_WIN * PUSH_WIN];
     __shared__ float dens_w[PUSH_WIN * PUSH_WIN];               // ion density of the window's cells
     __shared__ int cell_e0[PUSH_CELLS], cell_n[PUSH_CELLS];     // tile cells: first sorted slot, electrons
@@ -835,7 +834,7 @@ __global__ __launch_bounds__(PUSH_THREADS) void k_push_tiled(Batch b, int k, flo
         if (!FORCE && k == 0 && threadIdx.x == 0 && local == 0) S.ctr->substeps = sub;
         if (!S.close_on || !S.tile_dense[local]) continue;
         if (!table_loaded) {   // the pair table is shared by every scene
-            for (int q = threadIdx.x; q < PAIR_SAMPLES + 4; q += PUSH_THREADS) gt[q] = S.g_table[q];
+            for (int q = threadIdx.x; q < PAIR_SAMPLES + 4; q += PUSH_THREADS) gt[q] = S.g_lerp[q];
             table_loaded = true;
         }
         const int tiles_r = (S.rows + PUSH_TILE - 1) / PUSH_TILE;
@@ -966,9 +965,9 @@ __global__ __launch_bounds__(256) void k_push_sparse(Batch b, int k, float2 *for
         const int cc = cell / S.rows, rr = cell - (cell / S.rows) * S.rows;
         if (S.close_on && S.tile_dense[(cc / PUSH_TILE) * ((S.rows + PUSH_TILE - 1) / PUSH_TILE) + rr / PUSH_TILE]) continue;
         if (FORCE)
-            force_core(S, GlobalNeighbours{S}, S.g_table, slot, force_out);
+            force_core(S, GlobalNeighbours{S}, S.g_lerp, slot, force_out);
         else
-            push_core(S, GlobalNeighbours{S}, S.g_table, slot, k, h, t);
+            push_core(S, GlobalNeighbours{S}, S.g_lerp, slot, k, h, t);
     }
 }
 
@@ -986,7 +985,7 @@ constexpr int HEAVY_WIN = 2 * 4 + 1;   // window side for reach <= 4
 template <bool FORCE>
 __global__ __launch_bounds__(PUSH_THREADS) void k_push_heavy(Batch b, int k, float2 *force_out) {
     __shared__ int2 lp[PUSH_STAGE];
-    __shared__ float gt[PAIR_SAMPLES + 4];
+    __shared__ lerp2 gt[PAIR_SAMPLES + 4];
     __shared__ int lb[HEAVY_WIN * HEAVY_WIN], le[HEAVY_WIN * HEAVY_WIN];
     __shared__ float dens_w[HEAVY_WIN * HEAVY_WIN];
     __shared__ int col_base[HEAVY_WIN + 1], col_g0[HEAVY_WIN];
@@ -1008,7 +1007,7 @@ __global__ __launch_bounds__(PUSH_THREADS) void k_push_heavy(Batch b, int k, flo
         const int heavy_n = S.ctr->heavy_n;
         for (int hcell = block_in_scene(b, KC_HEAVY, s, blk); hcell < heavy_n; hcell += nblk) {
             if (!table_loaded) {   // the pair table is shared by every scene
-                for (int q = threadIdx.x; q < PAIR_SAMPLES + 4; q += PUSH_THREADS) gt[q] = S.g_table[q];
+                for (int q = threadIdx.x; q < PAIR_SAMPLES + 4; q += PUSH_THREADS) gt[q] = S.g_lerp[q];
                 table_loaded = true;
             }
             const int cell = S.heavy_cells[hcell];

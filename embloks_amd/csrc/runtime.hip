@@ -142,8 +142,14 @@ int shared_tables(EmbDevice *dev, const EmbSettings &s, int nx, int ny, const fl
             EMB_HIP(hipMalloc(&t.square, sq.size() * sizeof(float)));
             EMB_HIP(hipMemcpy(t.g, g.data(), g.size() * sizeof(float), hipMemcpyHostToDevice));
             EMB_HIP(hipMemcpy(t.square, sq.data(), sq.size() * sizeof(float), hipMemcpyHostToDevice));
-            std::vector<float2> gl(h->g.size() - 1);
-            for (size_t i = 0; i + 1 < h->g.size(); i++) gl[i] = make_float2((float)h->g[i], (float)(h->g[i + 1] - h->g[i]));
+            // (g[i], g[i+1] - g[i]) for every sample the lerp can start at,
+            // including the two zero samples past the end.
+            std::vector<double> gp(h->g.begin(), h->g.end());
+            gp.push_back(0.0);
+            gp.push_back(0.0);
+            gp.push_back(0.0);
+            std::vector<float2> gl(gp.size() - 1);
+            for (size_t i = 0; i + 1 < gp.size(); i++) gl[i] = make_float2((float)gp[i], (float)(gp[i + 1] - gp[i]));
             EMB_HIP(hipMalloc(&t.g_lerp, gl.size() * sizeof(float2)));
             EMB_HIP(hipMemcpy(t.g_lerp, gl.data(), gl.size() * sizeof(float2), hipMemcpyHostToDevice));
             std::vector<float2> sf(h->square.size() / 3);
