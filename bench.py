@@ -52,6 +52,8 @@ def algorithmic(phase: str, nx: int, ny: int, cols: int, rows: int, n: int):
         return 12.0 * cells, "byte", "hbm"
     if phase == "correct":
         return 16.0 * nodes, "byte", "hbm"
+    if phase == "finish":
+        return 16.0 * cells + 40.0 * cells, "byte", "hbm"   # node E, B in; display and current averages in and out
     return 0.0, "byte", "hbm"
 
 
@@ -102,6 +104,7 @@ def run_single(args):
 PHASE_KERNELS = {"trim": ["k_trim"], "push": ["k_push_tiled<false>", "k_push_sparse<false>"], "rest": ["k_rest"],
                  "count": ["k_count"], "scan": ["k_scan", "k_scan_top"], "scatter": ["k_scatter"], "canon": ["k_canon"],
                  "deposit_smooth": ["k_deposit_smooth"], "fdtd": ["k_fdtd"], "correct": ["k_correct"],
+                 "finish": ["k_finish"],
                  "dst_gemm": [f"k_dst_gemm<{i}>" for i in range(4)]}
 
 

@@ -123,6 +123,8 @@ struct DevScene {
     float j_from_fixed;       // q_e / 2^28 / 4^(2 passes)
     float i_from_fixed;       // q_e / 2^28 (cell-binned current)
     float pair_s2;            // g_scale 2^-48: a squared 24-bit fixed-point offset to a table coordinate
+    float alpha_current;      // min(dt / current_smoothing_time, 1) (sim.rs:256)
+    float alpha_display;      // 1 - exp(-dt / display_smoothing_time), 1 when that time is 0 (sim.rs:257-261)
     int32_t passes;
     int32_t pad1;
     // --- fields (FP32)
@@ -149,6 +151,8 @@ struct DevScene {
     int32_t *heavy_cells;     // cells holding >= HEAVY_CELL electrons (ctr->heavy_n of them)
     // --- deposit
     long long *cur;           // [2 * cells]: cell-binned sum of v, units 2^-28 (x q_e)
+    float *cur_smooth;        // [2 * cells]: exponential average of the cell current (channel-planar)
+    float *display;           // [3 * cells]: exponential average of E (x, y) and B at cell centres
     const long long *static_rho;   // ions + fixed charges per cell, units 2^-44 electron charges
     long long *dep_dump;      // optional [3 * cells]: unsmoothed electron deposit (diagnostics)
     // --- materials, tables, batteries

@@ -1544,6 +1544,44 @@ hipError_t setup_kernels(const Batch &b, hipStream_t stream) {
     return hipGetLastError();
 }
 
+// End of a tick (finish_tick, fields.wgsl:270-281): the displayed E and B at
+// the cell centres and the smoothed (ammeter) cell current move towards their
+// instantaneous values by alpha. sync = 1 (finish_sync, fields.wgsl:284-290,
+// after a scene is set up or edited): the display jumps to the exact fields
+// and the current average is left alone.
+__global__ __launch_bounds__(256) void k_finish(Batch b, int sync) {
+    for (int blk = blockIdx.x; blk < b.total[KC_CELLS]; blk += gridDim.x) {
+        const int s = scene_of(b, KC_CELLS, blk);
+        const DevScene &S = b.scenes[s];
+        const int cells = S.cols * S.rows;
+        const int cell = block_in_scene(b, KC_CELLS, s, blk) * 256 + (int)threadIdx.x;
+        if (cell >= cells) continue;
+        const int c = cell / S.rows, r = cell - (cell / S.rows) * S.rows;
+        const int i = c + S.margin, j = r + S.margin;
+        const int ny = S.ny;
+        const float ex = 0.5f * (S.ex[(size_t)i * ny + j] + S.ex[(size_t)(i + 1) * ny + j]);
+        const float ey = 0.5f * (S.ey[(size_t)i * (ny + 1) + j] + S.ey[(size_t)i * (ny + 1) + j + 1]);
+        const float bz = 0.25f * (S.bz[(size_t)i * (ny + 1) + j] + S.bz[(size_t)(i + 1) * (ny + 1) + j] +
+                                  S.bz[(size_t)i * (ny + 1) + j + 1] + S.bz[(size_t)(i + 1) * (ny + 1) + j + 1]);
+        const float a = sync ? 1.0f : S.alpha_display;
+        float *d = S.display;
+        d[cell] += a * (ex - d[cell]);
+        d[cells + cell] += a * (ey - d[cells + cell]);
+        d[2 * cells + cell] += a * (bz - d[2 * cells + cell]);
+        if (!sync) {
+            for (int k = 0; k < 2; k++) {
+                const float now = (float)S.cur[k * cells + cell] * S.i_from_fixed;
+                S.cur_smooth[k * cells + cell] += S.alpha_current * (now - S.cur_smooth[k * cells + cell]);
+            }
+        }
+    }
+}
+
+hipError_t finish_sync_kernel(const Batch &b, hipStream_t stream) {
+    EMB_LAUNCH(k_finish, KC_CELLS, 256, b, 1);
+    return hipGetLastError();
+}
+
 // One tick (Simulation::step, simulation.rs:63-72). With `marks`, an event is
 // recorded before each phase (TICK_PHASES of them) and one after the last.
 #define EMB_MARK(p) \
@@ -1578,11 +1616,13 @@ hipError_t tick_kernels(const Batch &b, int fdtd_substeps, hipStream_t stream, h
     EMB_MARK(10);
     EMB_LAUNCH(k_correct, KC_NODES, 256, b, 1);
     EMB_MARK(11);
+    EMB_LAUNCH(k_finish, KC_CELLS, 256, b, 0);
+    EMB_MARK(12);
     return hipGetLastError();
 }
 
 extern const char *const TICK_PHASE_NAMES[] = {"trim", "push", "rest", "count", "scan", "scatter", "canon",
-                                        "deposit_smooth", "fdtd", "dst_gemm", "correct"};
+                                        "deposit_smooth", "fdtd", "dst_gemm", "correct", "finish"};
 
 hipError_t deposit_kernel(const Batch &b, hipStream_t stream) {
     EMB_LAUNCH(k_deposit_smooth, KC_SMOOTH, DEP_THREADS, b, -1);

@@ -82,6 +82,8 @@ void layout(Bump &a, const Counts &c, DevScene &d, DevScene **d_scene, int32_t *
     d.tile_dense = a.take<uint8_t>((size_t)((c.cols + PUSH_TILE - 1) / PUSH_TILE) * ((c.rows + PUSH_TILE - 1) / PUSH_TILE));
     d.heavy_cells = a.take<int32_t>(n / HEAVY_CELL + 1);
     d.cur = a.take<long long>(2 * cells);
+    d.cur_smooth = a.take<float>(2 * cells);
+    d.display = a.take<float>(3 * cells);
     d.static_rho = a.take<long long>(cells);
     d.info = a.take<CellInfo>(cells);
     d.density = a.take<float>(cells);
@@ -267,6 +269,8 @@ int make_resident(EmbDevice *dev, const EmbSettings &s, HostScene &&hs_in, hipSt
     d.g_scale = s.electron_cutoff > 0.0 ? (float)(4096.0 / (s.electron_cutoff * s.electron_cutoff)) : 0.0f;
     d.pair_s2 = s.electron_cutoff > 0.0 ? (float)(4096.0 / (s.electron_cutoff * s.electron_cutoff) / 281474976710656.0) : 0.0f;
     d.k_pair_fx = (float)(k * q * q / 16777216.0);
+    d.alpha_current = (float)std::min(s.dt / s.current_smoothing_time, 1.0);
+    d.alpha_display = s.display_smoothing_time > 0.0 ? (float)(1.0 - std::exp(-s.dt / s.display_smoothing_time)) : 1.0f;
     d.half = (float)tables.half;
     d.square_samples = 32.0f;
     d.c2h = (float)(hs.h_fdtd * s.speed_of_light * s.speed_of_light);
@@ -486,6 +490,7 @@ int emb_sim_create(EmbDevice *dev, const EmbSettings *settings, const EmbScene *
         if (rc == EMB_OK) {
             sim->r.time = scene->time;
             e = setup_kernels(sim->r.batch, sim->stream);
+            if (e == hipSuccess) e = finish_sync_kernel(sim->r.batch, sim->stream);   // display = fields
             if (e == hipSuccess) e = hipStreamSynchronize(sim->stream);
             if (e != hipSuccess) rc = fail(EMB_E_DEVICE, std::string("setup: ") + hipGetErrorString(e));
         }
@@ -503,7 +508,6 @@ int emb_sim_create(EmbDevice *dev, const EmbSettings *settings, const EmbScene *
 }
 
 int emb_sim_upload(EmbSim *sim, const EmbScene *scene, int reset_display) {
-    (void)reset_display;   // the display average is not kept on this backend
     EMB_GUARD({
         if (!sim || !scene) return fail(EMB_E_INVALID, "null argument");
         if (scene->cols != sim->r.hs.cols || scene->rows != sim->r.hs.rows)
@@ -525,9 +529,14 @@ int emb_sim_upload(EmbSim *sim, const EmbScene *scene, int reset_display) {
         EMB_HIP(hipMemcpyAsync(d.ey, o.ey, nx * (ny + 1) * 4, hipMemcpyDeviceToDevice, sim->stream));
         EMB_HIP(hipMemcpyAsync(d.bz, o.bz, (nx + 1) * (ny + 1) * 4, hipMemcpyDeviceToDevice, sim->stream));
         EMB_HIP(hipMemcpyAsync(&d.ctr->tick, &o.ctr->tick, sizeof(uint32_t), hipMemcpyDeviceToDevice, sim->stream));
+        // So do the display and the smoothed current (the grid shape is unchanged).
+        const size_t cells = (size_t)o.cols * o.rows;
+        EMB_HIP(hipMemcpyAsync(d.display, o.display, 3 * cells * 4, hipMemcpyDeviceToDevice, sim->stream));
+        EMB_HIP(hipMemcpyAsync(d.cur_smooth, o.cur_smooth, 2 * cells * 4, hipMemcpyDeviceToDevice, sim->stream));
         fresh.time = sim->r.time;
         fresh.tick = sim->r.tick;
         EMB_HIP(setup_kernels(fresh.batch, sim->stream));
+        if (reset_display) EMB_HIP(finish_sync_kernel(fresh.batch, sim->stream));
         EMB_HIP(hipStreamSynchronize(sim->stream));
         free_resident(sim->r);
         sim->r = std::move(fresh);
@@ -645,6 +654,56 @@ int emb_sim_read_potential(EmbSim *sim, double *potential) {
         EMB_HIP(hipStreamSynchronize(sim->stream));
         const DevScene &d = sim->r.ds;
         return copy_back(sim, d.phi, (size_t)d.nx * d.ny, potential);
+    })
+}
+
+int emb_sim_read_display(EmbSim *sim, double *e, double *b) {
+    EMB_GUARD({
+        if (!sim || !e || !b) return fail(EMB_E_INVALID, "null argument");
+        EMB_HIP(hipStreamSynchronize(sim->stream));
+        const DevScene &d = sim->r.ds;
+        const size_t cells = (size_t)d.cols * d.rows;
+        std::vector<float> v(3 * cells);
+        EMB_HIP(hipMemcpy(v.data(), d.display, v.size() * 4, hipMemcpyDeviceToHost));
+        for (size_t c = 0; c < cells; c++) {
+            e[2 * c] = v[c];
+            e[2 * c + 1] = v[cells + c];
+            b[c] = v[2 * cells + c];
+        }
+        return EMB_OK;
+    })
+}
+
+int emb_sim_read_smoothed_current(EmbSim *sim, double *current) {
+    EMB_GUARD({
+        if (!sim || !current) return fail(EMB_E_INVALID, "null argument");
+        EMB_HIP(hipStreamSynchronize(sim->stream));
+        const DevScene &d = sim->r.ds;
+        const size_t cells = (size_t)d.cols * d.rows;
+        std::vector<float> v(2 * cells);
+        EMB_HIP(hipMemcpy(v.data(), d.cur_smooth, v.size() * 4, hipMemcpyDeviceToHost));
+        for (size_t c = 0; c < cells; c++) {
+            current[2 * c] = v[c];
+            current[2 * c + 1] = v[cells + c];
+        }
+        return EMB_OK;
+    })
+}
+
+int emb_sim_set_current(EmbSim *sim, const double *current) {
+    EMB_GUARD({
+        if (!sim || !current) return fail(EMB_E_INVALID, "null argument");
+        const DevScene &d = sim->r.ds;
+        const size_t cells = (size_t)d.cols * d.rows;
+        std::vector<float> v(2 * cells);
+        for (size_t c = 0; c < cells; c++) {
+            v[c] = (float)current[2 * c];
+            v[cells + c] = (float)current[2 * c + 1];
+        }
+        EMB_HIP(hipSetDevice(sim->dev->ordinal));
+        EMB_HIP(hipMemcpyAsync(d.cur_smooth, v.data(), v.size() * 4, hipMemcpyHostToDevice, sim->stream));
+        EMB_HIP(hipStreamSynchronize(sim->stream));
+        return EMB_OK;
     })
 }
 

@@ -96,11 +96,12 @@ void scene_blocks(const DevScene &d, int out[KC_COUNT]);
 // Kernel sequences over a batch (kernels.hip).
 hipError_t sort_kernels(const Batch &b, hipStream_t stream);
 hipError_t setup_kernels(const Batch &b, hipStream_t stream);   // sort, deposit, Gauss (no FDTD)
-constexpr int TICK_PHASES = 11;
+constexpr int TICK_PHASES = 12;
 extern const char *const TICK_PHASE_NAMES[];
 hipError_t tick_kernels(const Batch &b, int fdtd_substeps, hipStream_t stream, hipEvent_t *marks = nullptr);
 hipError_t deposit_kernel(const Batch &b, hipStream_t stream);
 hipError_t close_force_kernel(const Batch &b, float2 *out, hipStream_t stream);
+hipError_t finish_sync_kernel(const Batch &b, hipStream_t stream);
 hipError_t diagnostics_kernel(const DevScene *scene, double c2, double *out, int blocks, hipStream_t stream);
 
 // Write the per-tick AC phases and EMF targets of ticks [r.tick, r.tick + count)

@@ -240,6 +240,7 @@ int admit(EmbServer *sv) {
         }
         q->r.time = q->time0;
         EMB_HIP(setup_kernels(q->r.batch, sv->up_stream));
+        EMB_HIP(finish_sync_kernel(q->r.batch, sv->up_stream));
         sv->used_bytes += q->footprint;
         q->state = RUNNING;
         any = true;

@@ -37,6 +37,9 @@ SIGNATURES = {
     "emb_sim_write_yee": (C.c_int, [VP, D, D, D]),
     "emb_sim_read_potential": (C.c_int, [VP, D]),
     "emb_sim_read_current": (C.c_int, [VP, D]),
+    "emb_sim_read_display": (C.c_int, [VP, D, D]),
+    "emb_sim_read_smoothed_current": (C.c_int, [VP, D]),
+    "emb_sim_set_current": (C.c_int, [VP, D]),
     "emb_sim_read_charge_fixed": (C.c_int, [VP, I64]),
     "emb_sim_diagnostics": (C.c_int, [VP, D, D, D]),
     "emb_sim_kernel_times": (C.c_int, [VP, C.c_int, D, C.c_int]),
@@ -188,6 +191,24 @@ class GpuSim:
         out = np.zeros((self.cols, self.rows, 2))
         check(lib().emb_sim_read_current(self.h, _p(out)))
         return out
+
+    def read_display(self):
+        """GpuSim::read_display (sim.rs:831-840): averaged E (cols, rows, 2), B (cols, rows)."""
+        e, b = np.zeros((self.cols, self.rows, 2)), np.zeros((self.cols, self.rows))
+        check(lib().emb_sim_read_display(self.h, _p(e), _p(b)))
+        return e, b
+
+    def read_smoothed_current(self):
+        """The smoothed cell current (cols, rows, 2) that GpuSim::read_state returns."""
+        out = np.zeros((self.cols, self.rows, 2))
+        check(lib().emb_sim_read_smoothed_current(self.h, _p(out)))
+        return out
+
+    def set_current(self, current):
+        """GpuSim::set_current (sim.rs:531-539)."""
+        current = np.ascontiguousarray(current, dtype=np.float64)
+        assert current.shape == (self.cols, self.rows, 2)
+        check(lib().emb_sim_set_current(self.h, _p(current)))
 
     def read_charge_fixed(self):
         out = np.zeros((self.cols, self.rows), dtype=np.int64)

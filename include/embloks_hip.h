@@ -136,6 +136,18 @@ int emb_sim_write_yee(EmbSim *sim, const double *ex, const double *ey, const dou
 int emb_sim_read_potential(EmbSim *sim, double *potential);
 /* read_current (sim.rs:843-849): physics::current_density, (cols, rows, 2). */
 int emb_sim_read_current(EmbSim *sim, double *current);
+/* GpuSim::read_display (crates/embloks-gpu/src/sim.rs:831-840): the displayed E
+ * (cols x rows x 2) and B (cols x rows) at cell centres, exponential averages
+ * over display_smoothing_time updated at the end of every tick (finish_tick,
+ * fields.wgsl:270-281) and reset to the exact fields on create and on an upload
+ * with reset_display (finish_sync). */
+int emb_sim_read_display(EmbSim *sim, double *e, double *b);
+/* The smoothed (ammeter) cell current, cols x rows x 2, averaged over
+ * current_smoothing_time: what GpuSim::read_state / apply_state return
+ * (sim.rs:739-760). */
+int emb_sim_read_smoothed_current(EmbSim *sim, double *current);
+/* GpuSim::set_current (sim.rs:531-539): replace the smoothed current (from a save). */
+int emb_sim_set_current(EmbSim *sim, const double *current);
 /* The charge deposit as the int64 fixed-point grid the kernels hold: electron
  * charge in units of 2^-44 of one electron (cols * rows, unsmoothed, from the
  * electrons' current positions). No counterpart in the reference, which

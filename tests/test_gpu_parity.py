@@ -260,3 +260,43 @@ def test_crowded_cells_are_deterministic():
     b.run_ticks(5)
     for x, y in zip(a.read_yee() + a.read_state()[:2], b.read_yee() + b.read_state()[:2]):
         np.testing.assert_array_equal(x, y)
+
+
+@gpu
+def test_display_and_smoothed_current_follow_the_oracle():
+    # finish_tick (fields.wgsl:270-281): per tick, display += a_d (node fields -
+    # display) and smoothed current += a_c (cell current - smoothed current),
+    # with a_d = 1 - exp(-dt / display_smoothing_time) and
+    # a_c = min(dt / current_smoothing_time, 1) (sim.rs:256-261); the display
+    # starts at the exact fields (finish_sync) and the current average at zero.
+    import math
+    world = W.scene(5)
+    s = world.settings
+    a_d = 1.0 - math.exp(-s["dt"] / s["display_smoothing_time"])
+    a_c = min(s["dt"] / s["current_smoothing_time"], 1.0)
+    cpu = orc.Sim(world)
+    sim = gpusim()(world)
+    e_avg, b_avg = cpu.node_fields()
+    cur_avg = np.zeros_like(cpu.current())
+    e0, b0 = sim.read_display()
+    assert relative_error(e0, e_avg) < 1e-5
+    for _ in range(12):
+        cpu.step(1)
+        sim.run_ticks(1)
+        e, b = cpu.node_fields()
+        e_avg = e_avg + a_d * (e - e_avg)
+        b_avg = b_avg + a_d * (b - b_avg)
+        cur_avg = cur_avg + a_c * (cpu.current() - cur_avg)
+    e_gpu, b_gpu = sim.read_display()
+    err = {"e": relative_error(e_gpu, e_avg), "b": relative_error(b_gpu, b_avg),
+           "current": relative_error(sim.read_smoothed_current(), cur_avg)}
+    print("display and smoothed current after 12 ticks", err)
+    # coupled ticks: the reference GPU's 10-tick bars (tests/sim.rs:62-90)
+    assert err["e"] < 2e-2 and err["b"] < 5e-2 and err["current"] < 5e-2, err
+    # set_current replaces the average exactly (to FP32), and upload keeps it
+    want = np.random.default_rng(0).normal(size=cur_avg.shape)
+    sim.set_current(want)
+    assert np.max(np.abs(sim.read_smoothed_current() - want)) < 1e-6 * np.abs(want).max()
+    sim.upload(world, reset_display=False)
+    assert np.max(np.abs(sim.read_smoothed_current() - want)) < 1e-6 * np.abs(want).max()
+    assert np.array_equal(sim.read_display()[1], b_gpu)
