@@ -17,7 +17,7 @@ struct GemmArgs {
     int M = 0, N = 0, K = 0, lda = 0, ldb = 0, ldc = 0;
 };
 
-// One FP32 product with 64 x 64 or 128 x 128 output tiles.
+// One FP32 product with 64 x 64 (tile 64) or 128 x 64 (tile 128) output tiles.
 hipError_t launch_gemm(const GemmArgs &g, int tile, hipStream_t stream);
 
 // BF16x3: split an FP32 rows x cols matrix (pitch ld) into bf16 halves whose

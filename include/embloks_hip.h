@@ -234,7 +234,8 @@ int emb_poisson_create(EmbDevice *dev, int32_t nx, int32_t ny, EmbPoisson **out)
 int emb_poisson_set_rhs(EmbPoisson *p, const double *rhs);
 /* Run `repeats` solves back to back on the handle's stream and wait; *avg_ms
  * (if not NULL) gets the mean time per solve from HIP events around them.
- * tile: 64 or 128 (output tile side), 0 = by grid size. */
+ * tile: 64 (64 x 64 output tiles) or 128 (FP32: 128 x 64, BF16x3: 128 x 128),
+ * 0 = by grid size. */
 int emb_poisson_solve(EmbPoisson *p, int32_t precision, int32_t tile, int32_t repeats, double *avg_ms);
 /* The last solve's potential (nx * ny doubles). */
 int emb_poisson_read(EmbPoisson *p, double *phi);
