@@ -17,10 +17,14 @@
 #ifdef __HIPCC__
 #include <hip/hip_runtime.h>
 typedef float2 lerp2;
+typedef float2 force2;
 #else
 #define __host__
 #define __device__
 struct lerp2 {
+    float x, y;
+};
+struct force2 {
     float x, y;
 };
 #endif
@@ -141,6 +145,7 @@ struct DevScene {
     // --- cell list
     int32_t *scell;           // per sorted slot: flat cell index (c * rows + r)
     float *sfx, *sfy, *svx, *svy; // electrons in sorted order (snapshot of the substep start)
+    force2 *pforce;           // per sorted slot: close-range + ion force of the cell-mode electrons (k_push_tiled)
     int32_t *order;           // electron index per sorted slot
     int32_t *ecell, *erank;   // per electron: cell and rank within it
     int32_t *count;           // per cell (zero between sorts)
@@ -160,7 +165,7 @@ struct DevScene {
     const float *density;     // per cell (ion density)
     const float *ion_pot;     // per cell: close-range potential of all ions
     const float *g_table;     // PAIR_SAMPLES + 4: the reference's PAIR_SAMPLES + 2 samples, then two zeros
-    const lerp2 *g_lerp;      // PAIR_SAMPLES + 4: (g[i], g[i+1] - g[i]), zero past the reference's samples
+    const lerp2 *g_lerp;      // PAIR_SAMPLES + 4: (g[i], g[i+1] - g[i]) below the cutoff, then (0, 0)
     const float *square;      // square_n^2 * 3
     const lerp2 *square_f2;   // square_n^2: the force channels (1, 2) of the square table as pairs
     const float *ion_sum;     // ION_SUM^2 * 2: the square table's force channels summed over

@@ -536,13 +536,14 @@ __device__ __forceinline__ void ion_uniform(const DevScene &S, int home, float p
 // One pair term g(r^2) d, g lerped from the r^2-uniform table
 // (close_range.rs:359-372). The offset (ix, iy) is in 2^-24 cells; s2 =
 // g_scale 2^-48 turns its square into the table coordinate r^2 PAIR_SAMPLES /
-// cutoff^2. A pair at or beyond the cutoff reads the two zero samples past the
-// table's end, and the electron itself (d = 0) adds g(0) * 0: both contribute
-// exactly zero. The sums are in 2^-24 cells (k_pair_fx scales them back).
+// cutoff^2. A pair at or beyond the cutoff (d2 >= PAIR_SAMPLES) is clamped to
+// entries PAIR_SAMPLES or PAIR_SAMPLES + 1, which are both (0, 0), and the
+// electron itself (d = 0) adds g(0) * 0: both contribute exactly zero. The sums
+// are in 2^-24 cells (k_pair_fx scales them back).
 __device__ __forceinline__ void pair_term(const lerp2 *gt, float s2, int ix, int iy, float &sx, float &sy) {
     const float dx = (float)ix, dy = (float)iy;
     const float d2 = (dx * dx + dy * dy) * s2;
-    const float tt = d2 < (float)PAIR_SAMPLES ? d2 : (float)(PAIR_SAMPLES + 2);
+    const float tt = fminf(d2, (float)(PAIR_SAMPLES + 1));
     const lerp2 g = gt[(int)tt];
     const float l = g.x + g.y * __builtin_amdgcn_fractf(tt);
     sx += l * dx;
@@ -626,6 +627,21 @@ __device__ void push_core(const DevScene &S, const NB &nb, const lerp2 *gt, int 
     finish(S, id, hc, hr, px, py, vx, vy, info.damping, ax, ay, h);
 }
 
+// The substep of one electron whose close-range + ion force f is already known.
+__device__ __forceinline__ void kick_core(const DevScene &S, int slot, int k, float h, uint32_t t, float2 f) {
+    const int id = S.order[slot];
+    const int home = S.scell[slot];
+    const int hc = home / S.rows, hr = home - (home / S.rows) * S.rows;
+    const float px = S.sfx[slot], py = S.sfy[slot];
+    const float vx = S.svx[slot], vy = S.svy[slot];
+    const CellInfo info = S.info[home];
+    float ax, ay;
+    field_accel(S, hc, hr, px, py, vx, vy, info.flags, k, h, t, ax, ay);
+    ax += f.x * S.inv_mass;
+    ay += f.y * S.inv_mass;
+    finish(S, id, hc, hr, px, py, vx, vy, info.damping, ax, ay, h);
+}
+
 __device__ void push_one(const DevScene &S, int slot, int k, float h, uint32_t t) {
     push_core(S, GlobalNeighbours{S}, S.g_lerp, slot, k, h, t);
 }
@@ -676,82 +692,124 @@ struct GlobalRuns {
     }
 };
 
-// One work item of the cell mode: electrons [chunk I, chunk I + I) of home cell
-// (hc, hr), I = lanes_per(count), each spread over SL = 64 / I lanes that split
-// the neighbour runs; the partial sums meet in a fixed butterfly. FORCE: write
-// the close-range force to force_out[id] instead of pushing.
-template <bool FORCE, class RUNS>
+// Lanes per electron of a cell-mode work item over a cell of n electrons, E
+// electrons per lane: ceil(n / E) rounded up to a power of two, at most imax.
+__device__ __forceinline__ int item_lanes(int n, int E, int imax) { return min(lanes_per((n + E - 1) / E), imax); }
+
+// What a cell-mode work item does with an electron's close-range + ion force:
+// push the electron (ITEM_PUSH), write the force to force_out[id] (ITEM_FORCE),
+// or store it by sorted slot in S.pforce for k_push_sparse to push (ITEM_STORE).
+enum { ITEM_PUSH = 0, ITEM_FORCE = 1, ITEM_STORE = 2 };
+
+// One work item of the cell mode: the E I electrons [chunk E I, chunk E I + E I)
+// of home cell (hc, hr), I = item_lanes(n, E, imax); lane i + I sl holds
+// electrons chunk E I + e I + i (e < E) and walks slice sl of SL = 64 / I slices
+// of the neighbour runs, so each neighbour read from LDS serves E pair terms.
+// The partial sums meet in a fixed butterfly; lane sl = e then takes electron e
+// as MODE says.
+template <int MODE, int E, class RUNS>
 __device__ __forceinline__ void cell_item(const DevScene &S, const RUNS &runs, const lerp2 *gt, const float *dens_w,
                                           int hc, int hr, int e0, int n, int chunk, int wc0, int wr0, int W, int k,
                                           float h, uint32_t t, float2 *force_out, int imax = 64) {
+    static_assert(E == 1 || E == 2, "one or two electrons per lane");
     const int lane = threadIdx.x & 63;
     const int R = S.reach;
     const float s2 = S.pair_s2;
     const int home = hc * S.rows + hr;
-    const int I = min(lanes_per(n), imax), shift = __builtin_ctz(I), SL = 64 >> shift;
+    const int I = item_lanes(n, E, E == 2 ? min(imax, 32) : imax), shift = __builtin_ctz(I), SL = 64 >> shift;
     const int i = lane & (I - 1), sl = lane >> shift;
-    const int ei = chunk * I + i;
-    const bool live = ei < n;
-    const int slot = e0 + (live ? ei : n - 1);
-    const float px = S.sfx[slot], py = S.sfy[slot];
-    const int2 b = fixed_pos(hc - wc0, hr - wr0, px, py);
+    int slot[E];
+    bool live[E];
+    float px[E], py[E];
+    int2 b[E];
+    float sx[E], sy[E];
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+        const int ei = chunk * E * I + e * I + i;
+        live[e] = ei < n;
+        slot[e] = e0 + (live[e] ? ei : n - 1);
+        px[e] = S.sfx[slot[e]];
+        py[e] = S.sfy[slot[e]];
+        b[e] = fixed_pos(hc - wc0, hr - wr0, px[e], py[e]);
+        sx[e] = 0.0f;
+        sy[e] = 0.0f;
+    }
     const int c_lo = max(hc - R, 0), c_hi = min(hc + R, S.cols - 1);
     const int r_lo = max(hr - R, 0), r_hi = min(hr + R, S.rows - 1);
-    float sx = 0.0f, sy = 0.0f, fx = 0.0f, fy = 0.0f;
     for (int cc = c_lo; cc <= c_hi; cc++) {
         int jb, je;
         runs.run(cc, r_lo, r_hi, jb, je);
         jb = __builtin_amdgcn_readfirstlane(jb);
         je = __builtin_amdgcn_readfirstlane(je);
         int jj = jb;
-        for (; jj + 4 * SL <= je; jj += 4 * SL) {
+        for (; jj + 2 * SL <= je; jj += 2 * SL) {
             const int2 q0 = runs.at(jj + sl, cc), q1 = runs.at(jj + SL + sl, cc);
-            const int2 q2 = runs.at(jj + 2 * SL + sl, cc), q3 = runs.at(jj + 3 * SL + sl, cc);
-            pair_term(gt, s2, b.x - q0.x, b.y - q0.y, sx, sy);
-            pair_term(gt, s2, b.x - q1.x, b.y - q1.y, sx, sy);
-            pair_term(gt, s2, b.x - q2.x, b.y - q2.y, sx, sy);
-            pair_term(gt, s2, b.x - q3.x, b.y - q3.y, sx, sy);
+#pragma unroll
+            for (int e = 0; e < E; e++) {
+                pair_term(gt, s2, b[e].x - q0.x, b[e].y - q0.y, sx[e], sy[e]);
+                pair_term(gt, s2, b[e].x - q1.x, b[e].y - q1.y, sx[e], sy[e]);
+            }
         }
         for (; jj < je; jj += SL) {
             // past the run's end a lane contributes nothing
             const int2 q = runs.at(min(jj + sl, je - 1), cc);
             const bool in = jj + sl < je;
-            pair_term(gt, s2, in ? b.x - q.x : 0, in ? b.y - q.y : 0, sx, sy);
+#pragma unroll
+            for (int e = 0; e < E; e++)
+                pair_term(gt, s2, in ? b[e].x - q.x : 0, in ? b[e].y - q.y : 0, sx[e], sy[e]);
         }
     }
+    // From here on a lane works for electron e = sl % E (the ion terms) and, if
+    // sl < E, finishes it.
+    const int me = E == 2 ? (sl & 1) : 0;
+    const float mpx = E == 2 && me ? px[E - 1] : px[0], mpy = E == 2 && me ? py[E - 1] : py[0];
     const CellInfo info = S.info[home];
     const bool uniform = (info.flags & F_ION_UNIFORM) != 0u;
+    float fx = 0.0f, fy = 0.0f;
     if (!uniform) {
-        const IonWeights iw = ion_weights(S, px, py);
+        const IonWeights iw = ion_weights(S, mpx, mpy);
         const int nr = r_hi - r_lo + 1, cells = (c_hi - c_lo + 1) * nr;
-        for (int w = sl; w < cells; w += SL) {
+        for (int w = sl / E; w < cells; w += SL / E) {
             const int cc = c_lo + w / nr, rr = r_lo + w % nr;
             const float dens = dens_w[(cc - wc0) * W + (rr - wr0)];
             if (dens > 0.0f) ion_cell(S, iw, hc - cc, hr - rr, dens, fx, fy);
         }
     }
     for (int off = I; off < 64; off <<= 1) {
-        sx += __shfl_xor(sx, off, 64);
-        sy += __shfl_xor(sy, off, 64);
-        fx += __shfl_xor(fx, off, 64);
-        fy += __shfl_xor(fy, off, 64);
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+            sx[e] += __shfl_xor(sx[e], off, 64);
+            sy[e] += __shfl_xor(sy[e], off, 64);
+        }
+        if (off >= E * I) {   // the ion sums stay apart per electron
+            fx += __shfl_xor(fx, off, 64);
+            fy += __shfl_xor(fy, off, 64);
+        }
     }
-    if (sl == 0 && live) {
-        if (uniform) ion_uniform(S, home, px, py, fx, fy);
-        fx += S.k_pair_fx * sx;
-        fy += S.k_pair_fx * sy;
-        if (FORCE) {
-            force_out[S.order[slot]] = make_float2(fx, fy);
+    const bool mlive = E == 2 && me ? live[E - 1] : live[0];
+    if (sl < E && mlive) {
+        const int mslot = E == 2 && me ? slot[E - 1] : slot[0];
+        if (uniform) ion_uniform(S, home, mpx, mpy, fx, fy);
+        fx += S.k_pair_fx * (E == 2 && me ? sx[E - 1] : sx[0]);
+        fy += S.k_pair_fx * (E == 2 && me ? sy[E - 1] : sy[0]);
+        if (MODE == ITEM_FORCE) {
+            force_out[S.order[mslot]] = make_float2(fx, fy);
+        } else if (MODE == ITEM_STORE) {
+            S.pforce[mslot] = make_float2(fx, fy);
         } else {
-            const float vx = S.svx[slot], vy = S.svy[slot];
+            const float vx = S.svx[mslot], vy = S.svy[mslot];
             float ax, ay;
-            field_accel(S, hc, hr, px, py, vx, vy, info.flags, k, h, t, ax, ay);
+            field_accel(S, hc, hr, mpx, mpy, vx, vy, info.flags, k, h, t, ax, ay);
             ax += fx * S.inv_mass;
             ay += fy * S.inv_mass;
-            finish(S, S.order[slot], hc, hr, px, py, vx, vy, info.damping, ax, ay, h);
+            finish(S, S.order[mslot], hc, hr, mpx, mpy, vx, vy, info.damping, ax, ay, h);
         }
     }
 }
+
+// Electrons per lane in the cell mode: two share each neighbour read, except in
+// a cell of one.
+__device__ __forceinline__ int item_e(int n) { return n >= 2 ? 2 : 1; }
 
 // The close-range force alone on one electron (electron mode of FORCE).
 template <class NB>
@@ -801,14 +859,24 @@ __device__ void force_core(const DevScene &S, const NB &nb, const lerp2 *gt, int
 // neighbour list without divergence; the partial sums meet in a fixed
 // butterfly. The other tiles' electrons go through k_push_sparse. Every sum
 // runs in a fixed order, so a scene gives the same bits alone or batched.
+//
+// The tick's k_push_tiled runs 16 waves per block: two blocks fill all 32 wave
+// slots of a CU (8 per SIMD), which the pair loop's LDS and dependency latency
+// needs (the kernel is held to the 64 VGPRs that allows, with a few spills
+// outside the pair loop): 0.61 ms against 0.70 ms with 8 waves on the 512^2,
+// 16-carrier scene. The force pass k_force_tiled keeps 8 waves per block and
+// up to 128 VGPRs, which measured faster on the clustered 4096^2 scene (2.31
+// against 2.56 ms), where cells of up to 63 electrons make long items.
 constexpr int PUSH_WIN = PUSH_TILE + 2 * 4;   // window side for reach <= 4
 constexpr int PUSH_STAGE = 5120;   // positions (40 KB): with the 33 KB pair table, under 80 KB: two blocks per CU
 constexpr int PUSH_CELLS = PUSH_TILE * PUSH_TILE;
-constexpr int PUSH_THREADS = 512;             // 8 waves: two blocks fill a CU's 16 wave slots at <= 128 VGPRs
-constexpr int PUSH_WAVES = PUSH_THREADS / 64;
+constexpr int PUSH_THREADS = 1024;
+constexpr int FORCE_THREADS = 512;
 
-template <bool FORCE>
-__global__ __launch_bounds__(PUSH_THREADS) void k_push_tiled(Batch b, int k, float2 *force_out) {
+template <int MODE, int THREADS>
+__device__ __forceinline__ void push_tiled_body(Batch b, int k, float2 *force_out) {
+    constexpr bool FORCE = MODE == ITEM_FORCE;
+    constexpr int WAVES = THREADS / 64;
     __shared__ int2 lp[PUSH_STAGE];
     __shared__ lerp2 gt[PAIR_SAMPLES + 4];
     __shared__ int lb[PUSH_WIN * PUSH_WIN], le[PUSH_WIN * PUSH_WIN];
@@ -834,7 +902,7 @@ __global__ __launch_bounds__(PUSH_THREADS) void k_push_tiled(Batch b, int k, flo
         if (!FORCE && k == 0 && threadIdx.x == 0 && local == 0) S.ctr->substeps = sub;
         if (!S.close_on || !S.tile_dense[local]) continue;
         if (!table_loaded) {   // the pair table is shared by every scene
-            for (int q = threadIdx.x; q < PAIR_SAMPLES + 4; q += PUSH_THREADS) gt[q] = S.g_lerp[q];
+            for (int q = threadIdx.x; q < PAIR_SAMPLES + 4; q += THREADS) gt[q] = S.g_lerp[q];
             table_loaded = true;
         }
         const int tiles_r = (S.rows + PUSH_TILE - 1) / PUSH_TILE;
@@ -868,14 +936,14 @@ __global__ __launch_bounds__(PUSH_THREADS) void k_push_tiled(Batch b, int k, flo
             }
             cell_e0[lane] = first;
             cell_n[lane] = m;
-            const int I = lanes_per(m);
-            const int items = (m > 0 && m < HEAVY_CELL) ? (m + I - 1) / I : 0;
+            const int E = item_e(m), I = item_lanes(m, E, E == 2 ? 32 : 64);
+            const int items = (m > 0 && m < HEAVY_CELL) ? (m + E * I - 1) / (E * I) : 0;
             const int item_incl = wave_scan_incl(items);
             item_pref[lane] = item_incl - items;
             if (lane == 63) item_pref[PUSH_CELLS] = item_incl;
         }
         __syncthreads();
-        for (int q = threadIdx.x; q < W * W; q += PUSH_THREADS) {
+        for (int q = threadIdx.x; q < W * W; q += THREADS) {
             const int ci = q / W, ri = q - (q / W) * W;
             const int cc = wc0 + ci, rr = wr0 + ri;
             const bool in = cc >= 0 && cc < S.cols && rr >= 0 && rr < S.rows;
@@ -891,7 +959,7 @@ __global__ __launch_bounds__(PUSH_THREADS) void k_push_tiled(Batch b, int k, flo
             const int base = col_base[ca], staged_n = col_base[cb] - base;
             const bool staged = staged_n <= PUSH_STAGE;
             if (staged) {
-                for (int q = threadIdx.x; q < (cb - ca) * W; q += PUSH_THREADS) {
+                for (int q = threadIdx.x; q < (cb - ca) * W; q += THREADS) {
                     const int ci = ca + q / W, ri = q - (q / W) * W;
                     const int cc = wc0 + ci, rr = wr0 + ri;
                     const int w = ci * W + ri;
@@ -906,7 +974,7 @@ __global__ __launch_bounds__(PUSH_THREADS) void k_push_tiled(Batch b, int k, flo
                 }
                 // One pass over every staged electron (column found by search), so a
                 // sparse window costs one round of loads rather than one per column.
-                for (int j = threadIdx.x; j < staged_n; j += PUSH_THREADS) {
+                for (int j = threadIdx.x; j < staged_n; j += THREADS) {
                     int q = ca;
                     for (int step = PUSH_WIN / 2; step > 0; step >>= 1)
                         if (q + step < cb && col_base[q + step] - base <= j) q += step;
@@ -916,7 +984,7 @@ __global__ __launch_bounds__(PUSH_THREADS) void k_push_tiled(Batch b, int k, flo
                 }
             }
             __syncthreads();
-            for (int it = item_pref[la] + wave; it < item_pref[lz]; it += PUSH_WAVES) {
+            for (int it = item_pref[la] + wave; it < item_pref[lz]; it += WAVES) {
                 int lo = la, hi = lz;
                 while (hi - lo > 1) {
                     const int mid = (lo + hi) >> 1;
@@ -927,12 +995,19 @@ __global__ __launch_bounds__(PUSH_THREADS) void k_push_tiled(Batch b, int k, flo
                 const int hc = tc0 + lo / PUSH_TILE, hr = tr0 + lo % PUSH_TILE;
                 const int chunk = it - item_pref[lo];
                 const int e0 = __builtin_amdgcn_readfirstlane(cell_e0[lo]), n = __builtin_amdgcn_readfirstlane(cell_n[lo]);
-                if (staged)
-                    cell_item<FORCE>(S, LdsRuns{lp, lb, le, wc0, wr0, W}, gt, dens_w, hc, hr, e0, n, chunk, wc0, wr0, W,
-                                     k, h, t, force_out);
-                else
-                    cell_item<FORCE>(S, GlobalRuns{S, wc0, wr0}, gt, dens_w, hc, hr, e0, n, chunk, wc0, wr0, W, k, h, t,
-                                     force_out);
+                if (staged) {
+                    const LdsRuns runs{lp, lb, le, wc0, wr0, W};
+                    if (item_e(n) == 2)
+                        cell_item<MODE, 2>(S, runs, gt, dens_w, hc, hr, e0, n, chunk, wc0, wr0, W, k, h, t, force_out);
+                    else
+                        cell_item<MODE, 1>(S, runs, gt, dens_w, hc, hr, e0, n, chunk, wc0, wr0, W, k, h, t, force_out);
+                } else {
+                    const GlobalRuns runs{S, wc0, wr0};
+                    if (item_e(n) == 2)
+                        cell_item<MODE, 2>(S, runs, gt, dens_w, hc, hr, e0, n, chunk, wc0, wr0, W, k, h, t, force_out);
+                    else
+                        cell_item<MODE, 1>(S, runs, gt, dens_w, hc, hr, e0, n, chunk, wc0, wr0, W, k, h, t, force_out);
+                }
             }
             __syncthreads();
         }
@@ -963,7 +1038,13 @@ __global__ __launch_bounds__(256) void k_push_sparse(Batch b, int k, float2 *for
         if (slot >= S.n) continue;
         const int cell = S.scell[slot];
         const int cc = cell / S.rows, rr = cell - (cell / S.rows) * S.rows;
-        if (S.close_on && S.tile_dense[(cc / PUSH_TILE) * ((S.rows + PUSH_TILE - 1) / PUSH_TILE) + rr / PUSH_TILE]) continue;
+        if (S.close_on && S.tile_dense[(cc / PUSH_TILE) * ((S.rows + PUSH_TILE - 1) / PUSH_TILE) + rr / PUSH_TILE]) {
+            // k_push_tiled stored the close-range force of this electron unless its
+            // cell is k_push_heavy's; the field kick and move run here, one lane
+            // per electron.
+            if (!FORCE && S.cstart[cell + 1] - S.cstart[cell] < HEAVY_CELL) kick_core(S, slot, k, h, t, S.pforce[slot]);
+            continue;
+        }
         if (FORCE)
             force_core(S, GlobalNeighbours{S}, S.g_lerp, slot, force_out);
         else
@@ -976,14 +1057,17 @@ template __global__ void k_push_sparse<true>(Batch, int, float2 *);
 
 // Substep k (FORCE: the close-range force) for the electrons of crowded cells
 // (>= HEAVY_CELL, listed by the sort). A block takes whole cells from the list,
-// stages the cell's (2 reach + 1)^2 window and spreads the cell's 16-electron
-// chunks over its waves: a cluster's core is shared among many CUs instead of
-// weighing on its tile's one block.
+// stages the cell's (2 reach + 1)^2 window and spreads the cell's 32-electron
+// chunks (16 lanes per electron pair) over its waves: a cluster's core is shared
+// among many CUs instead of weighing on its tile's one block. Blocks of 8 waves:
+// a cell of a few hundred electrons has about that many chunks.
 constexpr int HEAVY_I = 16;
 constexpr int HEAVY_WIN = 2 * 4 + 1;   // window side for reach <= 4
+constexpr int HEAVY_THREADS = 512;
+constexpr int HEAVY_WAVES = HEAVY_THREADS / 64;
 
 template <bool FORCE>
-__global__ __launch_bounds__(PUSH_THREADS) void k_push_heavy(Batch b, int k, float2 *force_out) {
+__global__ __launch_bounds__(HEAVY_THREADS) void k_push_heavy(Batch b, int k, float2 *force_out) {
     __shared__ int2 lp[PUSH_STAGE];
     __shared__ lerp2 gt[PAIR_SAMPLES + 4];
     __shared__ int lb[HEAVY_WIN * HEAVY_WIN], le[HEAVY_WIN * HEAVY_WIN];
@@ -1007,7 +1091,7 @@ __global__ __launch_bounds__(PUSH_THREADS) void k_push_heavy(Batch b, int k, flo
         const int heavy_n = S.ctr->heavy_n;
         for (int hcell = block_in_scene(b, KC_HEAVY, s, blk); hcell < heavy_n; hcell += nblk) {
             if (!table_loaded) {   // the pair table is shared by every scene
-                for (int q = threadIdx.x; q < PAIR_SAMPLES + 4; q += PUSH_THREADS) gt[q] = S.g_lerp[q];
+                for (int q = threadIdx.x; q < PAIR_SAMPLES + 4; q += HEAVY_THREADS) gt[q] = S.g_lerp[q];
                 table_loaded = true;
             }
             const int cell = S.heavy_cells[hcell];
@@ -1031,7 +1115,7 @@ __global__ __launch_bounds__(PUSH_THREADS) void k_push_heavy(Batch b, int k, flo
             }
             __syncthreads();
             const bool staged = col_base[W] <= PUSH_STAGE;
-            for (int q = threadIdx.x; q < W * W; q += PUSH_THREADS) {
+            for (int q = threadIdx.x; q < W * W; q += HEAVY_THREADS) {
                 const int ci = q / W, ri = q - (q / W) * W;
                 const int cc = wc0 + ci, rr = wr0 + ri;
                 const bool in = cc >= 0 && cc < S.cols && rr >= 0 && rr < S.rows;
@@ -1044,7 +1128,7 @@ __global__ __launch_bounds__(PUSH_THREADS) void k_push_heavy(Batch b, int k, flo
             }
             if (staged) {
                 const int staged_n = col_base[W];
-                for (int j = threadIdx.x; j < staged_n; j += PUSH_THREADS) {
+                for (int j = threadIdx.x; j < staged_n; j += HEAVY_THREADS) {
                     int q = 0;
                     for (int step = 8; step > 0; step >>= 1)   // W <= HEAVY_WIN = 9
                         if (q + step < W && col_base[q + step] <= j) q += step;
@@ -1056,14 +1140,15 @@ __global__ __launch_bounds__(PUSH_THREADS) void k_push_heavy(Batch b, int k, flo
             __syncthreads();
             const int e0 = __builtin_amdgcn_readfirstlane(S.cstart[cell]);
             const int n = __builtin_amdgcn_readfirstlane(S.cstart[cell + 1]) - e0;
-            const int chunks = (n + HEAVY_I - 1) / HEAVY_I;
-            for (int ch = wave; ch < chunks; ch += PUSH_WAVES) {
+            const int chunks = (n + 2 * HEAVY_I - 1) / (2 * HEAVY_I);   // n >= HEAVY_CELL: I = HEAVY_I
+            for (int ch = wave; ch < chunks; ch += HEAVY_WAVES) {
+                constexpr int MODE = FORCE ? ITEM_FORCE : ITEM_PUSH;
                 if (staged)
-                    cell_item<FORCE>(S, LdsRuns{lp, lb, le, wc0, wr0, W}, gt, dens_w, hc, hr, e0, n, ch, wc0, wr0, W, k, h,
-                                     t, force_out, HEAVY_I);
+                    cell_item<MODE, 2>(S, LdsRuns{lp, lb, le, wc0, wr0, W}, gt, dens_w, hc, hr, e0, n, ch, wc0, wr0, W, k,
+                                       h, t, force_out, HEAVY_I);
                 else
-                    cell_item<FORCE>(S, GlobalRuns{S, wc0, wr0}, gt, dens_w, hc, hr, e0, n, ch, wc0, wr0, W, k, h, t,
-                                     force_out, HEAVY_I);
+                    cell_item<MODE, 2>(S, GlobalRuns{S, wc0, wr0}, gt, dens_w, hc, hr, e0, n, ch, wc0, wr0, W, k, h, t,
+                                       force_out, HEAVY_I);
             }
             __syncthreads();
         }
@@ -1073,8 +1158,13 @@ __global__ __launch_bounds__(PUSH_THREADS) void k_push_heavy(Batch b, int k, flo
 template __global__ void k_push_heavy<false>(Batch, int, float2 *);
 template __global__ void k_push_heavy<true>(Batch, int, float2 *);
 
-template __global__ void k_push_tiled<false>(Batch, int, float2 *);
-template __global__ void k_push_tiled<true>(Batch, int, float2 *);
+__global__ __launch_bounds__(PUSH_THREADS) __attribute__((amdgpu_waves_per_eu(8))) void k_push_tiled(Batch b, int k) {
+    push_tiled_body<ITEM_STORE, PUSH_THREADS>(b, k, nullptr);
+}
+
+__global__ __launch_bounds__(FORCE_THREADS) void k_force_tiled(Batch b, float2 *force_out) {
+    push_tiled_body<ITEM_FORCE, FORCE_THREADS>(b, 0, force_out);
+}
 
 // Substeps 1.. of a tick that needs several (fast electrons): one 1024-thread
 // block per scene re-sorts and pushes serially. Rare (electrons drift far below
@@ -1592,9 +1682,9 @@ hipError_t tick_kernels(const Batch &b, int fdtd_substeps, hipStream_t stream, h
     EMB_MARK(0);
     EMB_LAUNCH(k_trim, KC_STACKS, 1024, b);
     EMB_MARK(1);
-    EMB_LAUNCH(k_push_tiled<false>, KC_PUSH, PUSH_THREADS, b, 0, nullptr);
+    EMB_LAUNCH(k_push_tiled, KC_PUSH, PUSH_THREADS, b, 0);
     EMB_LAUNCH(k_push_sparse<false>, KC_ELECTRONS, 256, b, 0, nullptr);
-    EMB_LAUNCH(k_push_heavy<false>, KC_HEAVY, PUSH_THREADS, b, 0, nullptr);
+    EMB_LAUNCH(k_push_heavy<false>, KC_HEAVY, HEAVY_THREADS, b, 0, nullptr);
     EMB_MARK(2);
     if (b.total[KC_ELECTRONS] > 0) EMB_LAUNCH(k_rest, KC_SCENE, 1024, b);
     EMB_MARK(3);
@@ -1637,9 +1727,9 @@ hipError_t diagnostics_kernel(const DevScene *scene, double c2, double *out, int
 // The close-range force on every electron of the batch's scenes at their
 // current (sorted) positions, written by electron index.
 hipError_t close_force_kernel(const Batch &b, float2 *out, hipStream_t stream) {
-    EMB_LAUNCH(k_push_tiled<true>, KC_PUSH, PUSH_THREADS, b, 0, out);
+    EMB_LAUNCH(k_force_tiled, KC_PUSH, FORCE_THREADS, b, out);
     EMB_LAUNCH(k_push_sparse<true>, KC_ELECTRONS, 256, b, 0, out);
-    EMB_LAUNCH(k_push_heavy<true>, KC_HEAVY, PUSH_THREADS, b, 0, out);
+    EMB_LAUNCH(k_push_heavy<true>, KC_HEAVY, HEAVY_THREADS, b, 0, out);
     return hipGetLastError();
 }
 

@@ -72,6 +72,7 @@ void layout(Bump &a, const Counts &c, DevScene &d, DevScene **d_scene, int32_t *
     d.sfy = a.take<float>(n);
     d.svx = a.take<float>(n);
     d.svy = a.take<float>(n);
+    d.pforce = a.take<float2>(n);
     d.order = a.take<int32_t>(n);
     d.ecell = a.take<int32_t>(n);
     d.erank = a.take<int32_t>(n);
@@ -144,14 +145,16 @@ int shared_tables(EmbDevice *dev, const EmbSettings &s, int nx, int ny, const fl
             EMB_HIP(hipMalloc(&t.square, sq.size() * sizeof(float)));
             EMB_HIP(hipMemcpy(t.g, g.data(), g.size() * sizeof(float), hipMemcpyHostToDevice));
             EMB_HIP(hipMemcpy(t.square, sq.data(), sq.size() * sizeof(float), hipMemcpyHostToDevice));
-            // (g[i], g[i+1] - g[i]) for every sample the lerp can start at,
-            // including the two zero samples past the end.
+            // (g[i], g[i+1] - g[i]) for every sample the lerp can start at below
+            // the cutoff; entries PAIR_SAMPLES and up are (0, 0), where pairs at
+            // or beyond the cutoff are clamped.
             std::vector<double> gp(h->g.begin(), h->g.end());
             gp.push_back(0.0);
             gp.push_back(0.0);
             gp.push_back(0.0);
             std::vector<float2> gl(gp.size() - 1);
-            for (size_t i = 0; i + 1 < gp.size(); i++) gl[i] = make_float2((float)gp[i], (float)(gp[i + 1] - gp[i]));
+            for (size_t i = 0; i + 1 < gp.size(); i++)
+                gl[i] = i < (size_t)PAIR_SAMPLES ? make_float2((float)gp[i], (float)(gp[i + 1] - gp[i])) : make_float2(0.0f, 0.0f);
             EMB_HIP(hipMalloc(&t.g_lerp, gl.size() * sizeof(float2)));
             EMB_HIP(hipMemcpy(t.g_lerp, gl.data(), gl.size() * sizeof(float2), hipMemcpyHostToDevice));
             std::vector<float2> sf(h->square.size() / 3);
