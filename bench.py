@@ -139,6 +139,35 @@ def roofline(phases: dict, info: dict):
                  "avg_ms": round(phases[dom], 4)}
 
 
+# FP32 arithmetic of one candidate pair in the close-range sum (close_range.rs:359-372):
+# r^2 (3), the table coordinate (1), the lerp from (g, dg) (2), and the two force
+# accumulations (4). The fixed-point offset, index and clamp are not counted.
+FLOP_PER_PAIR = 10.0
+
+
+def candidate_pairs(pos: np.ndarray, cols: int, rows: int, cutoff: float) -> float:
+    """Pairs the close-range sum walks: each electron against every electron of
+    its (2 reach + 1)^2 cell window (itself included)."""
+    reach = int(math.ceil(cutoff))
+    cell = np.clip(np.floor(pos).astype(np.int64), 0, [cols - 1, rows - 1])
+    counts = np.zeros((cols, rows))
+    np.add.at(counts, (cell[:, 0], cell[:, 1]), 1.0)
+    pad = np.pad(counts, reach)
+    window = sum(pad[reach + dc: reach + dc + cols, reach + dr: reach + dr + rows]
+                 for dc in range(-reach, reach + 1) for dr in range(-reach, reach + 1))
+    return float((counts * window).sum())
+
+
+def push_compute(world, ms: float):
+    """The push is bound by its pair arithmetic, not by HBM: FP32 flops of the
+    candidate pairs per second against the FP32 peak."""
+    pairs = candidate_pairs(world.pos, world.cols, world.rows, world.settings["electron_cutoff"])
+    achieved = FLOP_PER_PAIR * pairs / (ms * 1e-3) / 1e12
+    return {"bound": "valu", "achieved": round(achieved, 2), "peak": FP32_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(achieved / FP32_MFMA_PEAK_TFLOPS, 4), "candidate_pairs": pairs,
+            "flop_per_candidate_pair": FLOP_PER_PAIR, "candidate_pairs_per_s": round(pairs / (ms * 1e-3), 1)}
+
+
 def run_poisson(n: int = 2048, repeats: int = 20):
     """The DST Poisson solve in isolation on an n x n node grid (200 Gaussian
     blobs, seed 3): FP32 and BF16x3 MFMA against FP64, with time per solve from
@@ -291,6 +320,8 @@ def main():
     nodes = info["nx"] * info["ny"]
     value = nodes * args.steps / elapsed
     dom, roof = roofline(phases, info)
+    if dom == "push":
+        roof["compute"] = push_compute(world, phases[dom])
     out = {
         "metric": "Yee cell-steps/s + p99 step latency, batched vs plain, <=1e-5 rel err vs FP64",
         "value": round(value, 1),
