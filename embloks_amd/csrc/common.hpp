@@ -54,6 +54,7 @@ constexpr int TICK_TABLE = 64;    // ticks of host-computed AC phases / EMF targ
 constexpr int PUSH_TILE = 8;      // cells per side of a push tile
 constexpr int CELL_MODE_MIN = 6;  // a push tile with a cell this crowded runs the cell mode
 constexpr int HEAVY_CELL = 64;    // a cell this crowded is pushed by k_push_heavy, spread over blocks
+constexpr int TRIM_BLOCKS = 8;    // blocks per battery stack in k_trim
 constexpr int FDTD_TI = 16, FDTD_TJ = 32;   // corners per FDTD tile (x, y)
 constexpr int ION_SUM = 33;       // samples per axis of the summed ion table
 constexpr int PAIR_SAMPLES = 4096;   // samples of the radial pair table (close_range.rs:30)
@@ -176,6 +177,8 @@ struct DevScene {
     const int32_t *stack_cells;
     float *gains;             // per stack
     int32_t *has_gain;        // per stack
+    float *trim_part;         // per stack and trim block: terminal potential sums (low, high)
+    int32_t *trim_done;       // per stack: trim blocks finished this tick (back to 0 by the last)
     const float *phase_tab;   // [TICK_TABLE][n_ac]: frac(f t) per AC cell and tick
     const float *target_tab;  // [TICK_TABLE][n_stacks]: EMF target per tick
     int32_t n_ac;
@@ -192,7 +195,7 @@ enum KernelClass {
     KC_FDTD,            // FDTD_TI x FDTD_TJ tiles of the (nx+1) x (ny+1) corner space
     KC_GEMM,            // 2 channels x 64 x 64 output tiles of nx x ny
     KC_NODES,           // 256 corners of (nx+1) x (ny+1) per block
-    KC_STACKS,          // one block per battery stack (at least one per scene)
+    KC_STACKS,          // TRIM_BLOCKS blocks per battery stack (at least one stack per scene)
     KC_SCENE,           // one block per scene
     KC_PUSH,            // PUSH_TILE x PUSH_TILE cell tiles
     KC_HEAVY,           // blocks sharing the scene's heavy cells (one per 16k electrons, at most 256)

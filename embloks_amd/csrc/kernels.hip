@@ -272,23 +272,27 @@ __device__ __forceinline__ float square_floor(const DevScene &S, int k, float u,
 }
 
 // ---------------------------------------------------------------------------------------
-// Battery trim (battery.rs:260-281; fields.wgsl:300-343). One 64-thread block per
+// Battery trim (battery.rs:260-281; fields.wgsl:300-343). TRIM_BLOCKS blocks per
 // stack. The potential a terminal reads is the node potential plus the
 // close-range potential of the ions and electrons near it (Simulation::potential,
 // simulation.rs:48-55; CloseRange::potential, close_range.rs:415-464).
 
-__global__ __launch_bounds__(1024) void k_trim(Batch b) {
-    __shared__ float part[2][16];
+constexpr int TRIM_THREADS = 512;
+
+__global__ __launch_bounds__(TRIM_THREADS) void k_trim(Batch b) {
+    __shared__ float part[2][TRIM_THREADS / 64];
+    __shared__ bool last;
     for (int blk = blockIdx.x; blk < b.total[KC_STACKS]; blk += gridDim.x) {
         const int s = scene_of(b, KC_STACKS, blk);
         const DevScene &S = b.scenes[s];
-        const int st = block_in_scene(b, KC_STACKS, s, blk);
+        const int local = block_in_scene(b, KC_STACKS, s, blk);
+        const int st = local / TRIM_BLOCKS, tb = local - st * TRIM_BLOCKS;
         const uint32_t t = S.ctr->tick;
-        if (st == 0 && threadIdx.x == 0) S.ctr->vmax_bits[(t + 1) & 1] = 0;
+        if (local == 0 && threadIdx.x == 0) S.ctr->vmax_bits[(t + 1) & 1] = 0;
         if (st >= S.n_stacks) continue;
         const StackInfo I = S.stacks[st];
         if (!I.both_wired || I.amplitude == 0.0f) {
-            if (threadIdx.x == 0) {
+            if (tb == 0 && threadIdx.x == 0) {
                 S.has_gain[st] = 0;
                 S.gains[st] = 1.0f;
             }
@@ -300,9 +304,11 @@ __global__ __launch_bounds__(1024) void k_trim(Batch b) {
         const int lists = I.low_n + I.high_n;
         float acc_lo = 0.0f, acc_hi = 0.0f;
         // Work item = (terminal cell, neighbour cell, one of TRIM_SPLIT lanes over
-        // that cell's electrons), so no thread walks a whole cell's electrons.
+        // that cell's electrons), so no thread walks a whole cell's electrons; the
+        // stack's TRIM_BLOCKS blocks take every TRIM_BLOCKS-th stride of items.
         constexpr int TRIM_SPLIT = 4;
-        for (int it = (int)threadIdx.x; it < lists * per * TRIM_SPLIT; it += 1024) {
+        for (int it = tb * TRIM_THREADS + (int)threadIdx.x; it < lists * per * TRIM_SPLIT;
+             it += TRIM_THREADS * TRIM_BLOCKS) {
             const int part = it % TRIM_SPLIT, item = it / TRIM_SPLIT;
             const int li = item / per, sub = item - li * per;
             const bool high = li >= I.low_n;
@@ -340,15 +346,31 @@ __global__ __launch_bounds__(1024) void k_trim(Batch b) {
             part[1][threadIdx.x >> 6] = acc_hi;
         }
         __syncthreads();
-        acc_lo = 0.0f;
-        acc_hi = 0.0f;
-        for (int w = 0; w < 16; w++) {
-            acc_lo += part[0][w];
-            acc_hi += part[1][w];
+        if (threadIdx.x == 0) {
+            float lo = 0.0f, hi = 0.0f;
+            for (int w = 0; w < TRIM_THREADS / 64; w++) {
+                lo += part[0][w];
+                hi += part[1][w];
+            }
+            float *mine = S.trim_part + 2 * ((size_t)st * TRIM_BLOCKS + tb);
+            __hip_atomic_store(mine, lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(mine + 1, hi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            // The block that finishes the stack's last part adds the parts in
+            // block order and updates the gain; the release/acquire pair on the
+            // counter makes every part's store visible to it.
+            const int done = __hip_atomic_fetch_add(S.trim_done + st, 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+            last = done == TRIM_BLOCKS - 1;
         }
         __syncthreads();
-        if (threadIdx.x == 0) {
-            const float v_low = acc_lo / (float)I.low_n, v_high = acc_hi / (float)I.high_n;
+        if (last && threadIdx.x == 0) {
+            float lo = 0.0f, hi = 0.0f;
+            const float *parts = S.trim_part + 2 * (size_t)st * TRIM_BLOCKS;
+            for (int q = 0; q < TRIM_BLOCKS; q++) {
+                lo += __hip_atomic_load(parts + 2 * q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                hi += __hip_atomic_load(parts + 2 * q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            __hip_atomic_store(S.trim_done + st, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const float v_low = lo / (float)I.low_n, v_high = hi / (float)I.high_n;
             const int cells = S.cols * S.rows;
             float isum = 0.0f;
             for (int k = 0; k < I.cur_n; k++) {
@@ -365,6 +387,7 @@ __global__ __launch_bounds__(1024) void k_trim(Batch b) {
             S.gains[st] = fminf(fmaxf(gain, S.gain_low), S.gain_high);
             S.has_gain[st] = 1;
         }
+        __syncthreads();
     }
 }
 
@@ -1680,7 +1703,7 @@ hipError_t finish_sync_kernel(const Batch &b, hipStream_t stream) {
     } while (0)
 hipError_t tick_kernels(const Batch &b, int fdtd_substeps, hipStream_t stream, hipEvent_t *marks) {
     EMB_MARK(0);
-    EMB_LAUNCH(k_trim, KC_STACKS, 1024, b);
+    EMB_LAUNCH(k_trim, KC_STACKS, TRIM_THREADS, b);
     EMB_MARK(1);
     EMB_LAUNCH(k_push_tiled, KC_PUSH, PUSH_THREADS, b, 0);
     EMB_LAUNCH(k_push_sparse<false>, KC_ELECTRONS, 256, b, 0, nullptr);

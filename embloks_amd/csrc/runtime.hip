@@ -94,6 +94,8 @@ void layout(Bump &a, const Counts &c, DevScene &d, DevScene **d_scene, int32_t *
     d.stack_cells = a.take<int32_t>(c.n_stack_cells);
     d.gains = a.take<float>(c.n_stacks);
     d.has_gain = a.take<int32_t>(c.n_stacks);
+    d.trim_part = a.take<float>(2 * (size_t)c.n_stacks * TRIM_BLOCKS);
+    d.trim_done = a.take<int32_t>(c.n_stacks);
     *tab = a.take<float>((size_t)TICK_TABLE * (c.n_ac + c.n_stacks));
     d.phase_tab = *tab;
     d.target_tab = *tab ? *tab + (size_t)TICK_TABLE * c.n_ac : nullptr;
@@ -206,7 +208,7 @@ void scene_blocks(const DevScene &d, int out[KC_COUNT]) {
     out[KC_FDTD] = cdiv(d.nx + 1, FDTD_TI) * cdiv(d.ny + 1, FDTD_TJ);
     out[KC_GEMM] = 2 * cdiv(d.nx, 64) * cdiv(d.ny, 64);
     out[KC_NODES] = cdiv((long long)(d.nx + 1) * (d.ny + 1), 256);
-    out[KC_STACKS] = std::max(1, d.n_stacks);
+    out[KC_STACKS] = std::max(1, d.n_stacks) * TRIM_BLOCKS;
     out[KC_SCENE] = 1;
     out[KC_PUSH] = cdiv(d.cols, PUSH_TILE) * cdiv(d.rows, PUSH_TILE);
     out[KC_HEAVY] = (int)std::min<long long>(256, std::max<long long>(1, d.n / 16384));
