@@ -138,7 +138,9 @@ struct DevScene {
     float *rho, *jx, *jy;     // smoothed sources on nodes
     float *w1, *w2, *phi;     // Poisson work: 2 channels each, channel stride nodes
     const float *damp_ex_x, *damp_ex_y, *damp_ey_x, *damp_ey_y, *damp_bz_x, *damp_bz_y;
-    const float *sine_x, *sine_y;     // DST-I matrices (nx^2, ny^2)
+    // halves of the DST-I matrices (sine_halves): even rows of Sx (ceil(nx/2)^2),
+    // odd rows of Sx (floor(nx/2)^2), the even rows of Sy transposed, odd rows of Sy
+    const float *sx_e, *sx_o, *sy_et, *sy_o;
     const float *inv_lap;             // norm / (lambda_x + lambda_y), (nx, ny) (maxwell.rs:60-67)
     // --- electrons (SoA, upload order): cell + fraction, velocity
     int32_t *cxy;

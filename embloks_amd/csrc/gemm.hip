@@ -10,9 +10,18 @@
 // The tick solves two right-hand sides (the charge's potential and the
 // divergence correction, maxwell.rs:200-204) side by side.
 //
+// S[k][N-1-n] = (-1)^k S[k][n], so S x splits into two half-size products:
+// rows 2i of the result are se . p and rows 2i+1 are so . q, with the folded
+// input p_n = x_n + x_{N-1-n}, q_n = x_n - x_{N-1-n} (n < N/2; p keeps the
+// middle x_n alone when N is odd) and se, so the even and odd rows of S over
+// the first half of its columns (sine_halves). The FP32 products fold while
+// loading their data operand and write every other row or column: half the
+// flops and half the K slices of the whole product.
+//
 // Two tile kernels compute C = A . B (row-major, optional elementwise scale):
-//   * FP32: v_mfma_f32_16x16x4f32, FP32 in and out; what the tick runs. Four
-//     K slices are in flight per block, which is what the tick's small K needs.
+//   * FP32: v_mfma_f32_16x16x4f32, FP32 in and out; what the tick runs, folded.
+//     Four K slices are in flight per block, which is what the tick's small K
+//     needs.
 //   * BF16x3: each operand split as x = hi + lo (hi = bf16(x), lo = bf16(x - hi)),
 //     A.B ~ lo.hi + hi.lo + hi.hi with v_mfma_f32_16x16x32_bf16 and FP32
 //     accumulation (the lo.lo term is dropped). Offered for the isolated solve;
@@ -25,6 +34,7 @@
 #include "device.hpp"
 #include "gemm.hpp"
 
+#include <algorithm>
 #include <type_traits>
 #include <utility>
 
@@ -70,27 +80,32 @@ __device__ __forceinline__ float4 load4(const float *p, int avail, bool vec) {
 // of B past N are clamped to the last valid ones, since they only reach output
 // rows and columns that are never stored.
 
-template <int BM, int BN, int GM = 2, int GN = 2, int GK = 16, int PF = 1, bool FAST = false>
+template <int BM, int BN, int GM = 2, int GN = 2, int GK = 16, int PF = 1, bool FAST = false, int FOLD = 0>
 __device__ __forceinline__ void tile_f32(const GemmArgs &g, int m0, int n0, float *lds) {
     constexpr int NT = 64 * GM * GN;
     constexpr int LA = BM + 4, LB = BN + 4, KV = GK / 4;
     constexpr int WM = BM / GM, WN = BN / GN, TM = WM / 16, TN = WN / 16;
     constexpr int AV = BM * GK / 4 / NT, BV = BN * GK / 4 / NT;
     static_assert(AV >= 1 && BV >= 1 && AV * NT * 4 == BM * GK && BV * NT * 4 == BN * GK, "tile and threads mismatch");
+    // FAST + FOLD: the mirror of the folded operand is loaded into slots of its
+    // own and added when the slice is stored (adding at load time would wait).
+    constexpr int A2 = (FAST && FOLD == 2) ? AV : 1, B2 = (FAST && FOLD == 1) ? BV : 1;
     float *As = lds, *Bs = lds + 2 * GK * LA;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = (wave / GN) * WM, wn = (wave % GN) * WN;
     const int l16 = lane & 15, kq = lane >> 4;
     const bool avec = (g.lda & 3) == 0 && (reinterpret_cast<uintptr_t>(g.A) & 15) == 0;
     const bool bvec = (g.ldb & 3) == 0 && (reinterpret_cast<uintptr_t>(g.B) & 15) == 0;
+    const float sgn = g.odd ? -1.0f : 1.0f;
+    const int fn = g.fold_n;
     floatx4 acc[TM][TN];
 #pragma unroll
     for (int x = 0; x < TM; x++)
 #pragma unroll
         for (int y = 0; y < TN; y++) acc[x][y] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
     // Per-thread operand addresses at slice 0 (FAST: clamped, so every load is valid).
-    const float *pa[AV];
-    const float *pb[BV];
+    const float *pa[AV], *pa2[A2];
+    const float *pb[BV], *pb2[B2];
     int ra_row[AV], ra_k[AV], rb_k[BV], rb_col[BV];
 #pragma unroll
     for (int v = 0; v < AV; v++) {
@@ -99,6 +114,7 @@ __device__ __forceinline__ void tile_f32(const GemmArgs &g, int m0, int n0, floa
         ra_k[v] = (idx % KV) * 4;
         const int gr = FAST ? min(m0 + ra_row[v], g.M - 1) : m0 + ra_row[v];
         pa[v] = g.A + (size_t)gr * g.lda + ra_k[v];
+        if (FAST && FOLD == 2) pa2[v] = g.A + (size_t)gr * g.lda + (fn - 4 - ra_k[v]);   // moves left as k grows
     }
 #pragma unroll
     for (int v = 0; v < BV; v++) {
@@ -107,45 +123,78 @@ __device__ __forceinline__ void tile_f32(const GemmArgs &g, int m0, int n0, floa
         rb_col[v] = (idx % (BN / 4)) * 4;
         const int gc = FAST ? min(n0 + rb_col[v], g.N - 4) : n0 + rb_col[v];
         pb[v] = g.B + (size_t)rb_k[v] * g.ldb + gc;
+        if (FAST && FOLD == 1) pb2[v] = g.B + (size_t)(fn - 1 - rb_k[v]) * g.ldb + gc;   // moves up as k grows
     }
     // Native vectors, not float4 structs: a struct copy between address spaces
     // goes through private memory.
-    floatx4 ra[PF][AV], rb[PF][BV];
+    floatx4 ra[PF][AV], rb[PF][BV], ra2[PF][A2], rb2[PF][B2];
     auto vec4 = [](float4 f) { return floatx4{f.x, f.y, f.z, f.w}; };
+    // Element e of a folded A row at column c (< K): A[r][c] + s A[r][fn - 1 - c].
+    auto fold_a = [&](const float *row, int c) {
+        const int c2 = fn - 1 - c;
+        return row[c] + (c2 == c ? 0.0f : sgn * row[c2]);
+    };
     auto load = [&](auto slot, int k0) {
 #pragma unroll
         for (int v = 0; v < AV; v++) {
             if (FAST) {
                 ra[slot][v] = *reinterpret_cast<const floatx4 *>(pa[v] + k0);
+                if (FOLD == 2) ra2[slot][v] = *reinterpret_cast<const floatx4 *>(pa2[v] - k0);
             } else {
                 const int gr = m0 + ra_row[v], gk = k0 + ra_k[v];
-                ra[slot][v] = gr < g.M ? vec4(load4(pa[v] + k0, g.K - gk, avec)) : floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+                if (FOLD == 2) {
+                    floatx4 r = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+                    if (gr < g.M) {
+                        const float *row = g.A + (size_t)gr * g.lda;
+#pragma unroll
+                        for (int e = 0; e < 4; e++)
+                            if (gk + e < g.K) r[e] = fold_a(row, gk + e);
+                    }
+                    ra[slot][v] = r;
+                } else {
+                    ra[slot][v] = gr < g.M ? vec4(load4(pa[v] + k0, g.K - gk, avec)) : floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+                }
             }
         }
 #pragma unroll
         for (int v = 0; v < BV; v++) {
             if (FAST) {
                 rb[slot][v] = *reinterpret_cast<const floatx4 *>(pb[v] + (size_t)k0 * g.ldb);
+                if (FOLD == 1) rb2[slot][v] = *reinterpret_cast<const floatx4 *>(pb2[v] - (size_t)k0 * g.ldb);
             } else {
                 const int gk = k0 + rb_k[v], gc = n0 + rb_col[v];
-                rb[slot][v] = gk < g.K ? vec4(load4(pb[v] + (size_t)k0 * g.ldb, g.N - gc, bvec))
-                                       : floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+                floatx4 r = gk < g.K ? vec4(load4(pb[v] + (size_t)k0 * g.ldb, g.N - gc, bvec))
+                                     : floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+                if (FOLD == 1 && gk < g.K && fn - 1 - gk != gk)
+                    r += sgn * vec4(load4(g.B + (size_t)(fn - 1 - gk) * g.ldb + gc, g.N - gc, bvec));
+                rb[slot][v] = r;
             }
         }
     };
-    auto store = [&](auto slot, int buf) {
+    auto store = [&](auto slot, int buf, int k0) {
         float *a = As + buf * GK * LA, *b = Bs + buf * GK * LB;
 #pragma unroll
         for (int v = 0; v < AV; v++) {
             const int row = ra_row[v], kk = ra_k[v];
-            a[(kk + 0) * LA + row] = ra[slot][v][0];
-            a[(kk + 1) * LA + row] = ra[slot][v][1];
-            a[(kk + 2) * LA + row] = ra[slot][v][2];
-            a[(kk + 3) * LA + row] = ra[slot][v][3];
+            floatx4 x = ra[slot][v];
+            if (FAST && FOLD == 2) {   // the mirror float4 runs backwards
+                const floatx4 y = ra2[slot][v];
+                x = x + sgn * floatx4{y[3], y[2], y[1], y[0]};
+            }
+            a[(kk + 0) * LA + row] = x[0];
+            a[(kk + 1) * LA + row] = x[1];
+            a[(kk + 2) * LA + row] = x[2];
+            a[(kk + 3) * LA + row] = x[3];
         }
 #pragma unroll
-        for (int v = 0; v < BV; v++)
-            *reinterpret_cast<floatx4 *>(b + rb_k[v] * LB + rb_col[v]) = rb[slot][v];
+        for (int v = 0; v < BV; v++) {
+            floatx4 x = rb[slot][v];
+            if (FAST && FOLD == 1) {
+                const int r = k0 + rb_k[v];
+                x = x + ((fn - 1 - r == r) ? 0.0f : sgn) * rb2[slot][v];
+            }
+            *reinterpret_cast<floatx4 *>(b + rb_k[v] * LB + rb_col[v]) = x;
+        }
     };
     auto multiply = [&](int buf) {
         const float *a = As + buf * GK * LA, *b = Bs + buf * GK * LB;
@@ -167,7 +216,7 @@ __device__ __forceinline__ void tile_f32(const GemmArgs &g, int m0, int n0, floa
     // unconditionally in the unrolled body (a repeated slice is never multiplied).
     const int steps = (g.K + GK - 1) / GK, last = (steps - 1) * GK;
     load(std::integral_constant<int, 0>{}, 0);
-    store(std::integral_constant<int, 0>{}, 0);
+    store(std::integral_constant<int, 0>{}, 0, 0);
     static_for<PF>([&](auto u) { load(u, min((1 + u) * GK, last)); });
     __syncthreads();
     int s0 = 0;
@@ -175,7 +224,7 @@ __device__ __forceinline__ void tile_f32(const GemmArgs &g, int m0, int n0, floa
         static_for<PF>([&](auto u) {
             const int st = s0 + u;
             multiply(st & 1);
-            store(u, (st + 1) & 1);
+            store(u, (st + 1) & 1, min((st + 1) * GK, last));
             load(u, min((st + 1 + PF) * GK, last));
             __syncthreads();
         });
@@ -185,10 +234,13 @@ __device__ __forceinline__ void tile_f32(const GemmArgs &g, int m0, int n0, floa
     static_for<PF>([&](auto u) {
         const int st = s0 + u;
         if (st < steps) multiply(st & 1);
-        store(u, (st + 1) & 1);
+        store(u, (st + 1) & 1, min((st + 1) * GK, last));
         __syncthreads();
     });
-    // D[row][col]: lane holds rows 4 * (lane / 16) + i of column lane % 16.
+    // D[row][col]: lane holds rows 4 * (lane / 16) + i of column lane % 16; a
+    // folded half writes every other row (FOLD 1) or column (FOLD 2) of C.
+    auto crow = [&](int row) { return FOLD == 1 ? 2 * row + g.odd : row; };
+    auto ccol = [&](int col) { return FOLD == 2 ? 2 * col + g.odd : col; };
     // Scale factors are read at clamped indices, all before the first is used.
     if (g.scale) {
 #pragma unroll
@@ -199,7 +251,7 @@ __device__ __forceinline__ void tile_f32(const GemmArgs &g, int m0, int n0, floa
                 for (int i = 0; i < 4; i++) {
                     const int row = min(m0 + wm + x * 16 + kq * 4 + i, g.M - 1);
                     const int col = min(n0 + wn + y * 16 + l16, g.N - 1);
-                    acc[x][y][i] *= g.scale[(size_t)row * g.ldc + col];
+                    acc[x][y][i] *= g.scale[(size_t)crow(row) * g.ldc + ccol(col)];
                 }
     }
 #pragma unroll
@@ -209,20 +261,43 @@ __device__ __forceinline__ void tile_f32(const GemmArgs &g, int m0, int n0, floa
 #pragma unroll
             for (int i = 0; i < 4; i++) {
                 const int row = m0 + wm + x * 16 + kq * 4 + i, col = n0 + wn + y * 16 + l16;
-                if (row < g.M && col < g.N) g.C[(size_t)row * g.ldc + col] = acc[x][y][i];
+                if (row < g.M && col < g.N) g.C[(size_t)crow(row) * g.ldc + ccol(col)] = acc[x][y][i];
             }
 }
 
 // Whether a product can take tile_f32's FAST path for slices of GK.
+// A column fold (2) also needs the mirror float4s aligned: fold_n a multiple of 4.
 __host__ __device__ inline bool gemm_fast(const GemmArgs &g, int gk) {
     return g.K % gk == 0 && g.N % 4 == 0 && g.N >= 4 && g.lda % 4 == 0 && g.ldb % 4 == 0 &&
-           (reinterpret_cast<uintptr_t>(g.A) & 15) == 0 && (reinterpret_cast<uintptr_t>(g.B) & 15) == 0;
+           (reinterpret_cast<uintptr_t>(g.A) & 15) == 0 && (reinterpret_cast<uintptr_t>(g.B) & 15) == 0 &&
+           (g.fold != 2 || g.fold_n % 4 == 0);
 }
 
 // ---------------------------------------------------------------------------------------
-// The tick's solve over a batch of scenes: blocks walk (scene, channel, tile).
+// The tick's solve over a batch of scenes: blocks walk (scene, channel, half,
+// tile). Each product runs as its two folded halves (sine_halves): half the
+// flops and half the K slices of the whole product.
 
 constexpr int TICK_BM = 64, TICK_BN = 64;
+
+// Output tiles of the even and odd halves of one stage's product.
+__host__ __device__ inline void dst_half_tiles(bool left, int nx, int ny, int &te, int &to) {
+    auto cdiv = [](int a, int b) { return (a + b - 1) / b; };
+    if (left) {   // rows of Sx: ceil(nx/2) even, floor(nx/2) odd
+        te = cdiv((nx + 1) / 2, TICK_BM) * cdiv(ny, TICK_BN);
+        to = cdiv(nx / 2, TICK_BM) * cdiv(ny, TICK_BN);
+    } else {      // columns of Sy
+        te = cdiv(nx, TICK_BM) * cdiv((ny + 1) / 2, TICK_BN);
+        to = cdiv(nx, TICK_BM) * cdiv(ny / 2, TICK_BN);
+    }
+}
+
+int dst_gemm_blocks(int nx, int ny) {
+    int le, lo, re, ro;
+    dst_half_tiles(true, nx, ny, le, lo);
+    dst_half_tiles(false, nx, ny, re, ro);
+    return 2 * std::max(le + lo, re + ro);
+}
 
 template <int STAGE>
 __global__ __launch_bounds__(256) void k_dst_gemm(Batch b) {
@@ -232,27 +307,50 @@ __global__ __launch_bounds__(256) void k_dst_gemm(Batch b) {
         const int s = scene_of(b, KC_GEMM, blk);
         const DevScene &S = b.scenes[s];
         const int nx = S.nx, ny = S.ny;
+        int te, to;
+        dst_half_tiles(LEFT, nx, ny, te, to);
         const int local = block_in_scene(b, KC_GEMM, s, blk);
-        const int tm_n = (nx + TICK_BM - 1) / TICK_BM, tn_n = (ny + TICK_BN - 1) / TICK_BN;
-        const int ch = local / (tm_n * tn_n);
-        const int rem = local - ch * tm_n * tn_n;
+        const int ch = local / (te + to);
+        if (ch >= 2) continue;   // the scene's block count covers the larger stage
+        const int rem0 = local - ch * (te + to);
+        const int odd = rem0 >= te ? 1 : 0;
+        const int rem = rem0 - odd * te;
         const size_t nodes = (size_t)nx * ny;
         const float *in = (LEFT ? S.w1 : S.w2) + ch * nodes;
         float *out = (STAGE == 3 ? S.phi : (STAGE == 1 ? S.w1 : S.w2)) + ch * nodes;
         GemmArgs g;
-        g.A = LEFT ? S.sine_x : in;
-        g.lda = LEFT ? nx : ny;
-        g.B = LEFT ? in : S.sine_y;
-        g.ldb = ny;
         g.C = out;
         g.ldc = ny;
         g.scale = STAGE == 1 ? S.inv_lap : nullptr;
-        g.M = nx;
-        g.N = ny;
-        g.K = LEFT ? nx : ny;
-        const int m0 = (rem / tn_n) * TICK_BM, n0 = (rem % tn_n) * TICK_BN;
-        if (gemm_fast(g, 16)) tile_f32<TICK_BM, TICK_BN, 2, 2, 16, 4, true>(g, m0, n0, lds);
-        else tile_f32<TICK_BM, TICK_BN, 2, 2, 16, 4, false>(g, m0, n0, lds);
+        g.odd = odd;
+        if (LEFT) {   // out = Sx . in: rows 2 i + odd from the half of Sx and the folded rows of in
+            const int mh = odd ? nx / 2 : (nx + 1) / 2;
+            g.A = odd ? S.sx_o : S.sx_e;
+            g.lda = mh;
+            g.B = in;
+            g.ldb = ny;
+            g.M = mh;
+            g.N = ny;
+            g.K = mh;
+            g.fold = 1;
+            g.fold_n = nx;
+        } else {      // out = in . Sy: columns 2 j + odd from the folded columns of in
+            const int mh = odd ? ny / 2 : (ny + 1) / 2;
+            g.A = in;
+            g.lda = ny;
+            g.B = odd ? S.sy_o : S.sy_et;
+            g.ldb = mh;
+            g.M = nx;
+            g.N = mh;
+            g.K = mh;
+            g.fold = 2;
+            g.fold_n = ny;
+        }
+        const int tn = (g.N + TICK_BN - 1) / TICK_BN;
+        const int m0 = (rem / tn) * TICK_BM, n0 = (rem % tn) * TICK_BN;
+        constexpr int F = LEFT ? 1 : 2;
+        if (gemm_fast(g, 16)) tile_f32<TICK_BM, TICK_BN, 2, 2, 16, 4, true, F>(g, m0, n0, lds);
+        else tile_f32<TICK_BM, TICK_BN, 2, 2, 16, 4, false, F>(g, m0, n0, lds);
         __syncthreads();
     }
 }
@@ -277,28 +375,45 @@ hipError_t launch_dst_gemms(const Batch &b, hipStream_t stream) {
 // ---------------------------------------------------------------------------------------
 // One product on plain pointers (the isolated solve).
 
-template <int BM, int BN, int PF, bool FAST>
-__global__ __launch_bounds__(256) void k_gemm_f32(GemmArgs g) {
+template <int BM, int BN, int PF, bool FAST, int FOLD>
+__global__ __launch_bounds__(256) void k_gemm_f32(GemmArgs g0, GemmArgs g1) {
     __shared__ __align__(16) float lds[2 * 16 * (BM + 4 + BN + 4)];
-    const int tn = (g.N + BN - 1) / BN, tiles = ((g.M + BM - 1) / BM) * tn;
-    for (int blk = blockIdx.x; blk < tiles; blk += gridDim.x) {
-        tile_f32<BM, BN, 2, 2, 16, PF, FAST>(g, (blk / tn) * BM, (blk % tn) * BN, lds);
+    const int tn0 = (g0.N + BN - 1) / BN, tiles0 = ((g0.M + BM - 1) / BM) * tn0;
+    const int tn1 = (g1.N + BN - 1) / BN, tiles1 = ((g1.M + BM - 1) / BM) * tn1;
+    for (int blk = blockIdx.x; blk < tiles0 + tiles1; blk += gridDim.x) {
+        const bool second = blk >= tiles0;
+        const GemmArgs &g = second ? g1 : g0;
+        const int t = second ? blk - tiles0 : blk, tn = second ? tn1 : tn0;
+        tile_f32<BM, BN, 2, 2, 16, PF, FAST, FOLD>(g, (t / tn) * BM, (t % tn) * BN, lds);
         __syncthreads();
     }
 }
 
-template <int BM, int BN>
-static hipError_t launch_gemm_tile(const GemmArgs &g, hipStream_t stream) {
-    const long long tiles = (long long)((g.M + BM - 1) / BM) * ((g.N + BN - 1) / BN);
-    const int grid = (int)(tiles < 8192 ? tiles : 8192);
+// Products g0 and g1 (g1.M = 0: none) in one launch, all FAST or none.
+template <int BM, int BN, int FOLD>
+static hipError_t launch_gemm_tiles(const GemmArgs &g0, const GemmArgs &g1, hipStream_t stream) {
+    auto tiles = [](const GemmArgs &g) {
+        return g.M > 0 && g.N > 0 ? (long long)((g.M + BM - 1) / BM) * ((g.N + BN - 1) / BN) : 0LL;
+    };
+    const long long total = tiles(g0) + tiles(g1);
+    const int grid = (int)(total < 8192 ? total : 8192);
     if (grid == 0) return hipSuccess;
-    if (gemm_fast(g, 16)) k_gemm_f32<BM, BN, 4, true><<<grid, 256, 0, stream>>>(g);
-    else k_gemm_f32<BM, BN, 4, false><<<grid, 256, 0, stream>>>(g);
+    const bool fast = gemm_fast(g0, 16) && (g1.M == 0 || gemm_fast(g1, 16));
+    if (fast) k_gemm_f32<BM, BN, 4, true, FOLD><<<grid, 256, 0, stream>>>(g0, g1);
+    else k_gemm_f32<BM, BN, 4, false, FOLD><<<grid, 256, 0, stream>>>(g0, g1);
     return hipGetLastError();
 }
 
 hipError_t launch_gemm(const GemmArgs &g, int tile, hipStream_t stream) {
-    return tile == 128 ? launch_gemm_tile<128, 64>(g, stream) : launch_gemm_tile<64, 64>(g, stream);
+    GemmArgs none;
+    return tile == 128 ? launch_gemm_tiles<128, 64, 0>(g, none, stream) : launch_gemm_tiles<64, 64, 0>(g, none, stream);
+}
+
+hipError_t launch_gemm_fold(const GemmArgs &even, const GemmArgs &odd, int tile, hipStream_t stream) {
+    if (even.fold != odd.fold || (even.fold != 1 && even.fold != 2)) return hipErrorInvalidValue;
+    if (even.fold == 1)
+        return tile == 128 ? launch_gemm_tiles<128, 64, 1>(even, odd, stream) : launch_gemm_tiles<64, 64, 1>(even, odd, stream);
+    return tile == 128 ? launch_gemm_tiles<128, 64, 2>(even, odd, stream) : launch_gemm_tiles<64, 64, 2>(even, odd, stream);
 }
 
 // ---------------------------------------------------------------------------------------

@@ -10,15 +10,27 @@ namespace emb {
 // C = A . B, all row-major FP32: A is M x K (row pitch lda), B is K x N (ldb),
 // C is M x N (ldc). When `scale` is set, C is multiplied elementwise by it
 // (an M x N array with pitch ldc).
+//
+// Folded (one half of a DST-I product through the matrix's symmetry, see
+// sine_halves): with fold = 1, row r of B is read as B[r] + s B[fold_n - 1 - r]
+// (B[r] alone where the two are the same row), and result row i goes to row
+// 2 i + odd of C; with fold = 2, column c of A is read as A[.][c] + s
+// A[.][fold_n - 1 - c] and result column j goes to column 2 j + odd. s = +1 for
+// the even half (odd = 0), -1 for the odd half; `scale` is read at C's position.
 struct GemmArgs {
     const float *A = nullptr, *B = nullptr;
     float *C = nullptr;
     const float *scale = nullptr;
     int M = 0, N = 0, K = 0, lda = 0, ldb = 0, ldc = 0;
+    int fold = 0, fold_n = 0, odd = 0;
 };
 
 // One FP32 product with 64 x 64 (tile 64) or 128 x 64 (tile 128) output tiles.
 hipError_t launch_gemm(const GemmArgs &g, int tile, hipStream_t stream);
+
+// A whole DST-I product through its two folded halves (same fold, odd = 0 and 1)
+// in one launch.
+hipError_t launch_gemm_fold(const GemmArgs &even, const GemmArgs &odd, int tile, hipStream_t stream);
 
 // BF16x3: split an FP32 rows x cols matrix (pitch ld) into bf16 halves whose
 // rows have pitch ldd (a multiple of 32, zero-padded), transposed or not ...
@@ -30,7 +42,9 @@ hipError_t launch_gemm_nt_bf16x3(const __bf16 *ah, const __bf16 *al, int lda, co
                                  int ldb, float *C, const float *scale, int ldc, int M, int N, int K, int tile,
                                  hipStream_t stream);
 
-// The four products of the tick's solve for every scene of a batch.
+// The four products of the tick's solve for every scene of a batch, and the
+// blocks per scene they need (KC_GEMM).
 hipError_t launch_dst_gemms(const Batch &b, hipStream_t stream);
+int dst_gemm_blocks(int nx, int ny);
 
 }  // namespace emb

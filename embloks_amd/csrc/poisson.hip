@@ -23,7 +23,8 @@ struct EmbPoisson {
     int nx = 0, ny = 0;
     hipStream_t stream = nullptr;
     float *rhs = nullptr, *w1 = nullptr, *w2 = nullptr, *phi = nullptr, *inv_lap = nullptr;
-    const float *sx = nullptr, *sy = nullptr;
+    const float *sx = nullptr, *sy = nullptr;                                   // whole DST-I matrices
+    const float *sx_e = nullptr, *sx_o = nullptr, *sy_et = nullptr, *sy_o = nullptr;   // their halves
     hipEvent_t e0 = nullptr, e1 = nullptr;
     // BF16x3: halves of Sx, Sy and of each product's data operand, rows padded to 32
     int kx = 0, ky = 0;
@@ -33,25 +34,56 @@ struct EmbPoisson {
 namespace emb {
 namespace {
 
+// out = Sx . in (rows of out from the halves of Sx and the folded rows of in).
+hipError_t left_product(const EmbPoisson &p, const float *in, float *out, int tile) {
+    GemmArgs h[2];
+    for (int odd = 0; odd < 2; odd++) {
+        GemmArgs &g = h[odd];
+        const int mh = odd ? p.nx / 2 : (p.nx + 1) / 2;
+        g.A = odd ? p.sx_o : p.sx_e;
+        g.lda = mh;
+        g.B = in;
+        g.ldb = p.ny;
+        g.C = out;
+        g.ldc = p.ny;
+        g.M = g.K = mh;
+        g.N = p.ny;
+        g.fold = 1;
+        g.fold_n = p.nx;
+        g.odd = odd;
+    }
+    return launch_gemm_fold(h[0], h[1], tile, p.stream);
+}
+
+// out = (in . Sy) .* scale (columns from the folded columns of in and the halves of Sy).
+hipError_t right_product(const EmbPoisson &p, const float *in, float *out, const float *scale, int tile) {
+    GemmArgs h[2];
+    for (int odd = 0; odd < 2; odd++) {
+        GemmArgs &g = h[odd];
+        const int mh = odd ? p.ny / 2 : (p.ny + 1) / 2;
+        g.A = in;
+        g.lda = p.ny;
+        g.B = odd ? p.sy_o : p.sy_et;
+        g.ldb = mh;
+        g.C = out;
+        g.ldc = p.ny;
+        g.scale = scale;
+        g.M = p.nx;
+        g.N = g.K = mh;
+        g.fold = 2;
+        g.fold_n = p.ny;
+        g.odd = odd;
+    }
+    return launch_gemm_fold(h[0], h[1], tile, p.stream);
+}
+
+// The four products, each as its two folded halves (half the flops of S . x).
 hipError_t enqueue_solve_fp32(const EmbPoisson &p, int tile) {
-    const int nx = p.nx, ny = p.ny;
-    GemmArgs g;
-    g.M = nx;
-    g.N = ny;
-    g.ldc = ny;
-    // stage 0: w1 = Sx . rhs
-    g.A = p.sx; g.lda = nx; g.B = p.rhs; g.ldb = ny; g.K = nx; g.C = p.w1; g.scale = nullptr;
-    hipError_t e = launch_gemm(g, tile, p.stream);
-    if (e != hipSuccess) return e;
-    // stage 1: w2 = (w1 . Sy) * inv_lap
-    g.A = p.w1; g.lda = ny; g.B = p.sy; g.ldb = ny; g.K = ny; g.C = p.w2; g.scale = p.inv_lap;
-    if ((e = launch_gemm(g, tile, p.stream)) != hipSuccess) return e;
-    // stage 2: w1 = Sx . w2
-    g.A = p.sx; g.lda = nx; g.B = p.w2; g.ldb = ny; g.K = nx; g.C = p.w1; g.scale = nullptr;
-    if ((e = launch_gemm(g, tile, p.stream)) != hipSuccess) return e;
-    // stage 3: phi = w1 . Sy
-    g.A = p.w1; g.lda = ny; g.B = p.sy; g.ldb = ny; g.K = ny; g.C = p.phi; g.scale = nullptr;
-    return launch_gemm(g, tile, p.stream);
+    hipError_t e;
+    if ((e = left_product(p, p.rhs, p.w1, tile)) != hipSuccess) return e;            // w1 = Sx . rhs
+    if ((e = right_product(p, p.w1, p.w2, p.inv_lap, tile)) != hipSuccess) return e;  // w2 = (w1 . Sy) * inv_lap
+    if ((e = left_product(p, p.w2, p.w1, tile)) != hipSuccess) return e;             // w1 = Sx . w2
+    return right_product(p, p.w1, p.phi, nullptr, tile);                             // phi = w1 . Sy
 }
 
 // The same four products as C = A . B^T on split operands: Sx is A as it is
@@ -116,9 +148,16 @@ int emb_poisson_create(EmbDevice *dev, int32_t nx, int32_t ny, EmbPoisson **out)
         if (hipStreamCreate(&p->stream) != hipSuccess || hipEventCreate(&p->e0) != hipSuccess ||
             hipEventCreate(&p->e1) != hipSuccess)
             return bail(fail(EMB_E_DEVICE, "stream or event creation failed"));
-        int rc = emb::device_sine(dev, nx, &p->sx);
-        if (rc == EMB_OK) rc = emb::device_sine(dev, ny, &p->sy);
+        const emb::SineSet *set_x = nullptr, *set_y = nullptr;
+        int rc = emb::device_sine(dev, nx, &set_x);
+        if (rc == EMB_OK) rc = emb::device_sine(dev, ny, &set_y);
         if (rc != EMB_OK) return bail(rc);
+        p->sx = set_x->full;
+        p->sy = set_y->full;
+        p->sx_e = set_x->e;
+        p->sx_o = set_x->o;
+        p->sy_et = set_y->e_t;
+        p->sy_o = set_y->o;
         p->kx = (nx + 31) / 32 * 32;
         p->ky = (ny + 31) / 32 * 32;
         const size_t halves[6] = {(size_t)nx * p->kx, (size_t)nx * p->kx, (size_t)ny * p->ky, (size_t)ny * p->ky,

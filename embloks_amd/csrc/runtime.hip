@@ -3,6 +3,8 @@
 
 #include "runtime.hpp"
 
+#include "gemm.hpp"
+
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -113,27 +115,43 @@ template <class T> hipError_t put(const T *dst, const std::vector<T> &src, hipSt
     return put(const_cast<T *>(dst), src, st);
 }
 
-// The DST-I matrix of length n on the device, built once per device; the
-// caller holds dev->lock.
-int sine_locked(EmbDevice *dev, int n, const float **out) {
+// The DST-I matrix of length n and its halves on the device, built once per
+// device; the caller holds dev->lock.
+int sine_locked(EmbDevice *dev, int n, const SineSet **out) {
     auto it = dev->sine.find(n);
     if (it == dev->sine.end()) {
-        std::vector<float> m = sine_matrix(n);
-        float *p = nullptr;
-        EMB_HIP(hipMalloc(&p, m.size() * sizeof(float)));
-        EMB_HIP(hipMemcpy(p, m.data(), m.size() * sizeof(float), hipMemcpyHostToDevice));
-        it = dev->sine.emplace(n, p).first;
+        const std::vector<float> full = sine_matrix(n);
+        const SineHalves hv = sine_halves(n);
+        SineSet set;
+        auto upload = [](float **p, const std::vector<float> &v) {
+            if (v.empty()) return hipSuccess;
+            hipError_t e = hipMalloc(p, v.size() * sizeof(float));
+            return e == hipSuccess ? hipMemcpy(*p, v.data(), v.size() * sizeof(float), hipMemcpyHostToDevice) : e;
+        };
+        hipError_t e = upload(&set.full, full);
+        if (e == hipSuccess) e = upload(&set.e, hv.se);
+        if (e == hipSuccess) e = upload(&set.e_t, hv.se_t);
+        if (e == hipSuccess) e = upload(&set.o, hv.so);
+        if (e != hipSuccess) {
+            for (float *p : {set.full, set.e, set.e_t, set.o}) (void)hipFree(p);
+            return fail(EMB_E_DEVICE, std::string("sine matrices: ") + hipGetErrorString(e));
+        }
+        it = dev->sine.emplace(n, set).first;
     }
-    *out = it->second;
+    *out = &it->second;
     return EMB_OK;
 }
 
-int shared_tables(EmbDevice *dev, const EmbSettings &s, int nx, int ny, const float **sx, const float **sy,
-                  DeviceTables &tab) {
+int shared_tables(EmbDevice *dev, const EmbSettings &s, int nx, int ny, DevScene &d, DeviceTables &tab) {
     std::lock_guard<std::mutex> guard(dev->lock);
-    int rc = sine_locked(dev, nx, sx);
-    if (rc == EMB_OK) rc = sine_locked(dev, ny, sy);
+    const SineSet *sx = nullptr, *sy = nullptr;
+    int rc = sine_locked(dev, nx, &sx);
+    if (rc == EMB_OK) rc = sine_locked(dev, ny, &sy);
     if (rc != EMB_OK) return rc;
+    d.sx_e = sx->e;
+    d.sx_o = sx->o;
+    d.sy_et = sy->e_t;
+    d.sy_o = sy->o;
     const auto key = std::make_tuple(s.electron_softening, (int)s.smoothing_passes, s.electron_cutoff);
     auto it = dev->tables.find(key);
     if (it == dev->tables.end()) {
@@ -180,7 +198,7 @@ int shared_tables(EmbDevice *dev, const EmbSettings &s, int nx, int ny, const fl
 
 }  // namespace
 
-int device_sine(EmbDevice *dev, int n, const float **out) {
+int device_sine(EmbDevice *dev, int n, const SineSet **out) {
     std::lock_guard<std::mutex> guard(dev->lock);
     return sine_locked(dev, n, out);
 }
@@ -206,7 +224,7 @@ void scene_blocks(const DevScene &d, int out[KC_COUNT]) {
     out[KC_CELLS] = cdiv(cells, 256);
     out[KC_SMOOTH] = cdiv(d.cols + 2 * d.passes, 32) * cdiv(d.rows + 2 * d.passes, 32);
     out[KC_FDTD] = cdiv(d.nx + 1, FDTD_TI) * cdiv(d.ny + 1, FDTD_TJ);
-    out[KC_GEMM] = 2 * cdiv(d.nx, 64) * cdiv(d.ny, 64);
+    out[KC_GEMM] = dst_gemm_blocks(d.nx, d.ny);
     out[KC_NODES] = cdiv((long long)(d.nx + 1) * (d.ny + 1), 256);
     out[KC_STACKS] = std::max(1, d.n_stacks) * TRIM_BLOCKS;
     out[KC_SCENE] = 1;
@@ -249,7 +267,7 @@ int make_resident(EmbDevice *dev, const EmbSettings &s, HostScene &&hs_in, hipSt
     d.passes = s.smoothing_passes;
     d.n_ac = (int32_t)hs.ac_freq.size();
     DeviceTables tables;
-    int rc = shared_tables(dev, s, hs.nx, hs.ny, &d.sine_x, &d.sine_y, tables);
+    int rc = shared_tables(dev, s, hs.nx, hs.ny, d, tables);
     if (rc != EMB_OK) return rc;
     d.close_on = s.electron_cutoff > 0.0 ? 1 : 0;
     d.reach = tables.reach;
@@ -460,7 +478,8 @@ int emb_device_open(int ordinal, EmbDevice **out) {
 int emb_device_close(EmbDevice *dev) {
     if (!dev) return EMB_OK;
     (void)hipSetDevice(dev->ordinal);
-    for (auto &kv : dev->sine) (void)hipFree(kv.second);
+    for (auto &kv : dev->sine)
+        for (float *p : {kv.second.full, kv.second.e, kv.second.e_t, kv.second.o}) (void)hipFree(p);
     for (auto &kv : dev->tables) {
         if (kv.second.g) (void)hipFree(kv.second.g);
         if (kv.second.square) (void)hipFree(kv.second.square);

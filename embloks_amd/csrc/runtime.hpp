@@ -47,11 +47,19 @@ struct DeviceTables {
 
 }  // namespace emb
 
+namespace emb {
+// The DST-I matrix of one length on the device, whole and as its halves
+// (sine_halves): the tick's products use the halves.
+struct SineSet {
+    float *full = nullptr, *e = nullptr, *e_t = nullptr, *o = nullptr;
+};
+}  // namespace emb
+
 struct EmbDevice {
     int ordinal = 0;
     int n_cu = 0;
     std::mutex lock;
-    std::map<int, float *> sine;   // DST-I matrices per length, shared by every scene
+    std::map<int, emb::SineSet> sine;   // DST-I matrices per length, shared by every scene
     std::map<std::tuple<double, int, double>, emb::DeviceTables> tables;
 };
 
@@ -78,7 +86,7 @@ struct Resident {
 };
 
 // The FP32 DST-I matrix of length n on the device (built once per device).
-int device_sine(EmbDevice *dev, int n, const float **out);
+int device_sine(EmbDevice *dev, int n, const SineSet **out);
 
 // Bytes a scene of this shape needs (arena only; shared tables excluded).
 size_t scene_bytes(const EmbSettings &s, int cols, int rows, int64_t n, int n_stacks, int n_stack_cells, int n_bats, int n_ac);

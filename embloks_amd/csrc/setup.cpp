@@ -277,6 +277,25 @@ std::vector<float> sine_matrix(int n) {
     return out;
 }
 
+SineHalves sine_halves(int n) {
+    const int m = (n + 1) / 2, h = n / 2;
+    auto S = [n](long long k, long long j) {
+        return (float)std::sin(PI * (double)(((k + 1) * (j + 1)) % (2 * (n + 1))) / (double)(n + 1));
+    };
+    SineHalves o;
+    o.se.resize((size_t)m * m);
+    o.se_t.resize((size_t)m * m);
+    o.so.resize((size_t)h * h);
+    for (int i = 0; i < m; i++)
+        for (int j = 0; j < m; j++) {
+            o.se[(size_t)i * m + j] = S(2 * i, j);
+            o.se_t[(size_t)j * m + i] = S(2 * i, j);
+        }
+    for (int i = 0; i < h; i++)
+        for (int j = 0; j < h; j++) o.so[(size_t)i * h + j] = S(2 * i + 1, j);
+    return o;
+}
+
 std::string prepare_scene(const EmbSettings &s, const EmbScene &sc, HostScene &o) {
     if (sc.cols <= 0 || sc.rows <= 0) return "scene has no cells";
     if (!sc.kind || !sc.resistance || !sc.density || !sc.emf || !sc.orientation || !sc.frequency)

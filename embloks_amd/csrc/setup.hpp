@@ -76,6 +76,17 @@ std::string prepare_scene(const EmbSettings &s, const EmbScene &scene, HostScene
 // The DST-I matrix sin(pi (k+1)(j+1) / (n+1)), FP64 rounded to FP32.
 std::vector<float> sine_matrix(int n);
 
+// What the symmetry S[k][n-1-j] = (-1)^k S[k][j] of that matrix S leaves of it:
+// the even rows se[i][j] = S[2i][j] over j < m (m x m, m = ceil(n/2)), their
+// transpose, and the odd rows so[i][j] = S[2i+1][j] over j < h (h x h,
+// h = floor(n/2); so is symmetric). S x is then se p (rows 2i) and so q (rows
+// 2i+1) with p_j = x_j + x_{n-1-j}, q_j = x_j - x_{n-1-j} (j < h) and, for odd n,
+// p_h = x_h.
+struct SineHalves {
+    std::vector<float> se, se_t, so;
+};
+SineHalves sine_halves(int n);
+
 // Electron substeps of the FDTD step (maxwell.rs:376-378).
 int fdtd_substeps_of(const EmbSettings &s);
 

@@ -85,6 +85,31 @@ def test_one_tick_matches_the_oracle(name, before):
             assert abs(g - c) < 1e-5
 
 
+def odd_world():
+    """A 31 x 32 conductor ring with one battery: 63 x 64 Yee nodes, so the
+    DST's row fold has a middle row (odd length) while its halves still take
+    the unguarded tile path where they can."""
+    w = W.World(31, 32, seed=7)
+    ring = [(c, r) for c in range(4, 27) for r in range(6, 24) if c in (4, 26) or r in (6, 23)]
+    w.paint(ring, W.CONDUCTOR, resistance=1.0)
+    w.paint([(15, 6)], W.BATTERY, emf=1.5, orientation=W.RIGHT)
+    return w
+
+
+@gpu
+@pytest.mark.parametrize("before", [0, 5])
+def test_odd_node_count_one_tick_matches_the_oracle(before):
+    cpu, sim = reset_and_compare(odd_world(), before)
+    ex, ey, bz = sim.read_yee()
+    cex, cey, cbz = cpu.yee()
+    assert ex.shape == (64, 64) and ey.shape == (63, 65)
+    err = {"ex": relative_error(ex, cex), "ey": relative_error(ey, cey),
+           "phi": relative_error(sim.read_potential(), cpu.potential_nodes())}
+    print(before, err)
+    for k in ("ex", "ey", "phi"):
+        assert err[k] < 1e-5, err
+
+
 @gpu
 def test_fast_light_takes_three_fdtd_substeps():
     # c = 40: c dt sqrt 2 / 0.9 > 2, so three FDTD substeps per tick (maxwell.rs:376-378).

@@ -19,7 +19,7 @@ def solver():
 
 
 @gpu
-@pytest.mark.parametrize("nx,ny", [(72, 64), (130, 97), (256, 256), (300, 200)])
+@pytest.mark.parametrize("nx,ny", [(72, 64), (130, 97), (97, 130), (33, 31), (256, 256), (300, 200)])
 def test_small_grids_match_the_oracle(nx, ny):
     rng = np.random.default_rng(nx * 1000 + ny)
     rhs = rng.normal(size=(nx, ny))
