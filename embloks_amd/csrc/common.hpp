@@ -52,6 +52,7 @@ constexpr int MAX_STACKS_PER_BLOCK = 64;
 constexpr int SCAN_TILE = 1024;   // cells per block of the cell-list scan
 constexpr int TICK_TABLE = 64;    // ticks of host-computed AC phases / EMF targets
 constexpr int PUSH_TILE = 8;      // cells per side of a push tile
+constexpr int DEP_TILE = 24;      // cells per side of a deposit + smoothing tile
 constexpr int CELL_MODE_MIN = 6;  // a push tile with a cell this crowded runs the cell mode
 constexpr int HEAVY_CELL = 64;    // a cell this crowded is pushed by k_push_heavy, spread over blocks
 constexpr int TRIM_BLOCKS = 8;    // blocks per battery stack in k_trim

@@ -1251,30 +1251,30 @@ __global__ __launch_bounds__(1024) void k_rest(Batch b) {
 }
 
 // ---------------------------------------------------------------------------------------
-// Deposit + smoothing (fields.rs:18-63, 91-112; maxwell.rs:280-316): per 32 x 32
-// tile of the smoothing region (visible cells extended by `passes`), electrons of
-// the tile's window scatter integer CIC weights into LDS int64 accumulators;
-// the binomial filter (passes x [1 2 1] per axis) then runs in LDS and the tile
-// writes rho, Jx, Jy on its nodes. Integer arithmetic makes the sum independent
-// of order: the result is bit-deterministic and charge is conserved exactly.
+// Deposit + smoothing (fields.rs:18-63, 91-112; maxwell.rs:280-316): per DEP_TILE^2
+// tile of the smoothing region (visible cells extended by `passes`), the cells
+// whose electrons reach the tile's window add their integer CIC sums to LDS int64
+// accumulators; the binomial filter (passes x [1 2 1] per axis) then runs in LDS
+// and the tile writes rho, Jx, Jy on its nodes. Integer sums are independent of
+// order: the result is bit-deterministic and charge is conserved exactly.
 
-constexpr int ST = 32;          // output tile
+constexpr int ST = DEP_TILE;    // output tile
 constexpr int MAX_PASSES = 4;
 constexpr int SW = ST + 2 * MAX_PASSES;   // LDS row pitch: window of the tile plus a halo of `passes`
-constexpr int DEP_THREADS = 1024;
-constexpr int DEP_SRC = (SW + 2) * (SW + 2);   // source cells of a window
+constexpr int DEP_THREADS = 512;    // two blocks per CU at up to 128 VGPRs: 289 tiles at 512^2 all resident at once
 
-__device__ __forceinline__ int wave_max_int(int v) {
-    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
-    return v;
+// x rounded to the nearest integer (ties to even) as int64: __float2ll_rn, or,
+// when the caller knows |x| < 2^31 (SMALL), the same value through the 32-bit
+// conversion (two instructions instead of a dozen).
+template <bool SMALL>
+__device__ __forceinline__ long long rint_ll(float x) {
+    return SMALL ? (long long)__float2int_rn(x) : __float2ll_rn(x);
 }
 
 __global__ __launch_bounds__(DEP_THREADS) void k_deposit_smooth(Batch b, int bias) {
     __shared__ long long dep[3][SW * SW];
     __shared__ long long tmp[ST * SW];
     __shared__ float wmax[DEP_THREADS / 64];
-    __shared__ int cell_first[DEP_SRC], cell_count[DEP_SRC];
-    __shared__ int wmost[DEP_THREADS / 64];
     for (int blk = blockIdx.x; blk < b.total[KC_SMOOTH]; blk += gridDim.x) {
         const int s = scene_of(b, KC_SMOOTH, blk);
         const DevScene &S = b.scenes[s];
@@ -1297,73 +1297,103 @@ __global__ __launch_bounds__(DEP_THREADS) void k_deposit_smooth(Batch b, int bia
             dep[2][i * SW + j] = 0;
         }
         // Electrons whose CIC corners reach the window live in cells [w0 - 1, w0 + W].
+        // One thread per such source cell sums what the cell's electrons put on
+        // the 3 x 3 nodes around it (CIC, fields.rs:27-40: the lower corner is
+        // floor(x - 1/2), 22-bit fixed-point weights per axis) in registers, in
+        // the cell's canonical electron order, then adds the nine sums to the
+        // window with LDS atomics: 27 atomics per cell, not 12 per electron.
+        // Charge: the exact int64 products of the weights. Current: the FP32 sum
+        // of v wx wy over the cell, rounded to int64 once per node. Integer atomics
+        // give the same bits in any order. A thread whose cell the tile owns also
+        // sums the cell current (physics.rs:34-43) and the speed maximum.
         const int sc0 = max(wc0 - 1, 0), sc1 = min(wc0 + W, S.cols - 1);
         const int sr0 = max(wr0 - 1, 0), sr1 = min(wr0 + W, S.rows - 1);
         const int ncy = max(sr1 - sr0 + 1, 0), ncell = max(sc1 - sc0 + 1, 0) * ncy;
-        int most = 0;
+        // The scene's arrays in registers: the LDS atomics would otherwise make the
+        // compiler reload them from the scene table every iteration.
+        const float *sfx = S.sfx, *sfy = S.sfy, *svx = S.svx, *svy = S.svy;
+        const int32_t *cstart = S.cstart;
+        const int rows = S.rows, cols = S.cols;
+        float vmax = 0.0f;
         for (int q = threadIdx.x; q < ncell; q += DEP_THREADS) {
-            const int cell = (sc0 + q / ncy) * S.rows + sr0 + q % ncy;
-            const int first = S.cstart[cell], n = S.cstart[cell + 1] - first;
-            cell_first[q] = first;
-            cell_count[q] = n;
-            most = max(most, n);
-        }
-        most = wave_max_int(most);
-        if ((threadIdx.x & 63) == 0) wmost[threadIdx.x >> 6] = most;
-        __syncthreads();
-        most = 0;
-        for (int w = 0; w < DEP_THREADS / 64; w++) most = max(most, wmost[w]);
-        // Lane q takes electron q / ncell of source cell q % ncell: neighbouring
-        // lanes work on neighbouring cells, so one atomic instruction mostly hits
-        // distinct addresses (in cell order, 16 lanes of a cell would share four).
-        // Integer sums: the same bits in any order.
-        for (int q = threadIdx.x; q < ncell * most; q += DEP_THREADS) {
-            const int ci = q % ncell, kk = q / ncell;
-            if (kk >= cell_count[ci]) continue;
-            const int k = cell_first[ci] + kk;
-            // CIC weights (fields.rs:27-40) from the cell + fraction position: the
-            // lower corner cell is floor(x - 1/2), with 22-bit fixed-point weights.
-            const int ce = sc0 + ci / ncy, re = sr0 + ci % ncy;
-            const float x = S.sfx[k], y = S.sfy[k];
-            const int c0 = x >= 0.5f ? ce : ce - 1, r0 = y >= 0.5f ? re : re - 1;
-            const float ux = x >= 0.5f ? x - 0.5f : x + 0.5f, uy = y >= 0.5f ? y - 0.5f : y + 0.5f;
-            const int fu = (int)rintf(ux * 4194304.0f), fv = (int)rintf(uy * 4194304.0f);
-            const int wx[2] = {(1 << W_BITS) - fu, fu}, wy[2] = {(1 << W_BITS) - fv, fv};
-            const int ca = clampi(c0, 0, S.cols - 1), cb = clampi(c0 + 1, 0, S.cols - 1);
-            const int ra = clampi(r0, 0, S.rows - 1), rb = clampi(r0 + 1, 0, S.rows - 1);
-            const float vx = S.svx[k], vy = S.svy[k];
-            for (int a = 0; a < 2; a++) {
-                const int li = (a ? cb : ca) - wc0;
-                if (li < 0 || li >= W || wx[a] == 0) continue;
-                for (int bb = 0; bb < 2; bb++) {
-                    const int lj = (bb ? rb : ra) - wr0;
-                    if (lj < 0 || lj >= W || wy[bb] == 0) continue;
-                    const long long w = (long long)wx[a] * (long long)wy[bb];
-                    const float wf = __fmul_rn((float)wx[a], (float)wy[bb]);
-                    const long long jx = __float2ll_rn(__fmul_rn(__fmul_rn(vx, wf), J_FROM_W));
-                    const long long jy = __float2ll_rn(__fmul_rn(__fmul_rn(vy, wf), J_FROM_W));
-                    const int at = li * SW + lj;
-                    atomicAdd((unsigned long long *)&dep[0][at], as_ull(-w));
-                    atomicAdd((unsigned long long *)&dep[1][at], as_ull(jx));
-                    atomicAdd((unsigned long long *)&dep[2][at], as_ull(jy));
+            const int ce = sc0 + q / ncy, re = sr0 + q % ncy;
+            const int cell = ce * rows + re;
+            const int k0 = cstart[cell], k1 = cstart[cell + 1];
+            unsigned long long qn[9];
+            float jxn[9], jyn[9];
+#pragma unroll
+            for (int n = 0; n < 9; n++) {
+                qn[n] = 0;
+                jxn[n] = 0.0f;
+                jyn[n] = 0.0f;
+            }
+            long long ix = 0, iy = 0;
+            // Four electrons per round, their loads all in flight before the sums;
+            // the padding slots past k1 add zero weights.
+            for (int kb = k0; kb < k1; kb += 4) {
+                float xs[4], ys[4], vxs[4], vys[4];
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const bool on = kb + u < k1;
+                    const int k = on ? kb + u : k0;
+                    xs[u] = sfx[k];
+                    ys[u] = sfy[k];
+                    vxs[u] = on ? svx[k] : 0.0f;
+                    vys[u] = on ? svy[k] : 0.0f;
+                }
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    if (kb + u >= k1) break;
+                    const float x = xs[u], y = ys[u], vx = vxs[u], vy = vys[u];
+                    const bool lx = x < 0.5f, ly = y < 0.5f;   // lower corner at ce - 1 / re - 1
+                    const int fu = (int)rintf((lx ? x + 0.5f : x - 0.5f) * 4194304.0f);
+                    const int fv = (int)rintf((ly ? y + 0.5f : y - 0.5f) * 4194304.0f);
+                    const int w0x = (1 << W_BITS) - fu, w0y = (1 << W_BITS) - fv;
+                    // weights on node columns ce - 1, ce, ce + 1 and rows re - 1, re, re + 1
+                    const int wxn[3] = {lx ? w0x : 0, lx ? fu : w0x, lx ? 0 : fu};
+                    const int wyn[3] = {ly ? w0y : 0, ly ? fv : w0y, ly ? 0 : fv};
+#pragma unroll
+                    for (int i = 0; i < 3; i++)
+#pragma unroll
+                        for (int j = 0; j < 3; j++) {
+                            qn[3 * i + j] += (unsigned long long)(unsigned)wxn[i] * (unsigned)wyn[j];
+                            const float wf = __fmul_rn((float)wxn[i], (float)wyn[j]);
+                            jxn[3 * i + j] = __fmaf_rn(vx, wf, jxn[3 * i + j]);
+                            jyn[3 * i + j] = __fmaf_rn(vy, wf, jyn[3 * i + j]);
+                        }
+                    if (fabsf(vx) < 8.0f && fabsf(vy) < 8.0f) {   // |v| 2^28 < 2^31
+                        ix += rint_ll<true>(__fmul_rn(vx, (float)J_SCALE));
+                        iy += rint_ll<true>(__fmul_rn(vy, (float)J_SCALE));
+                    } else {
+                        ix += rint_ll<false>(__fmul_rn(vx, (float)J_SCALE));
+                        iy += rint_ll<false>(__fmul_rn(vy, (float)J_SCALE));
+                    }
+                    vmax = fmaxf(vmax, sqrtf(vx * vx + vy * vy));
                 }
             }
-        }
-        // Cells owned by this tile: current (physics.rs:34-43) and speed maximum.
-        float vmax = 0.0f;
-        for (int q = threadIdx.x; q < ST * ST; q += DEP_THREADS) {
-            const int c = R0x - P + q / ST, r = R0y - P + (q % ST);
-            if (c < 0 || r < 0 || c >= S.cols || r >= S.rows) continue;
-            const int cell = c * S.rows + r;
-            long long ix = 0, iy = 0;
-            for (int k = S.cstart[cell]; k < S.cstart[cell + 1]; k++) {
-                const float vx = S.svx[k], vy = S.svy[k];
-                ix += __float2ll_rn(__fmul_rn(vx, (float)J_SCALE));
-                iy += __float2ll_rn(__fmul_rn(vy, (float)J_SCALE));
-                vmax = fmaxf(vmax, sqrtf(vx * vx + vy * vy));
+            // Cells owned by this tile: the cell current.
+            const int oc = ce - (R0x - P), orow = re - (R0y - P);
+            if (oc >= 0 && oc < ST && orow >= 0 && orow < ST) {
+                S.cur[cell] = ix;
+                S.cur[cells + cell] = iy;
             }
-            S.cur[cell] = ix;
-            S.cur[cells + cell] = iy;
+            if (k1 == k0) continue;
+            // Nodes past the grid's edge fold onto the edge (corners clamped,
+            // fields.rs:28-37); nodes outside the window belong to other tiles.
+#pragma unroll
+            for (int i = 0; i < 3; i++) {
+                const int li = clampi(ce - 1 + i, 0, cols - 1) - wc0;
+                if (li < 0 || li >= W) continue;
+#pragma unroll
+                for (int j = 0; j < 3; j++) {
+                    const int lj = clampi(re - 1 + j, 0, rows - 1) - wr0;
+                    if (lj < 0 || lj >= W || qn[3 * i + j] == 0) continue;
+                    const int at = li * SW + lj;
+                    atomicAdd((unsigned long long *)&dep[0][at], as_ull(-(long long)qn[3 * i + j]));
+                    atomicAdd((unsigned long long *)&dep[1][at], as_ull(__float2ll_rn(__fmul_rn(jxn[3 * i + j], J_FROM_W))));
+                    atomicAdd((unsigned long long *)&dep[2][at], as_ull(__float2ll_rn(__fmul_rn(jyn[3 * i + j], J_FROM_W))));
+                }
+            }
         }
         vmax = wave_max(vmax);
         if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = vmax;

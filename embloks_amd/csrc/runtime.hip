@@ -222,7 +222,7 @@ void scene_blocks(const DevScene &d, int out[KC_COUNT]) {
     out[KC_ELECTRONS] = cdiv(d.n, 256);
     out[KC_SCAN] = cdiv(cells, SCAN_TILE);
     out[KC_CELLS] = cdiv(cells, 256);
-    out[KC_SMOOTH] = cdiv(d.cols + 2 * d.passes, 32) * cdiv(d.rows + 2 * d.passes, 32);
+    out[KC_SMOOTH] = cdiv(d.cols + 2 * d.passes, DEP_TILE) * cdiv(d.rows + 2 * d.passes, DEP_TILE);
     out[KC_FDTD] = cdiv(d.nx + 1, FDTD_TI) * cdiv(d.ny + 1, FDTD_TJ);
     out[KC_GEMM] = dst_gemm_blocks(d.nx, d.ny);
     out[KC_NODES] = cdiv((long long)(d.nx + 1) * (d.ny + 1), 256);
