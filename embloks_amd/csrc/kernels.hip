@@ -169,7 +169,8 @@ __device__ __forceinline__ void canon_cell(const DevScene &S, int c, int s0, int
 constexpr int CANON_CAP = 6144;
 
 __global__ __launch_bounds__(256) void k_canon(Batch b) {
-    __shared__ int lorder[CANON_CAP];
+    __shared__ int lorder[CANON_CAP], lsorted[CANON_CAP];
+    __shared__ int lstart[257];
     for (int blk = blockIdx.x; blk < b.total[KC_CELLS]; blk += gridDim.x) {
         const int s = scene_of(b, KC_CELLS, blk);
         const DevScene &S = b.scenes[s];
@@ -199,18 +200,25 @@ __global__ __launch_bounds__(256) void k_canon(Batch b) {
             continue;
         }
         for (int q = S0 + (int)threadIdx.x; q < S1; q += 256) lorder[q - S0] = S.order[q];
+        lstart[threadIdx.x] = (mine ? s0 : S1) - S0;
+        if (threadIdx.x == 0) lstart[256] = S1 - S0;
         __syncthreads();
-        if (mine) {
-            int *o = lorder - S0;
-            for (int a = s0 + 1; a < s1; a++) {
-                const int key = o[a];
-                int p = a - 1;
-                while (p >= s0 && o[p] > key) {
-                    o[p + 1] = o[p];
-                    p--;
-                }
-                o[p + 1] = key;
+        // Rank sort within each cell, one thread per slot: an electron's place is
+        // the number of smaller indices in its cell (the indices are distinct).
+        // A crowded cell costs each of its lanes one pass over the cell, where a
+        // thread's insertion sort would cost it the square.
+        for (int a = threadIdx.x; a < S1 - S0; a += 256) {
+            int ci = 0;
+#pragma unroll
+            for (int step = 128; step > 0; step >>= 1) {   // the last cell starting at or before a
+                const int up = ci + step;
+                ci = lstart[up] <= a ? up : ci;
             }
+            const int b0 = lstart[ci], b1 = lstart[ci + 1];
+            const int key = lorder[a];
+            int rank = 0;
+            for (int j = b0; j < b1; j++) rank += lorder[j] < key ? 1 : 0;
+            lsorted[b0 + rank] = key;
         }
         __syncthreads();
         // Four slots per thread per round: their gathers are all in flight at once.
@@ -221,7 +229,7 @@ __global__ __launch_bounds__(256) void k_canon(Batch b) {
 #pragma unroll
             for (int u = 0; u < 4; u++) {
                 const int q = q0 + u * 256;
-                e[u] = q < S1 ? lorder[q - S0] : 0;
+                e[u] = q < S1 ? lsorted[q - S0] : 0;
             }
 #pragma unroll
             for (int u = 0; u < 4; u++) {
