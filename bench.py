@@ -32,8 +32,8 @@ FP32_MFMA_PEAK_TFLOPS = 157.3
 BF16_MFMA_PEAK_TFLOPS = 2500.0
 
 
-def algorithmic(phase: str, nx: int, ny: int, cols: int, rows: int, n: int):
-    """(amount, unit, bound) per launch of one kernel phase of one tick."""
+def algorithmic(phase: str, nx: int, ny: int, cols: int, rows: int, n: int, sub: int = 1):
+    """(amount, unit, bound) of one kernel phase over one tick (`sub` FDTD substeps)."""
     nodes, cells = nx * ny, cols * rows
     if phase == "dst_gemm":
         # Two right-hand sides, four products each, every product as its two
@@ -42,7 +42,9 @@ def algorithmic(phase: str, nx: int, ny: int, cols: int, rows: int, n: int):
     if phase == "push":
         return 32.0 * n, "byte", "hbm"      # x, y, vx, vy in and out (SoA FP32)
     if phase == "fdtd":
-        return 32.0 * nodes, "byte", "hbm"  # Ex, Ey, Bz, Jx, Jy in; Ex, Ey, Bz out
+        # Ex, Ey, Bz, Jx, Jy in and Ex, Ey, Bz out per substep; the last substep
+        # also reads rho and writes the two Gauss right-hand sides.
+        return 32.0 * nodes * sub + 12.0 * nodes, "byte", "hbm"
     if phase == "deposit_smooth":
         return 16.0 * n + 12.0 * nodes + 16.0 * cells, "byte", "hbm"
     if phase in ("count", "scatter"):
@@ -97,7 +99,7 @@ def run_single(args):
     phases = sim.kernel_times(max(10, min(args.steps, 100)))
     residual, charge, energy = sim.diagnostics()
     info = dict(nx=sim.nx, ny=sim.ny, cols=sim.cols, rows=sim.rows, n=sim.n, substeps=sim.dims().last_substeps,
-                gauss_residual=residual, electron_charge=charge)
+                fdtd_substeps=sim.dims().fdtd_substeps, gauss_residual=residual, electron_charge=charge)
     return world, elapsed, phases, info
 
 
@@ -127,7 +129,8 @@ def measured_traffic(phase: str):
 
 def roofline(phases: dict, info: dict):
     dom = max(phases, key=phases.get)
-    amount, unit, bound = algorithmic(dom, info["nx"], info["ny"], info["cols"], info["rows"], info["n"])
+    amount, unit, bound = algorithmic(dom, info["nx"], info["ny"], info["cols"], info["rows"], info["n"],
+                                      info["fdtd_substeps"])
     sec = phases[dom] / 1e3
     if unit == "flop":
         achieved = amount / sec / 1e12
@@ -230,6 +233,98 @@ def run_p3m(repeats: int = 10):
             "candidate_pairs_per_s": round(candidates / (ms * 1e-3), 1), "scene_setup_s": round(setup_s, 2)}
 
 
+# The reference's published FP64 CPU times per tick on one core (README.md:493-501).
+README_GRIDS = ((40, 32, 2.4), (100, 80, 15.0), (200, 160, 59.0))
+
+
+def run_readme_grids(cpu: bool):
+    """The reference bench's worlds at the README's three grids: 30% random
+    1-ohm wire, 8 electrons per wire cell, seed 1 (crates/embloks-gpu/examples/
+    bench.rs:16-22). GPU ms per tick over 64 ticks after 8 warm-up ticks
+    (bench.rs:47-54); the FP64 oracle on one thread over 5 ticks after 1
+    warm-up (bench.rs:38-46); the reference's published one-core figure."""
+    from embloks_amd import world as W
+    from embloks_amd.gpusim import GpuSim
+
+    out = []
+    for cols, rows, published in README_GRIDS:
+        w = W.random_wire(cols, rows)
+        sim = GpuSim(w)
+        sim.run_ticks(8)
+        t0 = time.perf_counter()
+        sim.step(64)
+        sim.sync()
+        gpu_ms = (time.perf_counter() - t0) / 64 * 1e3
+        entry = {"grid": f"{cols}x{rows}", "yee_nodes": sim.nx * sim.ny, "electrons": w.n_electrons,
+                 "gpu_ms_per_tick": round(gpu_ms, 4), "reference_cpu_ms_per_tick_published": published}
+        sim.destroy()
+        if cpu:
+            sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+            from oracle import orc
+            orc.set_threads(1)
+            cs = orc.Sim(w)
+            cs.step(1)
+            t0 = time.perf_counter()
+            cs.step(5)
+            entry["oracle_cpu_ms_per_tick_1_thread"] = round((time.perf_counter() - t0) / 5 * 1e3, 3)
+        out.append(entry)
+    return out
+
+
+def run_gauss_log(ticks: int = 10000, every: int = 100):
+    """The 512^2 scene's long run: `ticks` ticks with the FP64 Gauss residual
+    max |div E - 4 pi k rho| and the exact deposited electron charge read every
+    `every` ticks (each read waits for the ticks before it)."""
+    from embloks_amd import world as W
+    from embloks_amd.gpusim import GpuSim
+
+    world = W.rectangles_circuit(512, carriers_per_cell=16, seed=2)
+    sim = GpuSim(world)
+    res, charge_exact, charge_dev = [], 0, 0
+    t0 = time.perf_counter()
+    for _ in range(ticks // every):
+        sim.step(every)
+        r, _q, _e = sim.diagnostics()
+        res.append(r)
+        # the integer deposit: -2^44 per electron, exactly
+        dev = int(sim.read_charge_fixed().sum()) + sim.n * (1 << 44)
+        charge_exact += int(dev == 0)
+        charge_dev = max(charge_dev, abs(dev))
+    elapsed = time.perf_counter() - t0
+    sim.destroy()
+    return {"ticks": ticks, "checked_every": every, "checks": len(res), "residual_first": res[0],
+            "residual_max": max(res), "residual_last": res[-1],
+            "electron_charge_exact_checks": charge_exact, "electron_charge_max_dev_fixed": charge_dev,
+            "ms_per_tick_with_checks": round(elapsed / ticks * 1e3, 4)}
+
+
+def run_large(ticks: int = 5):
+    """The bandwidth-bound kernels at a large grid: a 4096^2 world of 30% random
+    wire at one electron per wire cell (4128^2 Yee nodes), phase times from HIP
+    events over `ticks` ticks: FDTD at 32 B per node-substep plus 12 B per node
+    for the Gauss right-hand sides, the push at 32 B per electron, against 8 TB/s."""
+    from embloks_amd import world as W
+    from embloks_amd.gpusim import GpuSim
+
+    w = W.random_wire(4096, 4096, carriers_per_cell=1, seed=6)
+    sim = GpuSim(w)
+    sim.run_ticks(2)
+    ph = sim.kernel_times(ticks)
+    nodes = sim.nx * sim.ny
+    sub = sim.dims().fdtd_substeps
+    fdtd_gbs = algorithmic("fdtd", sim.nx, sim.ny, 0, 0, 0, sub)[0] / (ph["fdtd"] * 1e-3) / 1e9
+    push_gbs = 32.0 * sim.n / (ph["push"] * 1e-3) / 1e9
+    out = {"grid": "4096x4096", "yee_nodes": nodes, "electrons": sim.n,
+           "world": "30% random 1-ohm wire, 1 electron per wire cell, seed 6",
+           "phases_ms": {k: round(v, 4) for k, v in ph.items()},
+           "fdtd": {"achieved": round(fdtd_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(fdtd_gbs / HBM_PEAK_GBS, 4)},
+           "push": {"achieved": round(push_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(push_gbs / HBM_PEAK_GBS, 4)}}
+    sim.destroy()
+    return out
+
+
 def cpu_baseline(world, budget_s: float = 20.0):
     """The FP64 oracle (the CPU restatement of embloks-core's tick) on this
     host over a bounded sample of the same scene: one thread (the reference's
@@ -312,6 +407,9 @@ def main():
     ap.add_argument("--serve-scale", type=float, default=0.02, help="fraction of each request's 1000-20000 steps")
     ap.add_argument("--no-poisson", action="store_true", help="skip the isolated 2048^2 DST solve")
     ap.add_argument("--no-p3m", action="store_true", help="skip the 4096^2 clustered close-range force")
+    ap.add_argument("--no-readme", action="store_true", help="skip the README grids (40x32, 100x80, 200x160)")
+    ap.add_argument("--no-gauss-log", action="store_true", help="skip the 10^4-tick Gauss residual log")
+    ap.add_argument("--no-large", action="store_true", help="skip the 4096^2 FDTD / push bandwidth leg")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a plain 1 GiB device-to-device copy 5 times (scale for HBM byte counters)")
     args = ap.parse_args()
@@ -352,6 +450,12 @@ def main():
         out["poisson"] = run_poisson()
     if not args.no_p3m:
         out["p3m"] = run_p3m()
+    if not args.no_large:
+        out["large"] = run_large()
+    if not args.no_gauss_log:
+        out["gauss_log"] = run_gauss_log()
+    if not args.no_readme:
+        out["readme_grids"] = run_readme_grids(cpu=not args.no_cpu)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(world)
     if not args.no_serve:

@@ -56,7 +56,19 @@ constexpr int DEP_TILE = 24;      // cells per side of a deposit + smoothing til
 constexpr int CELL_MODE_MIN = 6;  // a push tile with a cell this crowded runs the cell mode
 constexpr int HEAVY_CELL = 64;    // a cell this crowded is pushed by k_push_heavy, spread over blocks
 constexpr int TRIM_BLOCKS = 8;    // blocks per battery stack in k_trim
-constexpr int FDTD_TI = 16, FDTD_TJ = 32;   // corners per FDTD tile (x, y)
+constexpr int FDTD_STRIP = 60;    // corner columns per FDTD wave (64 lanes: two halo lanes each side)
+constexpr int FDTD_WAVES = 4;     // FDTD waves per block (adjacent strips)
+constexpr int FDTD_AHEAD = 4;     // corner rows an FDTD wave has loads in flight for
+constexpr int FDTD_MIN_BLOCKS = 4096;   // FDTD blocks a scene keeps while its rows per wave double
+// Corner rows one FDTD wave walks: 32 where the grid still gives FDTD_MIN_BLOCKS
+// blocks (each wave re-reads 3 rows of its neighbour above: 9% at 32 rows), fewer,
+// down to 4, on small grids, whose launch needs the waves more.
+__host__ __device__ inline int fdtd_rows(int nx, int ny) {
+    const long long sblocks = ((ny + 1 + FDTD_STRIP - 1) / FDTD_STRIP + FDTD_WAVES - 1) / FDTD_WAVES;
+    int rows = 32;
+    while (rows > 4 && sblocks * ((nx + 1 + rows - 1) / rows) < FDTD_MIN_BLOCKS) rows >>= 1;
+    return rows;
+}
 constexpr int ION_SUM = 33;       // samples per axis of the summed ion table
 constexpr int PAIR_SAMPLES = 4096;   // samples of the radial pair table (close_range.rs:30)
 
@@ -195,7 +207,7 @@ enum KernelClass {
     KC_SCAN,            // SCAN_TILE cells per block
     KC_CELLS,           // 256 cells per block
     KC_SMOOTH,          // 32 x 32 tiles of the smoothing region
-    KC_FDTD,            // FDTD_TI x FDTD_TJ tiles of the (nx+1) x (ny+1) corner space
+    KC_FDTD,            // FDTD_WAVES strips x fdtd_rows() rows of the (nx+1) x (ny+1) corner space
     KC_GEMM,            // 2 channels x 64 x 64 output tiles of nx x ny
     KC_NODES,           // 256 corners of (nx+1) x (ny+1) per block
     KC_STACKS,          // TRIM_BLOCKS blocks per battery stack (at least one stack per scene)

@@ -1457,116 +1457,160 @@ __global__ __launch_bounds__(DEP_THREADS) void k_deposit_smooth(Batch b, int bia
 
 // ---------------------------------------------------------------------------------------
 // FDTD leapfrog with the absorbing margin, fused (maxwell.rs:235-270;
-// fields.wgsl:106-186): per FDTD_TI x FDTD_TJ tile of the corner space, Bz' = Bz - h/2 curl E
-// on the tile plus halo, E' from curl Bz' and the edge currents, Bz'' = Bz' - h/2
-// curl E', then the damping factors. On the scene's last FDTD substep the tile
-// also writes the right-hand sides of the Gauss solve: -4 pi k rho and div E
-// (maxwell.rs:170-173, 200-204; fields.wgsl:192-203).
+// fields.wgsl:106-186): Bz' = Bz - h/2 curl E, E' from curl Bz' and the edge
+// currents, Bz'' = Bz' - h/2 curl E', then the damping factors. On the scene's
+// last FDTD substep the kernel also writes the right-hand sides of the Gauss
+// solve: -4 pi k rho and div E (maxwell.rs:170-173, 200-204; fields.wgsl:192-203).
+//
+// Streaming: a wave owns FDTD_STRIP corner columns (lane l at column
+// j = strip base + l - 2: two halo lanes on each side) and walks fdtd_rows()
+// corner rows t, reading each row of Ex, Ey, Bz, Jx, Jy once, coalesced, and writing
+// each row of the results once. Column neighbours come from the next lane; the
+// row before is held in registers. At step t it forms Bz'(t), Ex'(t), Ey'(t - 1)
+// and Bz''(t - 1) (each needs the row after it), so it starts two rows early.
+// Values outside the grid are zero, as the reference's boundaries have them;
+// the arithmetic is the reference's, term for term.
 
-constexpr int TI = FDTD_TI, TJ = FDTD_TJ;
+// Device arrays in the global address space: accesses compile to global_*
+// instructions (a generic pointer gives flat_*, whose completion count also
+// covers LDS traffic such as the lane shuffles) and cannot alias the scene table.
+typedef __attribute__((address_space(1))) float gfloat;
 
-__global__ __launch_bounds__(256) void k_fdtd(Batch b, int f) {
-    __shared__ float EX[(TI + 3) * (TJ + 3)];
-    __shared__ float EY[(TI + 3) * (TJ + 3)];
-    __shared__ float BZ[(TI + 2) * (TJ + 2)];
-    __shared__ float JX[(TI + 2) * (TJ + 2)];
-    __shared__ float JY[(TI + 2) * (TJ + 2)];
-    constexpr int WE = TJ + 3, WB = TJ + 2;
+// The value of the previous / next lane (column j - 1 / j + 1) by a whole-wave
+// DPP shift (wave_shr:1 = 0x138, wave_shl:1 = 0x130): a VALU move, no LDS
+// traffic. The end lanes get 0; they are halo lanes whose results are not kept.
+__device__ __forceinline__ float lane_prev(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xf, 0xf, true));
+}
+__device__ __forceinline__ float lane_next(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xf, 0xf, true));
+}
+
+// One corner row's loads: Ex, Bz of row t, Ey, Jx, Jy of node row t and rho of
+// node row t - 2 (the right-hand side written at that step).
+struct FdtdRow {
+    float ex, ey, bz, jx, jy, rho;
+};
+
+__global__ __launch_bounds__(64 * FDTD_WAVES) void k_fdtd(Batch b, int f) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     for (int blk = blockIdx.x; blk < b.total[KC_FDTD]; blk += gridDim.x) {
         const int s = scene_of(b, KC_FDTD, blk);
         const DevScene &S = b.scenes[s];
         if (f >= S.fdtd_substeps) continue;
         const int nx = S.nx, ny = S.ny;
         const int local = block_in_scene(b, KC_FDTD, s, blk);
-        const int tiles_j = (ny + 1 + TJ - 1) / TJ;
-        // Neighbouring tiles read halos of the fields this one writes: substep f
+        const int strips = (ny + 1 + FDTD_STRIP - 1) / FDTD_STRIP;
+        const int sblocks = (strips + FDTD_WAVES - 1) / FDTD_WAVES;
+        const int rows = fdtd_rows(nx, ny);
+        const int strip = (local % sblocks) * FDTD_WAVES + wave;
+        const int row_lo = (local / sblocks) * rows, row_hi = min(row_lo + rows, nx + 1);
+        const int steps = row_hi - row_lo + 3;   // rows t = row_lo - 2 .. row_hi (row_lo <= nx: at least 4)
+        const int j = strip * FDTD_STRIP + lane - 2;
+        const bool own_col = lane >= 2 && lane < 2 + FDTD_STRIP && strip < strips && j <= ny;
+        // Neighbouring waves read rows and columns this one writes: substep f
         // reads one buffer of the pair and writes the other.
         const bool odd = (f & 1) != 0;
-        const float *src_ex = odd ? S.ex2 : S.ex, *src_ey = odd ? S.ey2 : S.ey, *src_bz = odd ? S.bz2 : S.bz;
-        float *dst_ex = odd ? S.ex : S.ex2, *dst_ey = odd ? S.ey : S.ey2, *dst_bz = odd ? S.bz : S.bz2;
+        // (no two of these arrays overlap: the reads of later rows may move ahead
+        // of the writes of earlier ones)
+        const gfloat *__restrict__ src_ex = (const gfloat *)(odd ? S.ex2 : S.ex);
+        const gfloat *__restrict__ src_ey = (const gfloat *)(odd ? S.ey2 : S.ey);
+        const gfloat *__restrict__ src_bz = (const gfloat *)(odd ? S.bz2 : S.bz);
+        gfloat *__restrict__ dst_ex = (gfloat *)(odd ? S.ex : S.ex2);
+        gfloat *__restrict__ dst_ey = (gfloat *)(odd ? S.ey : S.ey2);
+        gfloat *__restrict__ dst_bz = (gfloat *)(odd ? S.bz : S.bz2);
+        const gfloat *__restrict__ jx = (const gfloat *)S.jx;
+        const gfloat *__restrict__ jy = (const gfloat *)S.jy;
+        const gfloat *__restrict__ rho = (const gfloat *)S.rho;
+        gfloat *__restrict__ w1 = (gfloat *)S.w1;
+        const gfloat *__restrict__ dex_x = (const gfloat *)S.damp_ex_x;
+        const gfloat *__restrict__ dey_x = (const gfloat *)S.damp_ey_x;
+        const gfloat *__restrict__ dbz_x = (const gfloat *)S.damp_bz_x;
         const bool rhs = f == S.fdtd_substeps - 1;
-        const int i0 = (local / tiles_j) * TI, j0 = (local % tiles_j) * TJ;
-        // Loads: E at [i0-2, i0+TI] x [j0-2, j0+TJ], Bz at [i0-1, i0+TI] x [j0-1, j0+TJ],
-        // node J at [i0-2, i0+TI) x [j0-2, j0+TJ).
-        for (int q = threadIdx.x; q < (TI + 3) * WE; q += 256) {
-            const int a = q / WE, c = q - (q / WE) * WE;
-            const int i = i0 - 2 + a, j = j0 - 2 + c;
-            EX[q] = (i >= 0 && i <= nx && j >= 0 && j < ny) ? src_ex[(size_t)i * ny + j] : 0.0f;
-            EY[q] = (i >= 0 && i < nx && j >= 0 && j <= ny) ? src_ey[(size_t)i * (ny + 1) + j] : 0.0f;
-        }
-        for (int q = threadIdx.x; q < (TI + 2) * WB; q += 256) {
-            const int a = q / WB, c = q - (q / WB) * WB;
-            const int i = i0 - 1 + a, j = j0 - 1 + c;
-            BZ[q] = (i >= 0 && i <= nx && j >= 0 && j <= ny) ? src_bz[(size_t)i * (ny + 1) + j] : 0.0f;
-            const int in = i - 1, jn = j - 1;
-            const bool node = in >= 0 && in < nx && jn >= 0 && jn < ny;
-            JX[q] = node ? S.jx[(size_t)in * ny + jn] : 0.0f;
-            JY[q] = node ? S.jy[(size_t)in * ny + jn] : 0.0f;
-        }
-        __syncthreads();
-        // Bz' on [i0-1, i0+TI] x [j0-1, j0+TJ]: curl E (maxwell.rs:207-227).
-        for (int q = threadIdx.x; q < (TI + 2) * WB; q += 256) {
-            const int a = q / WB, c = q - (q / WB) * WB;
-            const int i = i0 - 1 + a, j = j0 - 1 + c;
-            if (i < 0 || i > nx || j < 0 || j > ny) continue;
-            const float curl = ((EY[(a + 1) * WE + c + 1] - EY[a * WE + c + 1]) - EX[(a + 1) * WE + c + 1]) + EX[(a + 1) * WE + c];
-            BZ[q] = BZ[q] + (-S.half_h) * curl;
-        }
-        __syncthreads();
-        // E' on [i0-1, i0+TI) x [j0-1, j0+TJ) (maxwell.rs:255-264).
-        for (int q = threadIdx.x; q < (TI + 1) * (TJ + 1); q += 256) {
-            const int a = q / (TJ + 1), c = q - (q / (TJ + 1)) * (TJ + 1);
-            const int i = i0 - 1 + a, j = j0 - 1 + c;
-            const int e = (a + 1) * WE + c + 1;
-            if (i >= 0 && i <= nx && j >= 0 && j < ny) {
-                const float jxe = (i == 0 || i == nx) ? 0.0f : S.jscale * (JX[a * WB + c + 1] + JX[(a + 1) * WB + c + 1]);
-                EX[e] += S.c2h * (BZ[a * WB + c + 1] - BZ[a * WB + c]) - jxe;
-            }
-            if (i >= 0 && i < nx && j >= 0 && j <= ny) {
-                const float jye = (j == 0 || j == ny) ? 0.0f : S.jscale * (JY[(a + 1) * WB + c] + JY[(a + 1) * WB + c + 1]);
-                EY[e] -= S.c2h * (BZ[(a + 1) * WB + c] - BZ[a * WB + c]) + jye;
-            }
-        }
-        __syncthreads();
-        // Bz'' and damping on the owned corners [i0, i0+TI) x [j0, j0+TJ).
-        for (int q = threadIdx.x; q < TI * TJ; q += 256) {
-            const int a = q / TJ + 1, c = q % TJ + 1;   // Bz index of corner (i0 + a - 1, j0 + c - 1)
-            const int i = i0 + a - 1, j = j0 + c - 1;
-            if (i > nx || j > ny) continue;
-            const float curl = ((EY[(a + 1) * WE + c + 1] - EY[a * WE + c + 1]) - EX[(a + 1) * WE + c + 1]) + EX[(a + 1) * WE + c];
-            const float v = BZ[a * WB + c] + (-S.half_h) * curl;
-            dst_bz[(size_t)i * (ny + 1) + j] = v * fminf(S.damp_bz_x[i], S.damp_bz_y[j]);
-        }
-        __syncthreads();
-        // Damp E' in place over its whole range, then write the owned edges.
-        for (int q = threadIdx.x; q < (TI + 1) * (TJ + 1); q += 256) {
-            const int a = q / (TJ + 1), c = q - (q / (TJ + 1)) * (TJ + 1);
-            const int i = i0 - 1 + a, j = j0 - 1 + c;
-            const int e = (a + 1) * WE + c + 1;
-            if (i >= 0 && i <= nx && j >= 0 && j < ny) EX[e] *= fminf(S.damp_ex_x[i], S.damp_ex_y[j]);
-            if (i >= 0 && i < nx && j >= 0 && j <= ny) EY[e] *= fminf(S.damp_ey_x[i], S.damp_ey_y[j]);
-        }
-        __syncthreads();
-        for (int q = threadIdx.x; q < TI * TJ; q += 256) {
-            const int a = q / TJ, c = q % TJ;
-            const int i = i0 + a, j = j0 + c;
-            const int e = (a + 2) * WE + c + 2;
-            if (i <= nx && j < ny) dst_ex[(size_t)i * ny + j] = EX[e];
-            if (i < nx && j <= ny) dst_ey[(size_t)i * (ny + 1) + j] = EY[e];
-            if (rhs) {
-                // Node (i - 1, j - 1): its four edges lie in [i0-1, i0+TI) x [j0-1, j0+TJ).
-                const int in = i - 1, jn = j - 1;
-                if (in >= 0 && jn >= 0 && in < nx && jn < ny) {
-                    const int en = (a + 1) * WE + c + 1;
-                    const float div = (EX[en + WE] - EX[en]) + (EY[en + 1] - EY[en]);
-                    const size_t node = (size_t)in * ny + jn;
-                    const size_t nodes = (size_t)nx * ny;
-                    S.w1[node] = -S.four_pi_k * S.rho[node];
-                    S.w1[nodes + node] = div;
+        const float hh = S.half_h, c2h = S.c2h, js = S.jscale, four_pi_k = S.four_pi_k;
+        const bool jin_ex = j >= 0 && j < ny, jin_c = j >= 0 && j <= ny;   // Ex / node columns, Ey / corner columns
+        const float dby = jin_c ? S.damp_bz_y[j] : 0.0f, dey = jin_c ? S.damp_ey_y[j] : 0.0f;
+        const float dex = jin_ex ? S.damp_ex_y[j] : 0.0f;
+        float ey_prev = 0.0f, jx_prev = 0.0f, jy_prev = 0.0f, bzp_prev = 0.0f;
+        float exp_prev = 0.0f, eyp_prev = 0.0f, exd_prev = 0.0f, eyd_prev = 0.0f;
+        // Loads are unconditional, at indices clamped into the arrays, and the
+        // values outside the grid are zeroed afterwards; the loads of row
+        // t + FDTD_AHEAD are issued before the arithmetic of row t.
+        const int jc_ex = min(max(j, 0), ny - 1), jc_c = min(max(j, 0), ny);
+        const int nj = j - 1, njc = min(max(nj, 0), ny - 1);
+        auto load = [&](int t) {
+            FdtdRow r;
+            const size_t tc = (size_t)min(max(t, 0), nx), te = (size_t)min(max(t, 0), nx - 1);
+            r.ex = src_ex[tc * ny + jc_ex];
+            r.ey = src_ey[te * (ny + 1) + jc_c];
+            r.bz = src_bz[tc * (ny + 1) + jc_c];
+            r.jx = jx[te * ny + jc_ex];
+            r.jy = jy[te * ny + jc_ex];
+            r.rho = rhs ? rho[(size_t)min(max(t - 2, 0), nx - 1) * ny + njc] : 0.0f;
+            return r;
+        };
+        FdtdRow ring[FDTD_AHEAD];
+#pragma unroll
+        for (int k = 0; k < FDTD_AHEAD; k++) ring[k] = load(row_lo - 2 + k);
+        for (int base = 0; base < steps; base += FDTD_AHEAD) {
+#pragma unroll
+            for (int k = 0; k < FDTD_AHEAD; k++) {
+                const int t = row_lo - 2 + base + k;   // the row read at this step
+                const FdtdRow v = ring[k];
+                if (base + k + FDTD_AHEAD < steps) ring[k] = load(t + FDTD_AHEAD);
+                // (steps past the last one leave every store out: their tp is row_hi or more)
+                const bool c_row = t >= 0 && t <= nx, e_row = t >= 0 && t < nx;
+                const float ex_f = c_row && jin_ex ? v.ex : 0.0f;
+                const float ey_f = e_row && jin_c ? v.ey : 0.0f;
+                const float bz_f = c_row && jin_c ? v.bz : 0.0f;
+                const float jx_f = e_row && jin_ex ? v.jx : 0.0f;
+                const float jy_f = e_row && jin_ex ? v.jy : 0.0f;
+                // Bz'(t, j) (maxwell.rs:207-227)
+                const float curl = ((ey_f - ey_prev) - ex_f) + lane_prev(ex_f);
+                const float bzp = c_row && jin_c ? bz_f + (-hh) * curl : 0.0f;
+                // Ex'(t, j): edges on x = 0 and x = nx carry no current (maxwell.rs:243-248, 255-258)
+                // (every lane shuffles, outside any condition: a lane left out of a
+                // shuffle gives its neighbour nothing)
+                const float bzp_next = lane_next(bzp), jy_prev_m = lane_prev(jy_prev);
+                const float jxe = (t == 0 || t == nx) ? 0.0f : js * (jx_prev + jx_f);
+                const float exp_f = c_row && jin_ex ? ex_f + (c2h * (bzp_next - bzp) - jxe) : 0.0f;
+                // Ey'(t - 1, j) (maxwell.rs:259-264)
+                const int tp = t - 1;
+                const bool ep_row = tp >= 0 && tp < nx, cp_row = tp >= 0 && tp <= nx;
+                const float jye = (j == 0 || j == ny) ? 0.0f : js * (jy_prev_m + jy_prev);
+                const float eyp = ep_row && jin_c ? ey_prev - (c2h * (bzp - bzp_prev) + jye) : 0.0f;
+                // Bz''(t - 1, j) and the damped fields of row t - 1
+                const float curl2 = ((eyp - eyp_prev) - exp_prev) + lane_prev(exp_prev);
+                const float dexx = dex_x[min(max(tp, 0), nx)], deyx = dey_x[min(max(tp, 0), nx - 1)];
+                const float dbzx = dbz_x[min(max(tp, 0), nx)];
+                const float exd = cp_row && jin_ex ? exp_prev * fminf(dexx, dex) : 0.0f;
+                const float eyd = ep_row && jin_c ? eyp * fminf(deyx, dey) : 0.0f;
+                if (own_col && tp >= row_lo && tp < row_hi) {
+                    const float bzpp = bzp_prev + (-hh) * curl2;
+                    dst_bz[(size_t)tp * (ny + 1) + j] = bzpp * fminf(dbzx, dby);
+                    if (j < ny) dst_ex[(size_t)tp * ny + j] = exd;
+                    if (tp < nx) dst_ey[(size_t)tp * (ny + 1) + j] = eyd;
                 }
+                // div E' at node (t - 2, j - 1), with the Gauss right-hand sides
+                if (rhs) {   // uniform: every lane shuffles
+                    const float div = (lane_prev(exd) - lane_prev(exd_prev)) + (eyd_prev - lane_prev(eyd_prev));
+                    const int ni = t - 2;
+                    if (own_col && tp >= row_lo && tp < row_hi && ni >= 0 && ni < nx && nj >= 0 && nj < ny) {
+                        const size_t node = (size_t)ni * ny + nj;
+                        w1[node] = -four_pi_k * v.rho;
+                        w1[(size_t)nx * ny + node] = div;
+                    }
+                }
+                ey_prev = ey_f;
+                jx_prev = jx_f;
+                jy_prev = jy_f;
+                bzp_prev = bzp;
+                exp_prev = exp_f;
+                eyp_prev = eyp;
+                exd_prev = exd;
+                eyd_prev = eyd;
             }
         }
-        __syncthreads();
     }
 }
 
@@ -1760,7 +1804,7 @@ hipError_t tick_kernels(const Batch &b, int fdtd_substeps, hipStream_t stream, h
     EMB_MARK(7);
     EMB_LAUNCH(k_deposit_smooth, KC_SMOOTH, DEP_THREADS, b, 0);
     EMB_MARK(8);
-    for (int f = 0; f < fdtd_substeps; f++) EMB_LAUNCH(k_fdtd, KC_FDTD, 256, b, f);
+    for (int f = 0; f < fdtd_substeps; f++) EMB_LAUNCH(k_fdtd, KC_FDTD, 64 * FDTD_WAVES, b, f);
     EMB_MARK(9);
     hipError_t e = launch_dst_gemms(b, stream);
     if (e != hipSuccess) return e;
