@@ -56,6 +56,8 @@ constexpr int DEP_TILE = 24;      // cells per side of a deposit + smoothing til
 constexpr int CELL_MODE_MIN = 6;  // a push tile with a cell this crowded runs the cell mode
 constexpr int HEAVY_CELL = 64;    // a cell this crowded is pushed by k_push_heavy, spread over blocks
 constexpr int TRIM_BLOCKS = 8;    // blocks per battery stack in k_trim
+constexpr int REST_PHASES = 7;    // work phases of one extra electron substep in k_rest
+constexpr int REST_THREADS = 256; // threads per k_rest block
 constexpr int FDTD_STRIP = 60;    // corner columns per FDTD wave (64 lanes: two halo lanes each side)
 constexpr int FDTD_WAVES = 4;     // FDTD waves per block (adjacent strips)
 constexpr int FDTD_AHEAD = 4;     // corner rows an FDTD wave has loads in flight for
@@ -68,6 +70,9 @@ __host__ __device__ inline int fdtd_rows(int nx, int ny) {
     int rows = 32;
     while (rows > 4 && sblocks * ((nx + 1 + rows - 1) / rows) < FDTD_MIN_BLOCKS) rows >>= 1;
     return rows;
+}
+__host__ __device__ inline int rest_sync_len(int max_substeps) {
+    return 1 + REST_PHASES * (max_substeps > 1 ? max_substeps - 1 : 0);
 }
 constexpr int ION_SUM = 33;       // samples per axis of the summed ion table
 constexpr int PAIR_SAMPLES = 4096;   // samples of the radial pair table (close_range.rs:30)
@@ -107,7 +112,7 @@ struct SceneCounters {
     uint32_t vmax_bits[2];    // max |v| (float bits) of the electrons, by tick parity
     int32_t heavy_n;          // cells listed in heavy_cells by the last sort
     int32_t substeps;         // electron substeps of the current tick
-    int32_t rest_error;       // set when the serial substep path rejected a scene
+    int32_t rest_error;       // set when a k_rest wait ran past its limit (the tick's later substeps were dropped)
     uint32_t pad[2];
 };
 
@@ -194,6 +199,8 @@ struct DevScene {
     int32_t *has_gain;        // per stack
     float *trim_part;         // per stack and trim block: terminal potential sums (low, high)
     int32_t *trim_done;       // per stack: trim blocks finished this tick (back to 0 by the last)
+    int32_t *rest_sync;       // [rest_sync_len(max_substeps)]: k_rest's work ticket, then items done per
+                              // (substep, phase); zeroed by k_trim at the start of every tick
     const float *phase_tab;   // [TICK_TABLE][n_ac]: frac(f t) per AC cell and tick
     const float *target_tab;  // [TICK_TABLE][n_stacks]: EMF target per tick
     int32_t n_ac;
@@ -214,6 +221,7 @@ enum KernelClass {
     KC_SCENE,           // one block per scene
     KC_PUSH,            // PUSH_TILE x PUSH_TILE cell tiles
     KC_HEAVY,           // blocks sharing the scene's heavy cells (one per 16k electrons, at most 256)
+    KC_REST,            // k_rest work-queue blocks (one per 2048 electrons, at most 1024)
     KC_COUNT
 };
 

@@ -297,6 +297,8 @@ __global__ __launch_bounds__(TRIM_THREADS) void k_trim(Batch b) {
         const int st = local / TRIM_BLOCKS, tb = local - st * TRIM_BLOCKS;
         const uint32_t t = S.ctr->tick;
         if (local == 0 && threadIdx.x == 0) S.ctr->vmax_bits[(t + 1) & 1] = 0;
+        if (local == 0)   // k_rest's queue of this tick starts empty
+            for (int q = threadIdx.x; q < rest_sync_len(S.max_substeps); q += TRIM_THREADS) S.rest_sync[q] = 0;
         if (st >= S.n_stacks) continue;
         const StackInfo I = S.stacks[st];
         if (!I.both_wired || I.amplitude == 0.0f) {
@@ -1197,64 +1199,175 @@ __global__ __launch_bounds__(FORCE_THREADS) void k_force_tiled(Batch b, float2 *
     push_tiled_body<ITEM_FORCE, FORCE_THREADS>(b, 0, force_out);
 }
 
-// Substeps 1.. of a tick that needs several (fast electrons): one 1024-thread
-// block per scene re-sorts and pushes serially. Rare (electrons drift far below
-// max_step / h), so it trades speed for a single launch with no grid barrier.
-__global__ __launch_bounds__(1024) void k_rest(Batch b) {
-    __shared__ int wpart[1024];
-    for (int blk = blockIdx.x; blk < b.total[KC_SCENE]; blk += gridDim.x) {
-        const int s = scene_of(b, KC_SCENE, blk);
+// Substeps 1.. of a tick that needs several (fast electrons; physics.rs:124-162:
+// every substep takes its forces from the positions at its start, so the cell
+// list is rebuilt before each, sim.rs:643-651). The host queues ticks without
+// knowing the substep count, so one launch per tick covers whatever the tick
+// needs, as a work queue: the blocks of a scene draw numbered items from the
+// scene's ticket counter, the items of each substep in phase order
+//   0 count      1024 electrons: cell and rank within it (atomic count)
+//   1 tile sum   SCAN_TILE cells: their electron count
+//   2 top scan   one item: exclusive scan of the tile sums, cstart[cells] = n
+//   3 starts     SCAN_TILE cells: cstart from the scan, counts back to zero
+//   4 scatter    1024 electrons: order[cstart + rank] = electron
+//   5 canon      256 cells: in-cell order by electron index and the snapshot
+//   6 push       256 slots: push_one
+// and an item first waits until every item of the phase before it is done.
+// Those items were drawn earlier, by blocks already running, whose own waits
+// go back further in the same way to the first phase, which waits for nothing:
+// so every wait ends, whatever number of blocks is resident at once. When no
+// scene needs a second substep every block leaves after one read.
+//
+// A wait that runs past REST_SPIN_LIMIT polls (about a second) marks the scene's
+// rest_error and drops the rest of the tick's substeps for that block, so a
+// fault in this bookkeeping shows as an error instead of a hung GPU.
+constexpr int REST_SPIN_LIMIT = 1 << 24;
+
+__device__ __forceinline__ int rest_items(int p, int n, int cells) {
+    switch (p) {
+    case 0: case 4: return (n + 1023) / 1024;
+    case 1: case 3: return (cells + SCAN_TILE - 1) / SCAN_TILE;
+    case 2: return 1;
+    case 5: return (cells + 255) / 256;
+    default: return (n + 255) / 256;
+    }
+}
+
+// Block-wide exclusive scan of one value per thread (REST_THREADS threads);
+// returns the thread's prefix and sets `total`.
+__device__ __forceinline__ int rest_block_scan(int v, int *wsum, int &total) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int incl = wave_scan_incl(v);
+    if (lane == 63) wsum[w] = incl;
+    __syncthreads();
+    int base = 0, all = 0;
+    for (int q = 0; q < REST_THREADS / 64; q++) {
+        if (q < w) base += wsum[q];
+        all += wsum[q];
+    }
+    __syncthreads();
+    total = all;
+    return base + incl - v;
+}
+
+__global__ __launch_bounds__(REST_THREADS) void k_rest(Batch b) {
+    __shared__ int sh_ticket, sh_fail;
+    __shared__ int wsum[REST_THREADS / 64];
+    const int tid = threadIdx.x;
+    for (int blk = blockIdx.x; blk < b.total[KC_REST]; blk += gridDim.x) {
+        const int s = scene_of(b, KC_REST, blk);
         const DevScene &S = b.scenes[s];
         const uint32_t t = S.ctr->tick;
-        const int sub = substeps_of(S, t);
+        // (scalar copies: every branch on them below is uniform for the compiler too)
+        const int sub = __builtin_amdgcn_readfirstlane(substeps_of(S, t));
         if (sub <= 1) continue;
         const float h = S.dt / (float)sub;
-        const int cells = S.cols * S.rows;
-        const int tid = threadIdx.x;
-        for (int k = 1; k < sub; k++) {
-            __threadfence();
-            __syncthreads();
-            for (int i = tid; i < S.n; i += 1024) {
-                const int cell = cell_of(S, S.cxy[i]);
-                S.erank[i] = atomicAdd(&S.count[cell], 1);
-                S.ecell[i] = cell;
-            }
-            __threadfence();
-            __syncthreads();
-            const int chunk = (cells + 1023) / 1024;
-            const int c0 = min(tid * chunk, cells), c1 = min(c0 + chunk, cells);
-            int sum = 0;
-            for (int c = c0; c < c1; c++) sum += S.count[c];
-            wpart[tid] = sum;
-            __syncthreads();
+        const int n = __builtin_amdgcn_readfirstlane(S.n), cells = __builtin_amdgcn_readfirstlane(S.cols * S.rows);
+        int per_sub = 0;
+        for (int p = 0; p < REST_PHASES; p++) per_sub += rest_items(p, n, cells);
+        const int total = (sub - 1) * per_sub;
+        int *ticket_ctr = S.rest_sync;
+        int *done = S.rest_sync + 1;   // [(k - 1) * REST_PHASES + p]
+        // Thread 0 publishes the block's last item (release: the block's stores,
+        // ordered by the barrier that ended the item, become visible with the
+        // count) and draws the next ticket in one region. The ticket and the
+        // wait's outcome are read back through readfirstlane: the loop's exits
+        // are then uniform branches, so no lane can come back to the ticket read
+        // before thread 0 has drawn (with the ticket in a vector register the
+        // compiler took the exits as divergent and sent the other lanes back to
+        // the loop head without thread 0's draw, where they spun on the old ticket).
+        int pending = -1;
+        for (;;) {
             if (tid == 0) {
-                int acc = 0;
-                for (int q = 0; q < 1024; q++) {
-                    const int v = wpart[q];
-                    wpart[q] = acc;
-                    acc += v;
+                if (pending >= 0) __hip_atomic_fetch_add(done + pending, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                sh_ticket = __hip_atomic_fetch_add(ticket_ctr, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            __syncthreads();
+            const int ticket = __builtin_amdgcn_readfirstlane(sh_ticket);
+            __syncthreads();   // every thread holds its copy before thread 0 draws again
+            if (ticket >= total) break;
+            // The ticket's substep, phase and item.
+            const int k = 1 + ticket / per_sub;
+            int item = ticket - (k - 1) * per_sub, p = 0;
+            while (item >= rest_items(p, n, cells)) item -= rest_items(p, n, cells), p++;
+            // Wait for the phase before (the previous substep's push for phase 0).
+            const int prev = p > 0 ? (k - 1) * REST_PHASES + p - 1 : (k >= 2 ? (k - 2) * REST_PHASES + REST_PHASES - 1 : -1);
+            // (sh_fail is written only here, between the ticket's barriers and
+            // the one below, so no thread still reads the previous item's value)
+            if (tid == 0) {
+                int failed = 0;
+                if (prev >= 0) {
+                    const int want = rest_items(p > 0 ? p - 1 : REST_PHASES - 1, n, cells);
+                    int spins = 0;
+                    while (__hip_atomic_load(done + prev, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < want) {
+                        __builtin_amdgcn_s_sleep(2);
+                        if (++spins > REST_SPIN_LIMIT) {
+                            failed = 1;
+                            __hip_atomic_store(&S.ctr->rest_error, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                            break;
+                        }
+                    }
                 }
+                sh_fail = failed;
             }
             __syncthreads();
-            int run = wpart[tid];
-            for (int c = c0; c < c1; c++) {
-                S.cstart[c] = run;
-                run += S.count[c];
-                S.count[c] = 0;
+            if (__builtin_amdgcn_readfirstlane(sh_fail)) break;
+            if (p == 0) {
+                for (int i = item * 1024 + tid; i < min(n, item * 1024 + 1024); i += REST_THREADS) {
+                    const int cell = cell_of(S, S.cxy[i]);
+                    S.erank[i] = atomicAdd(&S.count[cell], 1);
+                    S.ecell[i] = cell;
+                }
+            } else if (p == 1 || p == 3) {
+                const int c0 = item * SCAN_TILE, per = SCAN_TILE / REST_THREADS;
+                int v[SCAN_TILE / REST_THREADS], sum = 0;
+#pragma unroll
+                for (int q = 0; q < SCAN_TILE / REST_THREADS; q++) {
+                    const int c = c0 + tid * per + q;
+                    v[q] = c < cells ? S.count[c] : 0;
+                    sum += v[q];
+                }
+                int tile_total;
+                int run = rest_block_scan(sum, wsum, tile_total);
+                if (p == 1) {
+                    if (tid == 0) S.toff[item] = tile_total;
+                } else {
+                    run += S.toff[item];
+#pragma unroll
+                    for (int q = 0; q < SCAN_TILE / REST_THREADS; q++) {
+                        const int c = c0 + tid * per + q;
+                        if (c < cells) {
+                            S.cstart[c] = run;
+                            S.count[c] = 0;
+                        }
+                        run += v[q];
+                    }
+                }
+            } else if (p == 2) {
+                const int tiles = rest_items(1, n, cells);
+                int carry = 0;
+                for (int q0 = 0; q0 < tiles; q0 += REST_THREADS) {
+                    const int q = q0 + tid;
+                    const int v = q < tiles ? S.toff[q] : 0;
+                    int chunk;
+                    const int pre = rest_block_scan(v, wsum, chunk);
+                    if (q < tiles) S.toff[q] = carry + pre;
+                    carry += chunk;
+                }
+                if (tid == 0) S.cstart[cells] = n;
+            } else if (p == 4) {
+                for (int i = item * 1024 + tid; i < min(n, item * 1024 + 1024); i += REST_THREADS)
+                    S.order[S.cstart[S.ecell[i]] + S.erank[i]] = i;
+            } else if (p == 5) {
+                const int c = item * 256 + tid;
+                if (c < cells) canon_cell(S, c, S.cstart[c], S.cstart[c + 1]);
+            } else {
+                const int slot = item * 256 + tid;
+                if (slot < n) push_one(S, slot, k, h, t);
             }
-            if (tid == 0) S.cstart[cells] = S.n;
-            __threadfence();
-            __syncthreads();
-            for (int i = tid; i < S.n; i += 1024) S.order[S.cstart[S.ecell[i]] + S.erank[i]] = i;
-            __threadfence();
-            __syncthreads();
-            for (int c = tid; c < cells; c += 1024) canon_cell(S, c, S.cstart[c], S.cstart[c + 1]);
-            __threadfence();
-            __syncthreads();
-            for (int slot = tid; slot < S.n; slot += 1024) push_one(S, slot, k, h, t);
+            __syncthreads();   // the item's stores, before thread 0 publishes it
+            pending = (k - 1) * REST_PHASES + p;
         }
-        __threadfence();
-        __syncthreads();
     }
 }
 
@@ -1791,7 +1904,7 @@ hipError_t tick_kernels(const Batch &b, int fdtd_substeps, hipStream_t stream, h
     EMB_LAUNCH(k_push_sparse<false>, KC_ELECTRONS, 256, b, 0, nullptr);
     EMB_LAUNCH(k_push_heavy<false>, KC_HEAVY, HEAVY_THREADS, b, 0, nullptr);
     EMB_MARK(2);
-    if (b.total[KC_ELECTRONS] > 0) EMB_LAUNCH(k_rest, KC_SCENE, 1024, b);
+    if (b.total[KC_ELECTRONS] > 0) EMB_LAUNCH(k_rest, KC_REST, REST_THREADS, b);
     EMB_MARK(3);
     EMB_LAUNCH(k_count, KC_ELECTRONS, 256, b);
     EMB_MARK(4);

@@ -38,6 +38,7 @@ struct Counts {
     int cols, rows, nx, ny;
     int64_t n;
     int n_stacks, n_stack_cells, n_bats, n_ac;
+    int max_substeps;
 };
 
 // Every per-scene array in one arena (sizes follow the reference's regions,
@@ -98,6 +99,7 @@ void layout(Bump &a, const Counts &c, DevScene &d, DevScene **d_scene, int32_t *
     d.has_gain = a.take<int32_t>(c.n_stacks);
     d.trim_part = a.take<float>(2 * (size_t)c.n_stacks * TRIM_BLOCKS);
     d.trim_done = a.take<int32_t>(c.n_stacks);
+    d.rest_sync = a.take<int32_t>(rest_sync_len(c.max_substeps));
     *tab = a.take<float>((size_t)TICK_TABLE * (c.n_ac + c.n_stacks));
     d.phase_tab = *tab;
     d.target_tab = *tab ? *tab + (size_t)TICK_TABLE * c.n_ac : nullptr;
@@ -205,7 +207,8 @@ int device_sine(EmbDevice *dev, int n, const SineSet **out) {
 
 size_t scene_bytes(const EmbSettings &s, int cols, int rows, int64_t n, int n_stacks, int n_stack_cells, int n_bats,
                    int n_ac) {
-    Counts c{cols, rows, cols + 2 * s.absorbing_margin, rows + 2 * s.absorbing_margin, n, n_stacks, n_stack_cells, n_bats, n_ac};
+    Counts c{cols, rows, cols + 2 * s.absorbing_margin, rows + 2 * s.absorbing_margin, n, n_stacks, n_stack_cells, n_bats, n_ac,
+             s.max_substeps};
     Bump a{nullptr};
     DevScene d{};
     DevScene *ds;
@@ -230,6 +233,7 @@ void scene_blocks(const DevScene &d, int out[KC_COUNT]) {
     out[KC_SCENE] = 1;
     out[KC_PUSH] = cdiv(d.cols, PUSH_TILE) * cdiv(d.rows, PUSH_TILE);
     out[KC_HEAVY] = (int)std::min<long long>(256, std::max<long long>(1, d.n / 16384));
+    out[KC_REST] = (int)std::min<long long>(1024, std::max<long long>(1, cdiv(d.n, 2048)));
 }
 
 int make_resident(EmbDevice *dev, const EmbSettings &s, HostScene &&hs_in, hipStream_t st, Resident &r) {
@@ -238,7 +242,7 @@ int make_resident(EmbDevice *dev, const EmbSettings &s, HostScene &&hs_in, hipSt
     r.hs = std::move(hs_in);
     HostScene &hs = r.hs;
     const Counts c{hs.cols, hs.rows, hs.nx, hs.ny, hs.n, (int)hs.stacks.size(), (int)hs.stack_cells.size(), (int)hs.bats.size(),
-                   (int)hs.ac_freq.size()};
+                   (int)hs.ac_freq.size(), s.max_substeps};
     Bump dry{nullptr};
     DevScene tmp{};
     DevScene *dsp;
@@ -586,6 +590,10 @@ int emb_sim_step(EmbSim *sim, int ticks) {
 int emb_sim_sync(EmbSim *sim) {
     if (!sim) return fail(EMB_E_INVALID, "null sim");
     EMB_HIP(hipStreamSynchronize(sim->stream));
+    // k_rest's wait limit (a queue fault, never expected) leaves a mark instead of a hang.
+    int32_t rest_error = 0;
+    EMB_HIP(hipMemcpy(&rest_error, &sim->r.ds.ctr->rest_error, sizeof(rest_error), hipMemcpyDeviceToHost));
+    if (rest_error) return fail(EMB_E_NUMERIC, "electron substeps after the first were dropped: k_rest wait limit");
     return EMB_OK;
 }
 

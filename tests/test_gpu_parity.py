@@ -110,6 +110,57 @@ def test_odd_node_count_one_tick_matches_the_oracle(before):
         assert err[k] < 1e-5, err
 
 
+def fast_world(world, speed, seed):
+    """`world` with its electrons thrown in random directions at up to `speed`
+    cells per time unit along each axis: vmax dt / max_step > 1, so a tick takes
+    several electron substeps, each after a fresh cell list (physics.rs:124-135,
+    sim.rs:643-651)."""
+    w = world.copy()
+    w.vel = np.random.default_rng(seed).uniform(-speed, speed, size=w.vel.shape)
+    return w
+
+
+def expected_substeps(world):
+    s = world.settings
+    vmax = float(np.max(np.hypot(world.vel[:, 0], world.vel[:, 1])))
+    return int(min(max(np.ceil(vmax * s["dt"] / s["max_step"]), 1), s["max_substeps"]))
+
+
+@gpu
+@pytest.mark.parametrize("name,speed", [("scene_f1", 20.0), ("scene_f1", 60.0), ("scene_f6", 200.0), ("wire128", 30.0)])
+def test_fast_electrons_take_several_substeps(name, speed):
+    base = W.random_wire(128, 128, carriers_per_cell=8, seed=9) if name == "wire128" else fixture_world(name)
+    world = fast_world(base, speed, 3)
+    want = expected_substeps(world)
+    assert want > 1
+    cpu, sim = reset_and_compare(world)
+    assert sim.dims().last_substeps == want
+    pos, vel, _ = sim.read_state()
+    cpos, cvel, _ = cpu.state()
+    ex, ey, _ = sim.read_yee()
+    cex, cey, _ = cpu.yee()
+    err = {"pos": float(np.max(np.abs(pos - cpos))), "vel": relative_error(vel, cvel),
+           "ex": relative_error(ex, cex), "ey": relative_error(ey, cey),
+           "phi": relative_error(sim.read_potential(), cpu.potential_nodes())}
+    print(name, speed, want, world.n_electrons, err)
+    assert err["pos"] < 1e-4, err
+    assert err["vel"] < 1e-5, err
+    for k in ("ex", "ey", "phi"):
+        assert err[k] < 1e-5, err
+
+
+@gpu
+def test_fast_electrons_are_deterministic():
+    world = fast_world(W.random_wire(128, 128, carriers_per_cell=8, seed=9), 30.0, 4)
+    runs = []
+    for _ in range(2):
+        sim = gpusim()(world)
+        sim.run_ticks(3)
+        runs.append(sim.read_state()[:2] + sim.read_yee())
+    for a, b in zip(*runs):
+        np.testing.assert_array_equal(a, b)
+
+
 @gpu
 def test_fast_light_takes_three_fdtd_substeps():
     # c = 40: c dt sqrt 2 / 0.9 > 2, so three FDTD substeps per tick (maxwell.rs:376-378).
