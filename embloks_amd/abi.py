@@ -64,6 +64,30 @@ class EmbScene(C.Structure):
     ]
 
 
+class EmbProbe(C.Structure):
+    """The info panel's readings (Probe, crates/embloks-gpu/src/sim.rs:82-101)."""
+    _fields_ = [
+        ("valid", C.c_int32),
+        ("electrons", C.c_int32),
+        ("potential", C.c_double),
+        ("e", C.c_double * 2),
+        ("b", C.c_double),
+        ("s", C.c_double * 2),
+        ("current", C.c_double * 2),
+        ("wire_current", C.c_double),
+        ("wire_width", C.c_int32),
+        ("stack_state", C.c_int32),
+        ("v_low", C.c_double),
+        ("v_high", C.c_double),
+        ("stack_current", C.c_double),
+        ("gain", C.c_double),
+        ("has_mean_e", C.c_int32),
+        ("pad", C.c_int32),
+        ("mean_e_in_wires", C.c_double),
+        ("histograms", (C.c_uint32 * 256) * 3),
+    ]
+
+
 class EmbDims(C.Structure):
     _fields_ = [
         ("cols", C.c_int32),

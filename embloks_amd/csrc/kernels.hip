@@ -305,6 +305,7 @@ __global__ __launch_bounds__(TRIM_THREADS) void k_trim(Batch b) {
             if (tb == 0 && threadIdx.x == 0) {
                 S.has_gain[st] = 0;
                 S.gains[st] = 1.0f;
+                S.stack_read[3 * st + 2] = 0.0f;   // the readings stay as they were (fields.wgsl:321-324)
             }
             continue;
         }
@@ -396,6 +397,9 @@ __global__ __launch_bounds__(TRIM_THREADS) void k_trim(Batch b) {
             gain += S.trim_rate * (target - delivered) * target / (I.amplitude * I.amplitude);
             S.gains[st] = fminf(fmaxf(gain, S.gain_low), S.gain_high);
             S.has_gain[st] = 1;
+            S.stack_read[3 * st] = v_low;
+            S.stack_read[3 * st + 1] = v_high;
+            S.stack_read[3 * st + 2] = 1.0f;
         }
         __syncthreads();
     }
@@ -1939,6 +1943,133 @@ hipError_t deposit_kernel(const Batch &b, hipStream_t stream) {
 
 hipError_t diagnostics_kernel(const DevScene *scene, double c2, double *out, int blocks, hipStream_t stream) {
     k_diagnostics<<<blocks, 256, 0, stream>>>(scene, c2, out, blocks);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------------------
+// Info-panel probe (GpuSim::encode_probe, sim.rs:663-687; probe_cell and
+// probe_grid, fields.wgsl:368-466), one block over the state the last tick
+// left (display and current averages, the trim's readings, the last sort's
+// cell list). Output words as the reference's: 0 valid, 1 electrons in the
+// cell, 2 potential, 3-4 E, 5 B, 6-7 S = (E x B) / mu0, 8-9 current, 10-11
+// current across the wire and its width in cells, 12 stack state, 13-14 the
+// terminal potentials, 15 stack current, 16 gain, 17 mean |E| over the wire
+// cells, 18 their number; then 3 x PROBE_BINS histogram counts (E, B c^2, |S|
+// of the arrow cells) over log10 bins of 18 / PROBE_BINS from 1e-12.
+constexpr int PROBE_BINS = 256;
+
+__device__ __forceinline__ void probe_hist(unsigned *h, float m) {
+    if (m > 0.0f) {
+        const float x = (log2f(m) * 0.30103f + 12.0f) / 18.0f * (float)PROBE_BINS;
+        atomicAdd(&h[(int)fminf(fmaxf(x, 0.0f), (float)(PROBE_BINS - 1))], 1u);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_probe(const DevScene *scene, int cell, int stride, float c2, float inv_mu0,
+                                               float *out, unsigned *hist_out) {
+    __shared__ unsigned hist[3 * PROBE_BINS];
+    __shared__ float psum[256], pcnt[256];
+    const DevScene &S = *scene;
+    const int tid = threadIdx.x, cells = S.cols * S.rows;
+    for (int k = tid; k < 3 * PROBE_BINS; k += 256) hist[k] = 0u;
+    __syncthreads();
+    // probe_grid
+    const int st = max(stride, 1);
+    float sum = 0.0f, cnt = 0.0f;
+    for (int q = tid; q < cells; q += 256) {
+        const float ex = S.display[q], ey = S.display[cells + q], b = S.display[2 * cells + q];
+        const float e = sqrtf(ex * ex + ey * ey);
+        if (S.info[q].flags & F_WIRE) {
+            sum += e;
+            cnt += 1.0f;
+        }
+        probe_hist(hist + PROBE_BINS, fabsf(b) * c2);
+        const int c = q / S.rows, r = q - c * S.rows;
+        if (c % st == st / 2 && r % st == st / 2) {
+            probe_hist(hist, e);
+            probe_hist(hist + 2 * PROBE_BINS, inv_mu0 * e * fabsf(b));
+        }
+    }
+    psum[tid] = sum;
+    pcnt[tid] = cnt;
+    __syncthreads();
+    for (int half = 128; half > 0; half >>= 1) {
+        if (tid < half) {
+            psum[tid] += psum[tid + half];
+            pcnt[tid] += pcnt[tid + half];
+        }
+        __syncthreads();
+    }
+    for (int k = tid; k < 3 * PROBE_BINS; k += 256) hist_out[k] = hist[k];
+    if (tid != 0) return;
+    out[17] = psum[0] / fmaxf(pcnt[0], 1.0f);
+    out[18] = pcnt[0];
+    // probe_cell
+    for (int k = 0; k < 17; k++) out[k] = 0.0f;
+    if (cell < 0 || cell >= cells) return;
+    const int c = cell / S.rows, r = cell - c * S.rows;
+    out[0] = 1.0f;
+    out[1] = (float)(S.cstart[cell + 1] - S.cstart[cell]);
+    // potential_at (common.wgsl:104-125): node potential, ions, electrons nearby
+    float pot = S.phi[(size_t)(c + S.margin) * S.ny + (r + S.margin)];
+    if (S.close_on) {
+        pot += S.ion_pot[cell];
+        const int R = S.reach;
+        float near = 0.0f;
+        for (int cc = max(c - R, 0); cc <= min(c + R, S.cols - 1); cc++)
+            for (int rr = max(r - R, 0); rr <= min(r + R, S.rows - 1); rr++) {
+                const int nb = cc * S.rows + rr;
+                const float ox = (float)(cc - c) - 0.5f + S.half, oy = (float)(rr - r) - 0.5f + S.half;
+                for (int k = S.cstart[nb]; k < S.cstart[nb + 1]; k++)
+                    near += square_floor(S, 0, (ox + S.sfx[k]) * S.square_samples, (oy + S.sfy[k]) * S.square_samples);
+            }
+        pot += S.k_q * near;
+    }
+    out[2] = pot;
+    const float ex = S.display[cell], ey = S.display[cells + cell], b = S.display[2 * cells + cell];
+    out[3] = ex;
+    out[4] = ey;
+    out[5] = b;
+    out[6] = inv_mu0 * ey * b;
+    out[7] = -inv_mu0 * ex * b;
+    const float ix = S.cur_smooth[cell], iy = S.cur_smooth[cells + cell];
+    out[8] = ix;
+    out[9] = iy;
+    // Clamp meter: the main direction of flow, summed across the wire.
+    const int along = fabsf(ix) >= fabsf(iy) ? 0 : 1;
+    float total = along == 0 ? ix : iy, width = 1.0f;
+    const int dc = along == 1 ? 1 : 0, dr = along == 1 ? 0 : 1;
+    for (int side = -1; side <= 1; side += 2) {
+        int ac = c + side * dc, ar = r + side * dr;
+        while (ac >= 0 && ar >= 0 && ac < S.cols && ar < S.rows && (S.info[ac * S.rows + ar].flags & F_CONDUCTIVE)) {
+            total += S.cur_smooth[(size_t)along * cells + ac * S.rows + ar];
+            width += 1.0f;
+            ac += side * dc;
+            ar += side * dr;
+        }
+    }
+    out[10] = total;
+    out[11] = width;
+    const uint32_t flags = S.info[cell].flags;
+    const int stack = (flags & F_BATTERY) ? S.bats[(flags >> F_BAT_SHIFT) - 1].stack : -1;
+    if (stack >= 0) {
+        const StackInfo I = S.stacks[stack];
+        out[12] = (I.both_wired && S.stack_read[3 * stack + 2] != 0.0f) ? 2.0f : 1.0f;
+        out[13] = S.stack_read[3 * stack];
+        out[14] = S.stack_read[3 * stack + 1];
+        float isum = 0.0f;
+        for (int k = 0; k < I.cur_n; k++) {
+            const int sc = S.stack_cells[I.cur_off + k];
+            isum += S.cur_smooth[sc] * I.dir_x + S.cur_smooth[cells + sc] * I.dir_y;
+        }
+        out[15] = isum / (float)I.cur_n;
+        out[16] = S.has_gain[stack] ? S.gains[stack] : 1.0f;
+    }
+}
+
+hipError_t probe_kernel(const DevScene *scene, int cell, int stride, float c2, float inv_mu0, float *out, unsigned *hist,
+                        hipStream_t stream) {
+    k_probe<<<1, 256, 0, stream>>>(scene, cell, stride, c2, inv_mu0, out, hist);
     return hipGetLastError();
 }
 

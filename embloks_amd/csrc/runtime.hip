@@ -99,6 +99,7 @@ void layout(Bump &a, const Counts &c, DevScene &d, DevScene **d_scene, int32_t *
     d.has_gain = a.take<int32_t>(c.n_stacks);
     d.trim_part = a.take<float>(2 * (size_t)c.n_stacks * TRIM_BLOCKS);
     d.trim_done = a.take<int32_t>(c.n_stacks);
+    d.stack_read = a.take<float>(3 * (size_t)c.n_stacks);
     d.rest_sync = a.take<int32_t>(rest_sync_len(c.max_substeps));
     *tab = a.take<float>((size_t)TICK_TABLE * (c.n_ac + c.n_stacks));
     d.phase_tab = *tab;
@@ -428,6 +429,7 @@ struct EmbSim {
     Resident r;
     double *diag = nullptr;
     float2 *force = nullptr;   // emb_sim_close_force output, allocated on first use
+    float *probe = nullptr;    // emb_sim_probe output: 32 words + 3 x 256 counts, allocated on first use
 };
 
 extern "C" {
@@ -777,6 +779,47 @@ int emb_sim_read_charge_fixed(EmbSim *sim, int64_t *rho) {
     })
 }
 
+int emb_sim_probe(EmbSim *sim, int32_t c, int32_t r, int32_t arrow_stride, EmbProbe *out) {
+    EMB_GUARD({
+        if (!sim || !out) return fail(EMB_E_INVALID, "null argument");
+        Resident &res = sim->r;
+        EMB_HIP(hipSetDevice(sim->dev->ordinal));
+        constexpr int WORDS = 32, BINS = 256;
+        if (!sim->probe) EMB_HIP(hipMalloc(&sim->probe, (WORDS + 3 * BINS) * sizeof(float)));
+        const int cols = res.hs.cols, rows = res.hs.rows;
+        const int cell = (c >= 0 && r >= 0 && c < cols && r < rows) ? c * rows + r : -1;
+        const double c2 = res.s.speed_of_light * res.s.speed_of_light;
+        const double inv_mu0 = c2 / (4.0 * M_PI * res.s.coulomb_k);   // sim.rs:262
+        unsigned *hist = reinterpret_cast<unsigned *>(sim->probe + WORDS);
+        EMB_HIP(probe_kernel(res.d_scene, cell, arrow_stride, (float)c2, (float)inv_mu0, sim->probe, hist, sim->stream));
+        std::vector<float> w(WORDS);
+        std::memset(out, 0, sizeof(*out));
+        EMB_HIP(hipMemcpyAsync(w.data(), sim->probe, WORDS * sizeof(float), hipMemcpyDeviceToHost, sim->stream));
+        EMB_HIP(hipMemcpyAsync(out->histograms, hist, 3 * BINS * sizeof(unsigned), hipMemcpyDeviceToHost, sim->stream));
+        EMB_HIP(hipStreamSynchronize(sim->stream));
+        out->valid = w[0] != 0.0f;
+        out->electrons = (int32_t)w[1];
+        out->potential = w[2];
+        out->e[0] = w[3];
+        out->e[1] = w[4];
+        out->b = w[5];
+        out->s[0] = w[6];
+        out->s[1] = w[7];
+        out->current[0] = w[8];
+        out->current[1] = w[9];
+        out->wire_current = w[10];
+        out->wire_width = (int32_t)w[11];
+        out->stack_state = (int32_t)w[12];
+        out->v_low = w[13];
+        out->v_high = w[14];
+        out->stack_current = w[15];
+        out->gain = w[16];
+        out->mean_e_in_wires = w[17];
+        out->has_mean_e = w[18] > 0.0f;
+        return EMB_OK;
+    })
+}
+
 int emb_sim_diagnostics(EmbSim *sim, double *gauss_residual, double *electron_charge, double *field_energy) {
     EMB_GUARD({
         if (!sim) return fail(EMB_E_INVALID, "null sim");
@@ -888,6 +931,7 @@ int emb_sim_destroy(EmbSim *sim) {
     (void)hipStreamSynchronize(sim->stream);
     free_resident(sim->r);
     if (sim->diag) (void)hipFree(sim->diag);
+    if (sim->probe) (void)hipFree(sim->probe);
     if (sim->force) (void)hipFree(sim->force);
     (void)hipStreamDestroy(sim->stream);
     delete sim;

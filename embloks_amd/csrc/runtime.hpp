@@ -111,6 +111,8 @@ hipError_t deposit_kernel(const Batch &b, hipStream_t stream);
 hipError_t close_force_kernel(const Batch &b, float2 *out, hipStream_t stream);
 hipError_t finish_sync_kernel(const Batch &b, hipStream_t stream);
 hipError_t diagnostics_kernel(const DevScene *scene, double c2, double *out, int blocks, hipStream_t stream);
+hipError_t probe_kernel(const DevScene *scene, int cell, int stride, float c2, float inv_mu0, float *out, unsigned *hist,
+                        hipStream_t stream);
 
 // Write the per-tick AC phases and EMF targets of ticks [r.tick, r.tick + count)
 // into the pinned staging and enqueue the copy. Advances r.time and r.tick.

@@ -45,6 +45,7 @@ SIGNATURES = {
     "emb_sim_kernel_times": (C.c_int, [VP, C.c_int, D, C.c_int]),
     "emb_sim_close_force": (C.c_int, [VP, C.c_int32, D, D]),
     "emb_tick_phase_name": (C.c_char_p, [C.c_int]),
+    "emb_sim_probe": (C.c_int, [VP, C.c_int32, C.c_int32, C.c_int32, C.POINTER(abi.EmbProbe)]),
     "emb_sim_destroy": (C.c_int, [VP]),
     "emb_server_create": (C.c_int, [VP, C.POINTER(abi.EmbSettings), C.POINTER(abi.EmbServerConfig), C.POINTER(VP)]),
     "emb_server_submit": (C.c_int, [VP, C.POINTER(abi.EmbScene), C.c_int64, C.POINTER(C.c_int64)]),
@@ -214,6 +215,22 @@ class GpuSim:
         out = np.zeros((self.cols, self.rows), dtype=np.int64)
         check(lib().emb_sim_read_charge_fixed(self.h, out.ctypes.data_as(I64)))
         return out
+
+    def read_probe(self, cell=None, arrow_stride: int = 2) -> dict:
+        """GpuSim::read_probe (sim.rs:663-715): the info panel's readings at
+        visible cell `cell` = (c, r) or None, as Probe's fields."""
+        p = abi.EmbProbe()
+        c, r = cell if cell is not None else (-1, -1)
+        check(lib().emb_sim_probe(self.h, int(c), int(r), int(arrow_stride), C.byref(p)))
+        return {
+            "cell": tuple(cell) if (cell is not None and p.valid) else None,
+            "electrons": p.electrons, "potential": p.potential,
+            "e": (p.e[0], p.e[1]), "b": p.b, "s": (p.s[0], p.s[1]), "current": (p.current[0], p.current[1]),
+            "wire_current": (p.wire_current, p.wire_width), "stack_state": p.stack_state,
+            "v_low": p.v_low, "v_high": p.v_high, "stack_current": p.stack_current, "gain": p.gain,
+            "mean_e_in_wires": p.mean_e_in_wires if p.has_mean_e else None,
+            "histograms": [np.array(p.histograms[k][:], dtype=np.uint32) for k in range(3)],
+        }
 
     def diagnostics(self):
         """(Gauss residual max |div E - 4 pi k rho|, total electron charge, field energy), FP64."""

@@ -98,6 +98,24 @@ typedef struct EmbDims {
     double time;
 } EmbDims;
 
+/* The info panel's readings (Probe, crates/embloks-gpu/src/sim.rs:82-101). */
+typedef struct EmbProbe {
+    int32_t valid;              /* a visible cell was probed (Probe::cell is Some) */
+    int32_t electrons;          /* electrons in the cell */
+    double potential;           /* Simulation::potential at the cell, close range included */
+    double e[2], b, s[2];       /* displayed E and B, S = (E x B) / mu0 */
+    double current[2];          /* smoothed cell current */
+    double wire_current;        /* Probe::wire_current: flow summed across the wire ... */
+    int32_t wire_width;         /* ... and the wire's width in cells */
+    int32_t stack_state;        /* 0 not a stacked battery cell, 1 not connected, 2 connected */
+    double v_low, v_high;       /* the stack's terminal potentials at its last trim */
+    double stack_current, gain;
+    int32_t has_mean_e;         /* the scene has wire cells */
+    int32_t pad;
+    double mean_e_in_wires;
+    uint32_t histograms[3][256];  /* |E|, |B| c^2, |S| (arrow cells): counts per log10 bin of width 18/256 from 1e-12 */
+} EmbProbe;
+
 typedef struct EmbDevice EmbDevice;
 typedef struct EmbSim EmbSim;
 
@@ -170,6 +188,11 @@ int emb_sim_kernel_times(EmbSim *sim, int ticks, double *avg_ms, int capacity);
  * stream; *avg_ms (if not NULL) gets the mean time per evaluation. */
 int emb_sim_close_force(EmbSim *sim, int32_t repeats, double *force, double *avg_ms);
 const char *emb_tick_phase_name(int phase);
+/* GpuSim::read_probe (sim.rs:663-715; probe_cell and probe_grid,
+ * fields.wgsl:368-466): the readings at visible cell (c, r) (c < 0: none) and
+ * the grid's mean |E| in wires and magnitude histograms, the S histogram over
+ * the cells (c, r) with c % arrow_stride == r % arrow_stride == arrow_stride / 2. */
+int emb_sim_probe(EmbSim *sim, int32_t c, int32_t r, int32_t arrow_stride, EmbProbe *out);
 int emb_sim_destroy(EmbSim *sim);
 
 /* --- Continuous-batching serving loop (no counterpart in the reference) ----------
