@@ -484,21 +484,6 @@ struct GlobalNeighbours {
     }
 };
 
-struct LdsNeighbours {
-    const int2 *lp;              // LDS: staged positions, window frame
-    const int *lb, *le;          // LDS: per window cell, range in lp
-    int wc0, wr0, W;
-    __device__ void range(int cc, int rr, int &b, int &e) const {
-        const int w = (cc - wc0) * W + (rr - wr0);
-        b = lb[w];
-        e = le[w];
-    }
-    __device__ int2 p(int j) const { return lp[j]; }
-    __device__ int2 origin(int hc, int hr, float px, float py, int, int) const {
-        return fixed_pos(hc - wc0, hr - wr0, px, py);
-    }
-};
-
 // Lorentz force of the gathered fields plus the battery's impressed field, as an
 // acceleration (physics.rs:49-51, 139-149; battery.rs:210-228).
 __device__ __forceinline__ void field_accel(const DevScene &S, int hc, int hr, float px, float py, float vx, float vy,
@@ -568,6 +553,29 @@ __device__ __forceinline__ void ion_uniform(const DevScene &S, int home, float p
     const float w00 = (1.0f - fu) * (1.0f - fv), w10 = fu * (1.0f - fv), w01 = (1.0f - fu) * fv, w11 = fu * fv;
     fx += coef * (tab[0] * w00 + tab[ION_SUM * 2] * w10 + tab[2] * w01 + tab[ION_SUM * 2 + 2] * w11);
     fy += coef * (tab[1] * w00 + tab[ION_SUM * 2 + 1] * w10 + tab[3] * w01 + tab[ION_SUM * 2 + 3] * w11);
+}
+
+// Cell-mode positions: a neighbour is held as (x fraction within its cell, y
+// relative to the window's centre row yc), so its x is the snapshot's FP32
+// fraction exactly and |y| < 8 cells resolves 5e-7 cells or better. A run's
+// neighbours share a column, so the home electron's x offset from that column
+// (fraction + column difference, |x| < 4: 2.4e-7 cells) is formed once per run
+// and the pair offset is one FP32 subtraction per axis. An electron and its own
+// staged copy give the same bits (its self term is g(0) * 0).
+__device__ __forceinline__ float2 cell_pos(int rr, int yc, float fx, float fy) {
+    return make_float2(fx, (float)(rr - yc) + fy);
+}
+
+// pair_term (below) over FP32 offsets in cells; gs = g_scale turns the squared
+// offset into the table coordinate and the sums are in cells (k_pair scales
+// them to a force).
+__device__ __forceinline__ void pair_term_c(const lerp2 *gt, float gs, float dx, float dy, float &sx, float &sy) {
+    const float d2 = (dx * dx + dy * dy) * gs;
+    const float tt = fminf(d2, (float)(PAIR_SAMPLES + 1));
+    const lerp2 g = gt[(int)tt];
+    const float l = g.x + g.y * __builtin_amdgcn_fractf(tt);
+    sx += l * dx;
+    sy += l * dy;
 }
 
 // One pair term g(r^2) d, g lerped from the r^2-uniform table
@@ -702,10 +710,10 @@ __device__ __forceinline__ int lanes_per(int n) {
 }
 
 // Column runs of the cell mode: the electrons of rows r_lo..r_hi of column cc,
-// contiguous in the sorted order, with fixed-point positions relative to the
-// window's corner cell (wc0, wr0). Staged in LDS, or read from the snapshot.
+// contiguous in the sorted order, with cell_pos positions (yc = wr0 + W / 2).
+// Staged in LDS, or read from the snapshot.
 struct LdsRuns {
-    const int2 *lp;
+    const float2 *lp;
     const int *lb, *le;
     int wc0, wr0, W;
     __device__ void run(int cc, int r_lo, int r_hi, int &jb, int &je) const {
@@ -713,7 +721,7 @@ struct LdsRuns {
         jb = lb[w + r_lo - wr0];
         je = le[w + r_hi - wr0];
     }
-    __device__ int2 at(int j, int) const { return lp[j]; }
+    __device__ float2 at(int j, int) const { return lp[j]; }
 };
 
 struct GlobalRuns {
@@ -723,9 +731,10 @@ struct GlobalRuns {
         jb = S.cstart[cc * S.rows + r_lo];
         je = S.cstart[cc * S.rows + r_hi + 1];
     }
-    __device__ int2 at(int j, int cc) const {
+    int yc;
+    __device__ float2 at(int j, int cc) const {
         const int rr = S.scell[j] - cc * S.rows;
-        return fixed_pos(cc - wc0, rr - wr0, S.sfx[j], S.sfy[j]);
+        return cell_pos(rr, yc, S.sfx[j], S.sfy[j]);
     }
 };
 
@@ -751,14 +760,14 @@ __device__ __forceinline__ void cell_item(const DevScene &S, const RUNS &runs, c
     static_assert(E == 1 || E == 2, "one or two electrons per lane");
     const int lane = threadIdx.x & 63;
     const int R = S.reach;
-    const float s2 = S.pair_s2;
     const int home = hc * S.rows + hr;
     const int I = item_lanes(n, E, E == 2 ? min(imax, 32) : imax), shift = __builtin_ctz(I), SL = 64 >> shift;
     const int i = lane & (I - 1), sl = lane >> shift;
     int slot[E];
     bool live[E];
-    float px[E], py[E];
-    int2 b[E];
+    const float gs = S.g_scale;
+    const int yc = wr0 + (W >> 1);
+    float px[E], by[E];
     float sx[E], sy[E];
 #pragma unroll
     for (int e = 0; e < E; e++) {
@@ -766,8 +775,7 @@ __device__ __forceinline__ void cell_item(const DevScene &S, const RUNS &runs, c
         live[e] = ei < n;
         slot[e] = e0 + (live[e] ? ei : n - 1);
         px[e] = S.sfx[slot[e]];
-        py[e] = S.sfy[slot[e]];
-        b[e] = fixed_pos(hc - wc0, hr - wr0, px[e], py[e]);
+        by[e] = cell_pos(hr, yc, px[e], S.sfy[slot[e]]).y;
         sx[e] = 0.0f;
         sy[e] = 0.0f;
     }
@@ -778,28 +786,32 @@ __device__ __forceinline__ void cell_item(const DevScene &S, const RUNS &runs, c
         runs.run(cc, r_lo, r_hi, jb, je);
         jb = __builtin_amdgcn_readfirstlane(jb);
         je = __builtin_amdgcn_readfirstlane(je);
+        float bx[E];   // x offset of the home electrons from column cc's cell corners
+#pragma unroll
+        for (int e = 0; e < E; e++) bx[e] = px[e] + (float)(hc - cc);
         int jj = jb;
         for (; jj + 2 * SL <= je; jj += 2 * SL) {
-            const int2 q0 = runs.at(jj + sl, cc), q1 = runs.at(jj + SL + sl, cc);
+            const float2 q0 = runs.at(jj + sl, cc), q1 = runs.at(jj + SL + sl, cc);
 #pragma unroll
             for (int e = 0; e < E; e++) {
-                pair_term(gt, s2, b[e].x - q0.x, b[e].y - q0.y, sx[e], sy[e]);
-                pair_term(gt, s2, b[e].x - q1.x, b[e].y - q1.y, sx[e], sy[e]);
+                pair_term_c(gt, gs, bx[e] - q0.x, by[e] - q0.y, sx[e], sy[e]);
+                pair_term_c(gt, gs, bx[e] - q1.x, by[e] - q1.y, sx[e], sy[e]);
             }
         }
         for (; jj < je; jj += SL) {
             // past the run's end a lane contributes nothing
-            const int2 q = runs.at(min(jj + sl, je - 1), cc);
+            const float2 q = runs.at(min(jj + sl, je - 1), cc);
             const bool in = jj + sl < je;
 #pragma unroll
             for (int e = 0; e < E; e++)
-                pair_term(gt, s2, in ? b[e].x - q.x : 0, in ? b[e].y - q.y : 0, sx[e], sy[e]);
+                pair_term_c(gt, gs, in ? bx[e] - q.x : 0.0f, in ? by[e] - q.y : 0.0f, sx[e], sy[e]);
         }
     }
     // From here on a lane works for electron e = sl % E (the ion terms) and, if
     // sl < E, finishes it.
     const int me = E == 2 ? (sl & 1) : 0;
-    const float mpx = E == 2 && me ? px[E - 1] : px[0], mpy = E == 2 && me ? py[E - 1] : py[0];
+    // (the y fraction is read again rather than kept through the pair loop)
+    const float mpx = E == 2 && me ? px[E - 1] : px[0], mpy = S.sfy[E == 2 && me ? slot[E - 1] : slot[0]];
     const CellInfo info = S.info[home];
     const bool uniform = (info.flags & F_ION_UNIFORM) != 0u;
     float fx = 0.0f, fy = 0.0f;
@@ -827,8 +839,8 @@ __device__ __forceinline__ void cell_item(const DevScene &S, const RUNS &runs, c
     if (sl < E && mlive) {
         const int mslot = E == 2 && me ? slot[E - 1] : slot[0];
         if (uniform) ion_uniform(S, home, mpx, mpy, fx, fy);
-        fx += S.k_pair_fx * (E == 2 && me ? sx[E - 1] : sx[0]);
-        fy += S.k_pair_fx * (E == 2 && me ? sy[E - 1] : sy[0]);
+        fx += S.k_pair * (E == 2 && me ? sx[E - 1] : sx[0]);
+        fy += S.k_pair * (E == 2 && me ? sy[E - 1] : sy[0]);
         if (MODE == ITEM_FORCE) {
             force_out[S.order[mslot]] = make_float2(fx, fy);
         } else if (MODE == ITEM_STORE) {
@@ -914,7 +926,7 @@ template <int MODE, int THREADS>
 __device__ __forceinline__ void push_tiled_body(Batch b, int k, float2 *force_out) {
     constexpr bool FORCE = MODE == ITEM_FORCE;
     constexpr int WAVES = THREADS / 64;
-    __shared__ int2 lp[PUSH_STAGE];
+    __shared__ float2 lp[PUSH_STAGE];
     __shared__ lerp2 gt[PAIR_SAMPLES + 4];
     __shared__ int lb[PUSH_WIN * PUSH_WIN], le[PUSH_WIN * PUSH_WIN];
     __shared__ float dens_w[PUSH_WIN * PUSH_WIN];               // ion density of the window's cells
@@ -1017,7 +1029,7 @@ __device__ __forceinline__ void push_tiled_body(Batch b, int k, float2 *force_ou
                         if (q + step < cb && col_base[q + step] - base <= j) q += step;
                     const int g = col_g0[q] + (j - (col_base[q] - base));
                     const int rr = S.scell[g] - (wc0 + q) * S.rows;
-                    lp[j] = fixed_pos(q, rr - wr0, S.sfx[g], S.sfy[g]);
+                    lp[j] = cell_pos(rr, wr0 + (W >> 1), S.sfx[g], S.sfy[g]);
                 }
             }
             __syncthreads();
@@ -1039,7 +1051,7 @@ __device__ __forceinline__ void push_tiled_body(Batch b, int k, float2 *force_ou
                     else
                         cell_item<MODE, 1>(S, runs, gt, dens_w, hc, hr, e0, n, chunk, wc0, wr0, W, k, h, t, force_out);
                 } else {
-                    const GlobalRuns runs{S, wc0, wr0};
+                    const GlobalRuns runs{S, wc0, wr0, wr0 + (W >> 1)};
                     if (item_e(n) == 2)
                         cell_item<MODE, 2>(S, runs, gt, dens_w, hc, hr, e0, n, chunk, wc0, wr0, W, k, h, t, force_out);
                     else
@@ -1105,7 +1117,7 @@ constexpr int HEAVY_WAVES = HEAVY_THREADS / 64;
 
 template <bool FORCE>
 __global__ __launch_bounds__(HEAVY_THREADS) void k_push_heavy(Batch b, int k, float2 *force_out) {
-    __shared__ int2 lp[PUSH_STAGE];
+    __shared__ float2 lp[PUSH_STAGE];
     __shared__ lerp2 gt[PAIR_SAMPLES + 4];
     __shared__ int lb[HEAVY_WIN * HEAVY_WIN], le[HEAVY_WIN * HEAVY_WIN];
     __shared__ float dens_w[HEAVY_WIN * HEAVY_WIN];
@@ -1171,7 +1183,7 @@ __global__ __launch_bounds__(HEAVY_THREADS) void k_push_heavy(Batch b, int k, fl
                         if (q + step < W && col_base[q + step] <= j) q += step;
                     const int g = col_g0[q] + (j - col_base[q]);
                     const int rr = S.scell[g] - (wc0 + q) * S.rows;
-                    lp[j] = fixed_pos(q, rr - wr0, S.sfx[g], S.sfy[g]);
+                    lp[j] = cell_pos(rr, wr0 + (W >> 1), S.sfx[g], S.sfy[g]);
                 }
             }
             __syncthreads();
@@ -1184,7 +1196,7 @@ __global__ __launch_bounds__(HEAVY_THREADS) void k_push_heavy(Batch b, int k, fl
                     cell_item<MODE, 2>(S, LdsRuns{lp, lb, le, wc0, wr0, W}, gt, dens_w, hc, hr, e0, n, ch, wc0, wr0, W, k,
                                        h, t, force_out, HEAVY_I);
                 else
-                    cell_item<MODE, 2>(S, GlobalRuns{S, wc0, wr0}, gt, dens_w, hc, hr, e0, n, ch, wc0, wr0, W, k, h, t,
+                    cell_item<MODE, 2>(S, GlobalRuns{S, wc0, wr0, wr0 + (W >> 1)}, gt, dens_w, hc, hr, e0, n, ch, wc0, wr0, W, k, h, t,
                                        force_out, HEAVY_I);
             }
             __syncthreads();
