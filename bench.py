@@ -286,8 +286,11 @@ def run_gauss_log(ticks: int = 10000, every: int = 100):
         sim.step(every)
         r, _q, _e = sim.diagnostics()
         res.append(r)
-        # the integer deposit: -2^44 per electron, exactly
-        dev = int(sim.read_charge_fixed().sum()) + sim.n * (1 << 44)
+        # the integer deposit: -2^44 per electron, exactly; the total passes the
+        # int64 range, so the cells are summed modulo 2^64 and compared there
+        wrapped = int(sim.read_charge_fixed().view(np.uint64).sum(dtype=np.uint64))
+        dev = (wrapped + sim.n * (1 << 44)) % (1 << 64)
+        dev = dev - (1 << 64) if dev >= (1 << 63) else dev
         charge_exact += int(dev == 0)
         charge_dev = max(charge_dev, abs(dev))
     elapsed = time.perf_counter() - t0

@@ -842,9 +842,10 @@ int emb_sim_diagnostics(EmbSim *sim, double *gauss_residual, double *electron_ch
             std::vector<int64_t> rho(cells);
             int rc = emb_sim_read_charge_fixed(sim, rho.data());
             if (rc != EMB_OK) return rc;
-            long long total = 0;
+            // In electron charges: the cells sum to exactly -N 2^44, which passes
+            // the int64 range beyond 2^19 electrons, so the sum is 128-bit.
+            __int128 total = 0;
             for (auto v : rho) total += v;
-            // In electron charges: the sum of integer weights is exactly -N * 2^40.
             *electron_charge = (double)total / RHO_SCALE * (r.s.cell_charge / r.s.carriers_per_cell);
         }
         return EMB_OK;

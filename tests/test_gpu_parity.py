@@ -241,6 +241,20 @@ def test_deposit_is_bit_exact_and_conserves_charge(index):
 
 
 @gpu
+def test_total_charge_is_exact_past_the_int64_range():
+    # 1.01M electrons: their deposit sums to -N 2^44, beyond int64 from 2^19 electrons on
+    world = W.rectangles_circuit(512, carriers_per_cell=16, seed=2)
+    sim = gpusim()(world)
+    assert sim.n > (1 << 19)
+    sim.run_ticks(2)
+    _, charge, _ = sim.diagnostics()
+    q0 = world.settings["cell_charge"] / world.settings["carriers_per_cell"]
+    assert charge == -sim.n * q0
+    wrapped = int(sim.read_charge_fixed().view(np.uint64).sum(dtype=np.uint64))
+    assert wrapped == (-sim.n * (1 << 44)) % (1 << 64)
+
+
+@gpu
 def test_gauss_law_holds_after_ticks():
     world = W.scene(0)
     sim = gpusim()(world)
