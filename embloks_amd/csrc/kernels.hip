@@ -998,13 +998,15 @@ __device__ __forceinline__ void push_tiled_body(Batch b, int k, float2 *force_ou
             const bool in = cc >= 0 && cc < S.cols && rr >= 0 && rr < S.rows;
             dens_w[q] = in ? S.density[cc * S.rows + rr] : 0.0f;
         }
-        // The whole window in one pass when it fits the staging; else one pass
-        // per column of home cells, whose window is only 2 reach + 1 columns wide
-        // (reading from global memory only if even that does not fit).
-        const int passes = col_base[W] <= PUSH_STAGE ? 1 : PUSH_TILE;
-        for (int pass = 0; pass < passes; pass++) {
-            const int ca = passes == 1 ? 0 : pass, cb = passes == 1 ? W : min(pass + 2 * R + 1, W);
-            const int la = passes == 1 ? 0 : pass * PUSH_TILE, lz = passes == 1 ? PUSH_CELLS : la + PUSH_TILE;
+        // As many consecutive columns of home cells per pass as the staging holds
+        // (home columns [a, b) need window columns [a, b + 2 reach)): the whole
+        // window in one pass when it fits; a single column that does not fit
+        // reads its neighbours from global memory.
+        for (int a = 0, b = 0; a < PUSH_TILE; a = b) {
+            b = a + 1;
+            while (b < PUSH_TILE && col_base[min(b + 1 + 2 * R, W)] - col_base[a] <= PUSH_STAGE) b++;
+            const int ca = a, cb = min(b + 2 * R, W);
+            const int la = a * PUSH_TILE, lz = b * PUSH_TILE;
             const int base = col_base[ca], staged_n = col_base[cb] - base;
             const bool staged = staged_n <= PUSH_STAGE;
             if (staged) {
