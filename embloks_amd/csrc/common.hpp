@@ -57,7 +57,7 @@ constexpr int CELL_MODE_MIN = 6;  // a push tile with a cell this crowded runs t
 constexpr int HEAVY_CELL = 64;    // a cell this crowded is pushed by k_push_heavy, spread over blocks
 constexpr int TRIM_BLOCKS = 8;    // blocks per battery stack in k_trim
 constexpr int REST_PHASES = 7;    // work phases of one extra electron substep in k_rest
-constexpr int REST_THREADS = 256; // threads per k_rest block
+constexpr int REST_THREADS = 1024; // threads per k_rest block (the push tiles' layout)
 constexpr int FDTD_STRIP = 60;    // corner columns per FDTD wave (64 lanes: two halo lanes each side)
 constexpr int FDTD_WAVES = 4;     // FDTD waves per block (adjacent strips)
 constexpr int FDTD_AHEAD = 4;     // corner rows an FDTD wave has loads in flight for
@@ -222,7 +222,8 @@ enum KernelClass {
     KC_SCENE,           // one block per scene
     KC_PUSH,            // PUSH_TILE x PUSH_TILE cell tiles
     KC_HEAVY,           // blocks sharing the scene's heavy cells (one per 16k electrons, at most 256)
-    KC_REST,            // k_rest work-queue blocks (one per 2048 electrons, at most 1024)
+    KC_REST,            // k_rest work-queue blocks (one per 1024 electrons, at most 256: launching more costs
+                        // ticks that need no second substep more than it gains)
     KC_COUNT
 };
 

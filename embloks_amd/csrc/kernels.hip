@@ -922,19 +922,146 @@ constexpr int PUSH_CELLS = PUSH_TILE * PUSH_TILE;
 constexpr int PUSH_THREADS = 1024;
 constexpr int FORCE_THREADS = 512;
 
+// LDS of the cell-mode tile body (the positions, the pair table, the window's
+// runs and densities, the tile's work items).
+struct PushTileLds {
+    float2 lp[PUSH_STAGE];
+    lerp2 gt[PAIR_SAMPLES + 4];
+    int lb[PUSH_WIN * PUSH_WIN], le[PUSH_WIN * PUSH_WIN];
+    float dens_w[PUSH_WIN * PUSH_WIN];               // ion density of the window's cells
+    int cell_e0[PUSH_CELLS], cell_n[PUSH_CELLS];     // tile cells: first sorted slot, electrons
+    int col_base[PUSH_WIN + 1], col_g0[PUSH_WIN];   // LDS base and first global slot per column
+    int item_pref[PUSH_CELLS + 1];                   // cell mode: prefix of work items per tile cell
+};
+
+// One push tile in cell mode (every thread of the block; the pair table must be
+// in L.gt). With HEAVY, cells of HEAVY_CELL or more electrons are pushed here
+// too (the tick leaves them to k_push_heavy).
+template <int MODE, int THREADS, bool HEAVY>
+__device__ __forceinline__ void push_tile(const DevScene &S, PushTileLds &L, int local, int k, float h, uint32_t t,
+                                          float2 *force_out) {
+    constexpr int WAVES = THREADS / 64;
+    float2 *lp = L.lp;
+    const lerp2 *gt = L.gt;
+    int *lb = L.lb, *le = L.le, *cell_e0 = L.cell_e0, *cell_n = L.cell_n, *col_base = L.col_base, *col_g0 = L.col_g0;
+    int *item_pref = L.item_pref;
+    float *dens_w = L.dens_w;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int tiles_r = (S.rows + PUSH_TILE - 1) / PUSH_TILE;
+    const int tc0 = (local / tiles_r) * PUSH_TILE, tr0 = (local % tiles_r) * PUSH_TILE;
+    const int tc1 = min(tc0 + PUSH_TILE, S.cols), tr1 = min(tr0 + PUSH_TILE, S.rows);
+    const int R = S.close_on ? S.reach : 0;
+    const int W = PUSH_TILE + 2 * R;
+    const int wc0 = tc0 - R, wr0 = tr0 - R;
+    const int rlo = max(wr0, 0), rhi = min(wr0 + W, S.rows) - 1;
+    // Wave 0 lays out the window (column runs, lane = column) and the tile's
+    // work (lane = tile cell, column-major).
+    if (wave == 0) {
+        const int cc = wc0 + lane;
+        int g0 = 0, n = 0;
+        if (lane < W && cc >= 0 && cc < S.cols && rlo <= rhi) {
+            g0 = S.cstart[cc * S.rows + rlo];
+            n = S.cstart[cc * S.rows + rhi + 1] - g0;
+        }
+        const int incl = wave_scan_incl(n);
+        if (lane < W) {
+            col_base[lane] = incl - n;
+            col_g0[lane] = g0;
+        }
+        if (lane == W - 1) col_base[W] = incl;
+        const int hc = tc0 + lane / PUSH_TILE, hr = tr0 + lane % PUSH_TILE;
+        int m = 0, first = 0;
+        if (hc < tc1 && hr < tr1) {
+            const int cell = hc * S.rows + hr;
+            first = S.cstart[cell];
+            m = S.cstart[cell + 1] - first;
+        }
+        cell_e0[lane] = first;
+        cell_n[lane] = m;
+        const int E = item_e(m), I = item_lanes(m, E, E == 2 ? 32 : 64);
+        const int items = (m > 0 && (HEAVY || m < HEAVY_CELL)) ? (m + E * I - 1) / (E * I) : 0;
+        const int item_incl = wave_scan_incl(items);
+        item_pref[lane] = item_incl - items;
+        if (lane == 63) item_pref[PUSH_CELLS] = item_incl;
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < W * W; q += THREADS) {
+        const int ci = q / W, ri = q - (q / W) * W;
+        const int cc = wc0 + ci, rr = wr0 + ri;
+        const bool in = cc >= 0 && cc < S.cols && rr >= 0 && rr < S.rows;
+        dens_w[q] = in ? S.density[cc * S.rows + rr] : 0.0f;
+    }
+    // As many consecutive columns of home cells per pass as the staging holds
+    // (home columns [a, b) need window columns [a, b + 2 reach)): the whole
+    // window in one pass when it fits; a single column that does not fit
+    // reads its neighbours from global memory.
+    for (int a = 0, b = 0; a < PUSH_TILE; a = b) {
+        b = a + 1;
+        while (b < PUSH_TILE && col_base[min(b + 1 + 2 * R, W)] - col_base[a] <= PUSH_STAGE) b++;
+        const int ca = a, cb = min(b + 2 * R, W);
+        const int la = a * PUSH_TILE, lz = b * PUSH_TILE;
+        const int base = col_base[ca], staged_n = col_base[cb] - base;
+        const bool staged = staged_n <= PUSH_STAGE;
+        if (staged) {
+            for (int q = threadIdx.x; q < (cb - ca) * W; q += THREADS) {
+                const int ci = ca + q / W, ri = q - (q / W) * W;
+                const int cc = wc0 + ci, rr = wr0 + ri;
+                const int w = ci * W + ri;
+                if (cc < 0 || cc >= S.cols || rr < 0 || rr >= S.rows) {
+                    lb[w] = 0;
+                    le[w] = 0;
+                } else {
+                    const int cell = cc * S.rows + rr;
+                    lb[w] = col_base[ci] - base + (S.cstart[cell] - col_g0[ci]);
+                    le[w] = col_base[ci] - base + (S.cstart[cell + 1] - col_g0[ci]);
+                }
+            }
+            // One pass over every staged electron (column found by search), so a
+            // sparse window costs one round of loads rather than one per column.
+            for (int j = threadIdx.x; j < staged_n; j += THREADS) {
+                int q = ca;
+                for (int step = PUSH_WIN / 2; step > 0; step >>= 1)
+                    if (q + step < cb && col_base[q + step] - base <= j) q += step;
+                const int g = col_g0[q] + (j - (col_base[q] - base));
+                const int rr = S.scell[g] - (wc0 + q) * S.rows;
+                lp[j] = cell_pos(rr, wr0 + (W >> 1), S.sfx[g], S.sfy[g]);
+            }
+        }
+        __syncthreads();
+        for (int it = item_pref[la] + wave; it < item_pref[lz]; it += WAVES) {
+            int lo = la, hi = lz;
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (item_pref[mid] <= it) lo = mid;
+                else hi = mid;
+            }
+            lo = __builtin_amdgcn_readfirstlane(lo);
+            const int hc = tc0 + lo / PUSH_TILE, hr = tr0 + lo % PUSH_TILE;
+            const int chunk = it - item_pref[lo];
+            const int e0 = __builtin_amdgcn_readfirstlane(cell_e0[lo]), n = __builtin_amdgcn_readfirstlane(cell_n[lo]);
+            if (staged) {
+                const LdsRuns runs{lp, lb, le, wc0, wr0, W};
+                if (item_e(n) == 2)
+                    cell_item<MODE, 2>(S, runs, gt, dens_w, hc, hr, e0, n, chunk, wc0, wr0, W, k, h, t, force_out);
+                else
+                    cell_item<MODE, 1>(S, runs, gt, dens_w, hc, hr, e0, n, chunk, wc0, wr0, W, k, h, t, force_out);
+            } else {
+                const GlobalRuns runs{S, wc0, wr0, wr0 + (W >> 1)};
+                if (item_e(n) == 2)
+                    cell_item<MODE, 2>(S, runs, gt, dens_w, hc, hr, e0, n, chunk, wc0, wr0, W, k, h, t, force_out);
+                else
+                    cell_item<MODE, 1>(S, runs, gt, dens_w, hc, hr, e0, n, chunk, wc0, wr0, W, k, h, t, force_out);
+            }
+        }
+        __syncthreads();
+    }
+}
+
 template <int MODE, int THREADS>
 __device__ __forceinline__ void push_tiled_body(Batch b, int k, float2 *force_out) {
     constexpr bool FORCE = MODE == ITEM_FORCE;
-    constexpr int WAVES = THREADS / 64;
-    __shared__ float2 lp[PUSH_STAGE];
-    __shared__ lerp2 gt[PAIR_SAMPLES + 4];
-    __shared__ int lb[PUSH_WIN * PUSH_WIN], le[PUSH_WIN * PUSH_WIN];
-    __shared__ float dens_w[PUSH_WIN * PUSH_WIN];               // ion density of the window's cells
-    __shared__ int cell_e0[PUSH_CELLS], cell_n[PUSH_CELLS];     // tile cells: first sorted slot, electrons
-    __shared__ int col_base[PUSH_WIN + 1], col_g0[PUSH_WIN];   // LDS base and first global slot per column
-    __shared__ int item_pref[PUSH_CELLS + 1];                   // cell mode: prefix of work items per tile cell
+    __shared__ PushTileLds L;
     bool table_loaded = false;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     for (int blk = blockIdx.x; blk < b.total[KC_PUSH]; blk += gridDim.x) {
         const int s = scene_of(b, KC_PUSH, blk);
         const DevScene &S = b.scenes[s];
@@ -951,117 +1078,10 @@ __device__ __forceinline__ void push_tiled_body(Batch b, int k, float2 *force_ou
         if (!FORCE && k == 0 && threadIdx.x == 0 && local == 0) S.ctr->substeps = sub;
         if (!S.close_on || !S.tile_dense[local]) continue;
         if (!table_loaded) {   // the pair table is shared by every scene
-            for (int q = threadIdx.x; q < PAIR_SAMPLES + 4; q += THREADS) gt[q] = S.g_lerp[q];
+            for (int q = threadIdx.x; q < PAIR_SAMPLES + 4; q += THREADS) L.gt[q] = S.g_lerp[q];
             table_loaded = true;
         }
-        const int tiles_r = (S.rows + PUSH_TILE - 1) / PUSH_TILE;
-        const int tc0 = (local / tiles_r) * PUSH_TILE, tr0 = (local % tiles_r) * PUSH_TILE;
-        const int tc1 = min(tc0 + PUSH_TILE, S.cols), tr1 = min(tr0 + PUSH_TILE, S.rows);
-        const int R = S.close_on ? S.reach : 0;
-        const int W = PUSH_TILE + 2 * R;
-        const int wc0 = tc0 - R, wr0 = tr0 - R;
-        const int rlo = max(wr0, 0), rhi = min(wr0 + W, S.rows) - 1;
-        // Wave 0 lays out the window (column runs, lane = column) and the tile's
-        // work (lane = tile cell, column-major).
-        if (wave == 0) {
-            const int cc = wc0 + lane;
-            int g0 = 0, n = 0;
-            if (lane < W && cc >= 0 && cc < S.cols && rlo <= rhi) {
-                g0 = S.cstart[cc * S.rows + rlo];
-                n = S.cstart[cc * S.rows + rhi + 1] - g0;
-            }
-            const int incl = wave_scan_incl(n);
-            if (lane < W) {
-                col_base[lane] = incl - n;
-                col_g0[lane] = g0;
-            }
-            if (lane == W - 1) col_base[W] = incl;
-            const int hc = tc0 + lane / PUSH_TILE, hr = tr0 + lane % PUSH_TILE;
-            int m = 0, first = 0;
-            if (hc < tc1 && hr < tr1) {
-                const int cell = hc * S.rows + hr;
-                first = S.cstart[cell];
-                m = S.cstart[cell + 1] - first;
-            }
-            cell_e0[lane] = first;
-            cell_n[lane] = m;
-            const int E = item_e(m), I = item_lanes(m, E, E == 2 ? 32 : 64);
-            const int items = (m > 0 && m < HEAVY_CELL) ? (m + E * I - 1) / (E * I) : 0;
-            const int item_incl = wave_scan_incl(items);
-            item_pref[lane] = item_incl - items;
-            if (lane == 63) item_pref[PUSH_CELLS] = item_incl;
-        }
-        __syncthreads();
-        for (int q = threadIdx.x; q < W * W; q += THREADS) {
-            const int ci = q / W, ri = q - (q / W) * W;
-            const int cc = wc0 + ci, rr = wr0 + ri;
-            const bool in = cc >= 0 && cc < S.cols && rr >= 0 && rr < S.rows;
-            dens_w[q] = in ? S.density[cc * S.rows + rr] : 0.0f;
-        }
-        // As many consecutive columns of home cells per pass as the staging holds
-        // (home columns [a, b) need window columns [a, b + 2 reach)): the whole
-        // window in one pass when it fits; a single column that does not fit
-        // reads its neighbours from global memory.
-        for (int a = 0, b = 0; a < PUSH_TILE; a = b) {
-            b = a + 1;
-            while (b < PUSH_TILE && col_base[min(b + 1 + 2 * R, W)] - col_base[a] <= PUSH_STAGE) b++;
-            const int ca = a, cb = min(b + 2 * R, W);
-            const int la = a * PUSH_TILE, lz = b * PUSH_TILE;
-            const int base = col_base[ca], staged_n = col_base[cb] - base;
-            const bool staged = staged_n <= PUSH_STAGE;
-            if (staged) {
-                for (int q = threadIdx.x; q < (cb - ca) * W; q += THREADS) {
-                    const int ci = ca + q / W, ri = q - (q / W) * W;
-                    const int cc = wc0 + ci, rr = wr0 + ri;
-                    const int w = ci * W + ri;
-                    if (cc < 0 || cc >= S.cols || rr < 0 || rr >= S.rows) {
-                        lb[w] = 0;
-                        le[w] = 0;
-                    } else {
-                        const int cell = cc * S.rows + rr;
-                        lb[w] = col_base[ci] - base + (S.cstart[cell] - col_g0[ci]);
-                        le[w] = col_base[ci] - base + (S.cstart[cell + 1] - col_g0[ci]);
-                    }
-                }
-                // One pass over every staged electron (column found by search), so a
-                // sparse window costs one round of loads rather than one per column.
-                for (int j = threadIdx.x; j < staged_n; j += THREADS) {
-                    int q = ca;
-                    for (int step = PUSH_WIN / 2; step > 0; step >>= 1)
-                        if (q + step < cb && col_base[q + step] - base <= j) q += step;
-                    const int g = col_g0[q] + (j - (col_base[q] - base));
-                    const int rr = S.scell[g] - (wc0 + q) * S.rows;
-                    lp[j] = cell_pos(rr, wr0 + (W >> 1), S.sfx[g], S.sfy[g]);
-                }
-            }
-            __syncthreads();
-            for (int it = item_pref[la] + wave; it < item_pref[lz]; it += WAVES) {
-                int lo = la, hi = lz;
-                while (hi - lo > 1) {
-                    const int mid = (lo + hi) >> 1;
-                    if (item_pref[mid] <= it) lo = mid;
-                    else hi = mid;
-                }
-                lo = __builtin_amdgcn_readfirstlane(lo);
-                const int hc = tc0 + lo / PUSH_TILE, hr = tr0 + lo % PUSH_TILE;
-                const int chunk = it - item_pref[lo];
-                const int e0 = __builtin_amdgcn_readfirstlane(cell_e0[lo]), n = __builtin_amdgcn_readfirstlane(cell_n[lo]);
-                if (staged) {
-                    const LdsRuns runs{lp, lb, le, wc0, wr0, W};
-                    if (item_e(n) == 2)
-                        cell_item<MODE, 2>(S, runs, gt, dens_w, hc, hr, e0, n, chunk, wc0, wr0, W, k, h, t, force_out);
-                    else
-                        cell_item<MODE, 1>(S, runs, gt, dens_w, hc, hr, e0, n, chunk, wc0, wr0, W, k, h, t, force_out);
-                } else {
-                    const GlobalRuns runs{S, wc0, wr0, wr0 + (W >> 1)};
-                    if (item_e(n) == 2)
-                        cell_item<MODE, 2>(S, runs, gt, dens_w, hc, hr, e0, n, chunk, wc0, wr0, W, k, h, t, force_out);
-                    else
-                        cell_item<MODE, 1>(S, runs, gt, dens_w, hc, hr, e0, n, chunk, wc0, wr0, W, k, h, t, force_out);
-                }
-            }
-            __syncthreads();
-        }
+        push_tile<MODE, THREADS, false>(S, L, local, k, h, t, force_out);
     }
 }
 
@@ -1225,11 +1245,14 @@ __global__ __launch_bounds__(FORCE_THREADS) void k_force_tiled(Batch b, float2 *
 // scene's ticket counter, the items of each substep in phase order
 //   0 count      1024 electrons: cell and rank within it (atomic count)
 //   1 tile sum   SCAN_TILE cells: their electron count
-//   2 top scan   one item: exclusive scan of the tile sums, cstart[cells] = n
+//   2 top scan   one item: exclusive scan of the tile sums, cstart[cells] = n,
+//                dense-tile flags cleared
 //   3 starts     SCAN_TILE cells: cstart from the scan, counts back to zero
 //   4 scatter    1024 electrons: order[cstart + rank] = electron
-//   5 canon      256 cells: in-cell order by electron index and the snapshot
-//   6 push       256 slots: push_one
+//   5 canon      REST_THREADS cells: in-cell order by electron index, the
+//                snapshot, the dense-tile flags
+//   6 push       one push tile: a dense tile in cell mode (push_tile, crowded
+//                cells included), the others electron by electron (push_one)
 // and an item first waits until every item of the phase before it is done.
 // Those items were drawn earlier, by blocks already running, whose own waits
 // go back further in the same way to the first phase, which waits for nothing:
@@ -1241,13 +1264,13 @@ __global__ __launch_bounds__(FORCE_THREADS) void k_force_tiled(Batch b, float2 *
 // fault in this bookkeeping shows as an error instead of a hung GPU.
 constexpr int REST_SPIN_LIMIT = 1 << 24;
 
-__device__ __forceinline__ int rest_items(int p, int n, int cells) {
+__device__ __forceinline__ int rest_items(int p, int n, int cells, int ptiles) {
     switch (p) {
     case 0: case 4: return (n + 1023) / 1024;
     case 1: case 3: return (cells + SCAN_TILE - 1) / SCAN_TILE;
     case 2: return 1;
-    case 5: return (cells + 255) / 256;
-    default: return (n + 255) / 256;
+    case 5: return (cells + REST_THREADS - 1) / REST_THREADS;
+    default: return ptiles;
     }
 }
 
@@ -1271,6 +1294,8 @@ __device__ __forceinline__ int rest_block_scan(int v, int *wsum, int &total) {
 __global__ __launch_bounds__(REST_THREADS) void k_rest(Batch b) {
     __shared__ int sh_ticket, sh_fail;
     __shared__ int wsum[REST_THREADS / 64];
+    __shared__ PushTileLds L;
+    bool table_loaded = false;
     const int tid = threadIdx.x;
     for (int blk = blockIdx.x; blk < b.total[KC_REST]; blk += gridDim.x) {
         const int s = scene_of(b, KC_REST, blk);
@@ -1281,8 +1306,14 @@ __global__ __launch_bounds__(REST_THREADS) void k_rest(Batch b) {
         if (sub <= 1) continue;
         const float h = S.dt / (float)sub;
         const int n = __builtin_amdgcn_readfirstlane(S.n), cells = __builtin_amdgcn_readfirstlane(S.cols * S.rows);
+        const int tiles_r = (S.rows + PUSH_TILE - 1) / PUSH_TILE;
+        const int ptiles = __builtin_amdgcn_readfirstlane(((S.cols + PUSH_TILE - 1) / PUSH_TILE) * tiles_r);
+        if (S.close_on && !table_loaded) {   // the pair table is shared by every scene
+            for (int q = tid; q < PAIR_SAMPLES + 4; q += REST_THREADS) L.gt[q] = S.g_lerp[q];
+            table_loaded = true;   // (the first tile's barriers order these stores before its reads)
+        }
         int per_sub = 0;
-        for (int p = 0; p < REST_PHASES; p++) per_sub += rest_items(p, n, cells);
+        for (int p = 0; p < REST_PHASES; p++) per_sub += rest_items(p, n, cells, ptiles);
         const int total = (sub - 1) * per_sub;
         int *ticket_ctr = S.rest_sync;
         int *done = S.rest_sync + 1;   // [(k - 1) * REST_PHASES + p]
@@ -1307,7 +1338,7 @@ __global__ __launch_bounds__(REST_THREADS) void k_rest(Batch b) {
             // The ticket's substep, phase and item.
             const int k = 1 + ticket / per_sub;
             int item = ticket - (k - 1) * per_sub, p = 0;
-            while (item >= rest_items(p, n, cells)) item -= rest_items(p, n, cells), p++;
+            while (item >= rest_items(p, n, cells, ptiles)) item -= rest_items(p, n, cells, ptiles), p++;
             // Wait for the phase before (the previous substep's push for phase 0).
             const int prev = p > 0 ? (k - 1) * REST_PHASES + p - 1 : (k >= 2 ? (k - 2) * REST_PHASES + REST_PHASES - 1 : -1);
             // (sh_fail is written only here, between the ticket's barriers and
@@ -1315,7 +1346,7 @@ __global__ __launch_bounds__(REST_THREADS) void k_rest(Batch b) {
             if (tid == 0) {
                 int failed = 0;
                 if (prev >= 0) {
-                    const int want = rest_items(p > 0 ? p - 1 : REST_PHASES - 1, n, cells);
+                    const int want = rest_items(p > 0 ? p - 1 : REST_PHASES - 1, n, cells, ptiles);
                     int spins = 0;
                     while (__hip_atomic_load(done + prev, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < want) {
                         __builtin_amdgcn_s_sleep(2);
@@ -1362,7 +1393,7 @@ __global__ __launch_bounds__(REST_THREADS) void k_rest(Batch b) {
                     }
                 }
             } else if (p == 2) {
-                const int tiles = rest_items(1, n, cells);
+                const int tiles = rest_items(1, n, cells, ptiles);
                 int carry = 0;
                 for (int q0 = 0; q0 < tiles; q0 += REST_THREADS) {
                     const int q = q0 + tid;
@@ -1373,15 +1404,27 @@ __global__ __launch_bounds__(REST_THREADS) void k_rest(Batch b) {
                     carry += chunk;
                 }
                 if (tid == 0) S.cstart[cells] = n;
+                for (int q = tid; q < ptiles; q += REST_THREADS) S.tile_dense[q] = 0;
             } else if (p == 4) {
                 for (int i = item * 1024 + tid; i < min(n, item * 1024 + 1024); i += REST_THREADS)
                     S.order[S.cstart[S.ecell[i]] + S.erank[i]] = i;
             } else if (p == 5) {
-                const int c = item * 256 + tid;
-                if (c < cells) canon_cell(S, c, S.cstart[c], S.cstart[c + 1]);
-            } else {
-                const int slot = item * 256 + tid;
-                if (slot < n) push_one(S, slot, k, h, t);
+                const int c = item * REST_THREADS + tid;
+                if (c < cells) {
+                    const int s0 = S.cstart[c], s1 = S.cstart[c + 1];
+                    canon_cell(S, c, s0, s1);
+                    if (s1 - s0 >= CELL_MODE_MIN)   // (every writer stores the same 1)
+                        S.tile_dense[(c / S.rows / PUSH_TILE) * tiles_r + (c % S.rows) / PUSH_TILE] = 1;
+                }
+            } else if (S.close_on && S.tile_dense[item]) {
+                push_tile<ITEM_PUSH, REST_THREADS, true>(S, L, item, k, h, t, nullptr);
+            } else {   // a sparse tile: its columns' runs, one thread per electron
+                const int tc0 = (item / tiles_r) * PUSH_TILE, tr0 = (item % tiles_r) * PUSH_TILE;
+                const int tr1 = min(tr0 + PUSH_TILE, S.rows);
+                for (int c = tc0; c < min(tc0 + PUSH_TILE, S.cols); c++) {
+                    const int g0 = S.cstart[c * S.rows + tr0], g1 = S.cstart[c * S.rows + tr1];
+                    for (int slot = g0 + tid; slot < g1; slot += REST_THREADS) push_one(S, slot, k, h, t);
+                }
             }
             __syncthreads();   // the item's stores, before thread 0 publishes it
             pending = (k - 1) * REST_PHASES + p;

@@ -234,7 +234,7 @@ void scene_blocks(const DevScene &d, int out[KC_COUNT]) {
     out[KC_SCENE] = 1;
     out[KC_PUSH] = cdiv(d.cols, PUSH_TILE) * cdiv(d.rows, PUSH_TILE);
     out[KC_HEAVY] = (int)std::min<long long>(256, std::max<long long>(1, d.n / 16384));
-    out[KC_REST] = (int)std::min<long long>(1024, std::max<long long>(1, cdiv(d.n, 2048)));
+    out[KC_REST] = (int)std::min<long long>(256, std::max<long long>(1, cdiv(d.n, 1024)));
 }
 
 int make_resident(EmbDevice *dev, const EmbSettings &s, HostScene &&hs_in, hipStream_t st, Resident &r) {
