@@ -465,6 +465,7 @@ def main():
         serve = {}
         for mode in ("batched", "plain"):
             st = run_serving(args.serve_requests, args.serve_scale, mode == "batched")
+            print(f"serving {mode}: {st['cell_steps_per_s'] / 1e9:.3f} G cell-steps/s", file=sys.stderr, flush=True)
             serve[mode] = {"cell_steps_per_s": round(st["cell_steps_per_s"], 1),
                            "p50_ms": round(st["step_latency_p50_ms"], 4), "p99_ms": round(st["step_latency_p99_ms"], 4),
                            "completed": st["completed"], "peak_resident": st["peak_resident"],
