@@ -360,7 +360,18 @@ def test_ring_circuit_two_thousand_ticks():
            "bz": relative_error(bz, cbz), "phi": relative_error(sim.read_potential(), cpu.potential_nodes()),
            "gain": abs(gains[0] - cgains[0])}
     residual, _, _ = sim.diagnostics()
-    print("ring 2000 ticks", err, "gauss residual", residual)
+    # The current along the top and bottom edges, averaged over 20 samples, beside
+    # Ohm's EMF / (71 wire cells x 1 ohm + the battery's 0.1). At one carrier per
+    # cell the ring does not reach it in the FP64 oracle either (1.1e-4 there,
+    # 0.0126 = 89% at 8 carriers per cell), so it is reported, not asserted;
+    # Ohm's law is held at 8 carriers by test_loop_current_follows_ohms_law.
+    samples = []
+    for _ in range(20):
+        sim.run_ticks(15)
+        cur = sim.read_current()
+        samples.append(0.5 * (abs(cur[10:30, 8, 0].mean()) + abs(cur[10:30, 23, 0].mean())))
+    ohm = 1.0 / (71 * 1.0 + world.settings["battery_resistance"])
+    print("ring 2000 ticks", err, "gauss residual", residual, "current", np.mean(samples), "Ohm", ohm)
     assert err["pos"] < 1e-2, err
     assert err["ex"] < 1e-2 and err["ey"] < 1e-2 and err["phi"] < 1e-2, err
     assert err["gain"] < 1e-3, err
