@@ -42,6 +42,7 @@ struct Request {
     std::vector<double> pos;                   // kept results
     std::vector<float> vel, ex, ey, bz;
     bool has_result = false;
+    bool rest_error = false;                   // k_rest's wait limit hit during the run
 };
 
 double now_s() {
@@ -141,7 +142,12 @@ int capture_graph(EmbServer *sv) {
 
 int retire(EmbServer *sv, Request *q) {
     Resident &r = q->r;
-    if (q->state == RUNNING && sv->cfg.keep_results) {
+    if (q->state == RUNNING) {   // a queue fault of k_rest, never expected, fails the request loudly
+        int32_t flag = 0;
+        EMB_HIP(hipMemcpy(&flag, &r.ds.ctr->rest_error, sizeof(flag), hipMemcpyDeviceToHost));
+        q->rest_error = flag != 0;
+    }
+    if (q->state == RUNNING && sv->cfg.keep_results && !q->rest_error) {
         const DevScene &d = r.ds;
         const size_t n = (size_t)d.n, nx = d.nx, ny = d.ny;
         q->pos.resize(2 * n);
@@ -390,6 +396,7 @@ int emb_server_result(EmbServer *sv, int64_t id, double *pos, double *vel, doubl
     if (it == sv->all.end()) return fail(EMB_E_INVALID, "no such request");
     Request *q = it->second.get();
     if (q->state != DONE) return fail(EMB_E_STATE, "request not finished");
+    if (q->rest_error) return fail(EMB_E_NUMERIC, "electron substeps after the first were dropped: k_rest wait limit");
     if (!q->has_result) return fail(EMB_E_STATE, "result not kept (keep_results = 0) or already read");
     auto out = [](double *dst, const auto &src) {
         if (dst)
