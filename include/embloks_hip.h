@@ -139,6 +139,8 @@ int emb_sim_upload(EmbSim *sim, const EmbScene *scene, int reset_display);
  * physics ticks (Simulation::step, simulation.rs:63-72) on the sim's stream.
  * Asynchronous; emb_sim_sync waits. */
 int emb_sim_step(EmbSim *sim, int ticks);
+/* Waits for the queued ticks. EMB_E_NUMERIC if a tick's later electron substeps
+ * were dropped (the substep queue's wait limit, a fault never expected). */
 int emb_sim_sync(EmbSim *sim);
 int emb_sim_dims(EmbSim *sim, EmbDims *out);
 /* read_state (sim.rs:719-768): electrons in upload order and the gains per
@@ -239,7 +241,8 @@ int emb_server_poll(EmbServer *srv, int32_t *active);
 int emb_server_done(EmbServer *srv, int64_t id);
 /* Result of a finished request (kept when keep_results is set): electrons and
  * fields as emb_sim_read_state / emb_sim_read_yee would give them. Any pointer
- * may be NULL. Reading releases the result. */
+ * may be NULL. Reading releases the result. EMB_E_NUMERIC if the run dropped
+ * electron substeps (the substep queue's wait limit, a fault never expected). */
 int emb_server_result(EmbServer *srv, int64_t id, double *pos, double *vel, double *ex, double *ey, double *bz);
 int emb_server_stats(EmbServer *srv, EmbServerStats *out);
 int emb_server_destroy(EmbServer *srv);
