@@ -165,8 +165,10 @@ __device__ __forceinline__ void canon_cell(const DevScene &S, int c, int s0, int
 // 256 cells per block: their slots are loaded into LDS, each cell's slots are
 // ordered by electron index there, and the ordered slots and the snapshot are
 // written back with coalesced stores. Blocks whose cells hold more than
-// CANON_CAP electrons sort in global memory instead.
-constexpr int CANON_CAP = 6144;
+// CANON_CAP electrons sort in global memory instead. 4096 (256 cells at 16
+// electrons, 32 KB of LDS) measured 29 us on the 512^2 scene against 33 us at
+// 6144 (more blocks resident) and 134 us at 2048 (most blocks fall back).
+constexpr int CANON_CAP = 4096;
 
 __global__ __launch_bounds__(256) void k_canon(Batch b) {
     __shared__ int lorder[CANON_CAP], lsorted[CANON_CAP];
