@@ -1093,7 +1093,10 @@ __device__ __forceinline__ void push_tiled_body(Batch b, int k, float2 *force_ou
 // electrons per cell there is little to share between neighbours, and a tile
 // would be a chain of dependent loads for a handful of threads.
 template <bool FORCE>
-__global__ __launch_bounds__(256) void k_push_sparse(Batch b, int k, float2 *force_out) {
+// (held to 64 VGPRs, 8 waves per SIMD: its gathers are latency-bound; 7 us
+// faster on the 512^2 scene than at the 100 VGPRs it would take)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_push_sparse(Batch b, int k,
+                                                                                             float2 *force_out) {
     for (int blk = blockIdx.x; blk < b.total[KC_ELECTRONS]; blk += gridDim.x) {
         const int s = scene_of(b, KC_ELECTRONS, blk);
         const DevScene &S = b.scenes[s];
