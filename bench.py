@@ -205,6 +205,29 @@ def run_poisson(n: int = 2048, repeats: int = 20):
     return out
 
 
+def run_tables(cpu: bool, repeats: int = 5):
+    """The close-range tables (close_range::tables, close_range.rs:120-146) built
+    afresh on the device for the default parameters (softening 0.25, 2 passes,
+    cutoff 3) and two more sets: milliseconds per build on the host clock
+    (launches, the 64 Poisson solves and gathers of grid_force, the radial,
+    square and ion tables, the copy back), best of `repeats`; beside it the
+    FP64 oracle's single-threaded host build of the same tables."""
+    from embloks_amd.gpusim import close_tables
+
+    out = {}
+    for soft, passes, cutoff in ((0.25, 2, 3.0), (0.25, 2, 2.0), (0.2, 3, 4.0)):
+        ms = min(close_tables(soft, passes, cutoff, timed=True)[-1] for _ in range(repeats))
+        row = {"device_ms": round(ms, 3)}
+        if cpu:
+            from oracle import orc
+            orc.set_threads(1)
+            t0 = time.perf_counter()
+            orc.tables(soft, passes, cutoff * (1.0 + 1e-12))   # a parameter set the oracle has not cached
+            row["oracle_host_ms"] = round((time.perf_counter() - t0) * 1e3, 1)
+        out[f"softening {soft}, passes {passes}, cutoff {cutoff}"] = row
+    return out
+
+
 def run_p3m(repeats: int = 10):
     """The close-range force alone (CloseRange::force, close_range.rs:314-376)
     on the 4096 x 4096 scene with 4M clustered electrons at cutoff 2: time per
@@ -410,6 +433,7 @@ def main():
     ap.add_argument("--serve-scale", type=float, default=0.02, help="fraction of each request's 1000-20000 steps")
     ap.add_argument("--no-poisson", action="store_true", help="skip the isolated 2048^2 DST solve")
     ap.add_argument("--no-p3m", action="store_true", help="skip the 4096^2 clustered close-range force")
+    ap.add_argument("--no-tables", action="store_true", help="skip the close-range table build")
     ap.add_argument("--no-readme", action="store_true", help="skip the README grids (40x32, 100x80, 200x160)")
     ap.add_argument("--no-gauss-log", action="store_true", help="skip the 10^4-tick Gauss residual log")
     ap.add_argument("--no-large", action="store_true", help="skip the 4096^2 FDTD / push bandwidth leg")
@@ -453,6 +477,8 @@ def main():
         out["poisson"] = run_poisson()
     if not args.no_p3m:
         out["p3m"] = run_p3m()
+    if not args.no_tables:
+        out["close_range_tables"] = run_tables(cpu=not args.no_cpu)
     if not args.no_large:
         out["large"] = run_large()
     if not args.no_gauss_log:

@@ -17,7 +17,9 @@ CSRC = HERE / "csrc"
 OUT = HERE / "libembloks_hip.so"
 BUILD = HERE.parent / "_build" / "embloks_amd"
 
-HIP_SOURCES = ["kernels.hip", "gemm.hip", "runtime.hip", "server.hip", "poisson.hip"]
+HIP_SOURCES = ["kernels.hip", "gemm.hip", "runtime.hip", "server.hip", "poisson.hip", "tables.hip"]
+# The table build sums in the reference's order without fused multiply-adds.
+EXTRA_FLAGS = {"tables.hip": ["-ffp-contract=off"]}
 CXX_SOURCES = ["setup.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",
@@ -37,7 +39,7 @@ def build(verbose: bool = False) -> pathlib.Path:
     BUILD.mkdir(parents=True, exist_ok=True)
     jobs = []
     for src in HIP_SOURCES:
-        jobs.append([HIPCC, *HIP_FLAGS, "-c", str(CSRC / src), "-o", str(BUILD / (src + ".o"))])
+        jobs.append([HIPCC, *HIP_FLAGS, *EXTRA_FLAGS.get(src, []), "-c", str(CSRC / src), "-o", str(BUILD / (src + ".o"))])
     cxx = shutil.which("g++") or "c++"
     for src in CXX_SOURCES:
         jobs.append([cxx, *CXX_FLAGS, "-c", str(CSRC / src), "-o", str(BUILD / (src + ".o"))])

@@ -160,7 +160,8 @@ int shared_tables(EmbDevice *dev, const EmbSettings &s, int nx, int ny, DevScene
     if (it == dev->tables.end()) {
         DeviceTables t;
         if (s.electron_cutoff > 0.0) {
-            auto h = host_tables(s.electron_softening, s.smoothing_passes, s.electron_cutoff);
+            auto h = find_tables(s.electron_softening, s.smoothing_passes, s.electron_cutoff);
+            if (!h) return fail(EMB_E_INVALID, "close-range tables not built for these settings");
             std::vector<float> g(h->g.begin(), h->g.end()), sq(h->square.begin(), h->square.end());
             g.push_back(0.0f);   // two zero samples past the end: where pairs beyond the cutoff read
             g.push_back(0.0f);
@@ -505,6 +506,8 @@ int64_t emb_scene_footprint(const EmbSettings *s, int32_t cols, int32_t rows, in
 int emb_sim_create(EmbDevice *dev, const EmbSettings *settings, const EmbScene *scene, EmbSim **out) {
     EMB_GUARD({
         if (!dev || !settings || !scene || !out) return fail(EMB_E_INVALID, "null argument");
+        int trc = ensure_tables(dev, settings->electron_softening, settings->smoothing_passes, settings->electron_cutoff);
+        if (trc != EMB_OK) return trc;
         HostScene hs;
         const std::string why = prepare_scene(*settings, *scene, hs);
         if (!why.empty()) return fail(EMB_E_INVALID, why);

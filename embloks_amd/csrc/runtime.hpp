@@ -85,6 +85,10 @@ struct Resident {
     long long *dep_dump = nullptr;
 };
 
+// The close-range tables of (softening, passes, cutoff), built on `dev` in FP64
+// and cached for the process (tables.hip); a no-op when cutoff <= 0 or built.
+int ensure_tables(EmbDevice *dev, double softening, int passes, double cutoff);
+
 // The FP32 DST-I matrix of length n on the device (built once per device).
 int device_sine(EmbDevice *dev, int n, const SineSet **out);
 

@@ -206,6 +206,8 @@ int emb_server_create(EmbDevice *dev, const EmbSettings *settings, const EmbServ
 int emb_server_submit(EmbServer *sv, const EmbScene *scene, int64_t steps, int64_t *id) {
     try {
         if (!sv || !scene || !id || steps < 0) return fail(EMB_E_INVALID, "bad argument");
+        int trc = ensure_tables(sv->dev, sv->s.electron_softening, sv->s.smoothing_passes, sv->s.electron_cutoff);
+        if (trc != EMB_OK) return trc;
         auto q = std::make_unique<Request>();
         const std::string why = prepare_scene(sv->s, *scene, q->hs);
         if (!why.empty()) return fail(EMB_E_INVALID, why);

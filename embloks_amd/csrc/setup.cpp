@@ -15,247 +15,20 @@ namespace emb {
 namespace {
 
 constexpr double PI = 3.14159265358979323846;
-constexpr int SQUARE_SAMPLES = 32;     // close_range.rs:32
-constexpr int SOURCE_SAMPLES = 8;      // close_range.rs:35
-constexpr int DIRECTIONS = 16;         // close_range.rs:36
-constexpr double RADIAL_STEP = 0.01;   // close_range.rs:37
 
-// g of a unit line charge smeared as a Gaussian of width `eps`: field g d,
-// 2 (1 - exp(-r^2/eps^2)) / r^2 (close_range.rs:41-49).
-double compact(double r2, double eps) {
-    const double x = r2 / (eps * eps);
-    const double v = x < 1e-4 ? 2.0 * (1.0 - x / 2.0 + x * x / 6.0) : -2.0 * std::expm1(-x) / x;
-    return v / (eps * eps);
-}
-
-// Piecewise-linear interpolation, clamped at the ends (numpy.interp).
-double lerp_table(double x, const std::vector<double> &xs, const std::vector<double> &ys) {
-    if (x <= xs.front()) return ys.front();
-    if (x >= xs.back()) return ys.back();
-    const size_t i = (size_t)(std::upper_bound(xs.begin(), xs.end(), x) - xs.begin()) - 1;
-    return ys[i] + (ys[i + 1] - ys[i]) * (x - xs[i]) / (xs[i + 1] - xs[i]);
-}
-
-// Dense row-major n x n product C = A B.
-void matmul(int n, const std::vector<double> &A, const std::vector<double> &B, std::vector<double> &C) {
-    C.assign((size_t)n * n, 0.0);
-    for (int i = 0; i < n; i++)
-        for (int k = 0; k < n; k++) {
-            const double a = A[(size_t)i * n + k];
-            if (a == 0.0) continue;
-            const double *b = &B[(size_t)k * n];
-            double *c = &C[(size_t)i * n];
-            for (int j = 0; j < n; j++) c[j] += a * b[j];
-        }
-}
-
-// The radial force between two unit charges through a square grid of the
-// default margin, measured as close_range::grid_force does (close_range.rs:73-108):
-// one charge deposited (CIC), smoothed, its static field solved (DST-I Poisson
-// and E = -grad phi) and read back bilinearly around it; averaged over 8 x 8
-// source positions in the cell and 16 directions.
-void grid_force(int passes, double reach, std::vector<double> &radii, std::vector<double> &force) {
-    const int n = 2 * (int)std::ceil(reach) + 24;
-    const int m = 16;          // Settings::default().absorbing_margin
-    const int N = n + 2 * m;
-    // Sine matrix and inverse Laplacian of the N x N node grid (maxwell.rs:56-67).
-    std::vector<double> S((size_t)N * N), L((size_t)N * N);
-    for (int k = 0; k < N; k++)
-        for (int j = 0; j < N; j++)
-            S[(size_t)k * N + j] = std::sin(PI * (double)(((long long)(k + 1) * (j + 1)) % (2 * (N + 1))) / (double)(N + 1));
-    std::vector<double> lam(N);
-    for (int k = 0; k < N; k++) lam[k] = 2.0 * std::cos(PI * (double)(k + 1) / (double)(N + 1)) - 2.0;
-    const double norm = 4.0 / (double)((N + 1) * (N + 1));
-    for (int i = 0; i < N; i++)
-        for (int j = 0; j < N; j++) L[(size_t)i * N + j] = norm / (lam[i] + lam[j]);
-
-    const size_t count = (size_t)std::llround(reach / RADIAL_STEP) + 1;
-    radii.resize(count);
-    for (size_t i = 0; i < count; i++) radii[i] = 0.0 + (reach - 0.0) * (double)i / (double)(count - 1);
-    double dir[DIRECTIONS][2];
-    for (int k = 0; k < DIRECTIONS; k++) {
-        const double a = 2.0 * PI * ((double)k + 0.5) / (double)DIRECTIONS;
-        dir[k][0] = std::cos(a);
-        dir[k][1] = std::sin(a);
-    }
-    std::vector<double> total(count, 0.0), rhs((size_t)N * N), t1, t2;
-    std::vector<double> ex((size_t)(N + 1) * N), ey((size_t)N * (N + 1));
-    const int k_pass = passes;
-    const int P = n + 2 * k_pass;
-    std::vector<double> pad((size_t)P * P), nxt((size_t)P * P);
-    auto bilin = [](const std::vector<double> &a, int w, int h, double gx, double gy) {
-        const long long i = std::clamp((long long)std::floor(gx), 0LL, (long long)w - 2);
-        const long long j = std::clamp((long long)std::floor(gy), 0LL, (long long)h - 2);
-        const double fx = gx - (double)i, fy = gy - (double)j;
-        return a[(size_t)i * h + j] * (1.0 - fx) * (1.0 - fy) + a[(size_t)(i + 1) * h + j] * fx * (1.0 - fy) +
-               a[(size_t)i * h + j + 1] * (1.0 - fx) * fy + a[(size_t)(i + 1) * h + j + 1] * fx * fy;
-    };
-    for (int si = 0; si < SOURCE_SAMPLES; si++)
-        for (int sj = 0; sj < SOURCE_SAMPLES; sj++) {
-            const double src[2] = {(double)(n / 2) + ((double)si + 0.5) / SOURCE_SAMPLES,
-                                   (double)(n / 2) + ((double)sj + 0.5) / SOURCE_SAMPLES};
-            // CIC of the unit charge on the n x n grid (fields.rs:27-51), padded by `passes`.
-            std::fill(pad.begin(), pad.end(), 0.0);
-            const double u = src[0] - 0.5, v = src[1] - 0.5;
-            const long long c0 = (long long)std::floor(u), r0 = (long long)std::floor(v);
-            const double fu = u - (double)c0, fv = v - (double)r0;
-            for (int a = 0; a < 2; a++)
-                for (int bb = 0; bb < 2; bb++) {
-                    const long long c = std::clamp(c0 + a, 0LL, (long long)n - 1), r = std::clamp(r0 + bb, 0LL, (long long)n - 1);
-                    pad[(size_t)(c + k_pass) * P + (r + k_pass)] += (a ? fu : 1.0 - fu) * (bb ? fv : 1.0 - fv);
-                }
-            // `passes` passes of the 1-2-1 filter, x then y, zero beyond the padding (maxwell.rs:397-414).
-            for (int p = 0; p < k_pass; p++)
-                for (int axis = 0; axis < 2; axis++) {
-                    for (int i = 0; i < P; i++)
-                        for (int j = 0; j < P; j++) {
-                            const int di = axis == 0 ? 1 : 0, dj = axis == 0 ? 0 : 1;
-                            double s = 0.5 * pad[(size_t)i * P + j];
-                            if ((axis == 0 ? i : j) > 0) s += 0.25 * pad[(size_t)(i - di) * P + (j - dj)];
-                            if ((axis == 0 ? i : j) + 1 < P) s += 0.25 * pad[(size_t)(i + di) * P + (j + dj)];
-                            nxt[(size_t)i * P + j] = s;
-                        }
-                    pad.swap(nxt);
-                }
-            std::fill(rhs.begin(), rhs.end(), 0.0);
-            for (int i = 0; i < P; i++)
-                for (int j = 0; j < P; j++) rhs[(size_t)(i + m - k_pass) * N + (j + m - k_pass)] = -4.0 * PI * pad[(size_t)i * P + j];
-            // phi = S ((S rhs S) * L) S; the field of a static charge is -grad phi.
-            matmul(N, S, rhs, t1);
-            matmul(N, t1, S, t2);
-            for (size_t q = 0; q < t2.size(); q++) t2[q] *= L[q];
-            matmul(N, S, t2, t1);
-            matmul(N, t1, S, t2);
-            const std::vector<double> &phi = t2;
-            for (int i = 0; i <= N; i++)
-                for (int j = 0; j < N; j++)
-                    ex[(size_t)i * N + j] = -((i < N ? phi[(size_t)i * N + j] : 0.0) - (i > 0 ? phi[(size_t)(i - 1) * N + j] : 0.0));
-            for (int i = 0; i < N; i++)
-                for (int j = 0; j <= N; j++)
-                    ey[(size_t)i * (N + 1) + j] = -((j < N ? phi[(size_t)i * N + j] : 0.0) - (j > 0 ? phi[(size_t)i * N + j - 1] : 0.0));
-            for (size_t k = 0; k < count; k++) {
-                double along = 0.0;
-                for (int d = 0; d < DIRECTIONS; d++) {
-                    const double x = src[0] + radii[k] * dir[d][0], y = src[1] + radii[k] * dir[d][1];
-                    const double fx = bilin(ex, N + 1, N, x + m, y - 0.5 + m);
-                    const double fy = bilin(ey, N, N + 1, x - 0.5 + m, y + m);
-                    along += fx * dir[d][0] + fy * dir[d][1];
-                }
-                total[k] += along / DIRECTIONS;
-            }
-        }
-    force.resize(count);
-    for (size_t k = 0; k < count; k++) force[k] = total[k] / (double)(SOURCE_SAMPLES * SOURCE_SAMPLES);
-}
-
-double radial_at(const std::vector<double> &table, double r2, double cutoff) {  // close_range.rs:174-181
-    if (r2 >= cutoff * cutoff) return 0.0;
-    const double x = std::min(r2 * ((double)PAIR_SAMPLES / (cutoff * cutoff)), (double)PAIR_SAMPLES);
-    const size_t i = (size_t)x;
-    return table[i] + (table[i + 1] - table[i]) * (x - (double)i);
-}
-
-std::shared_ptr<HostTables> build_tables(double softening, int passes, double cutoff) {
-    auto t = std::make_shared<HostTables>();
-    t->softening = softening;
-    t->passes = passes;
-    t->cutoff = cutoff;
-    t->reach = (int)std::ceil(cutoff);
-    if (cutoff <= 0.0) return t;
-    grid_force(passes, cutoff, t->grid_r, t->grid_f);
-    const auto &r = t->grid_r, &fg = t->grid_f;
-    // The correction's potential: the integral of (compact - grid) force inwards
-    // from the cutoff (close_range.rs:151-171).
-    std::vector<double> near(r.size()), pot(r.size(), 0.0);
-    for (size_t i = 0; i < r.size(); i++) near[i] = r[i] * compact(r[i] * r[i], softening) - fg[i];
-    for (size_t i = r.size() - 1; i-- > 0;) pot[i] = pot[i + 1] + 0.5 * (near[i + 1] + near[i]) * (r[i + 1] - r[i]);
-    for (int i = 0; i <= PAIR_SAMPLES; i++) {
-        const double r2 = 0.0 + (cutoff * cutoff - 0.0) * (double)i / (double)PAIR_SAMPLES;
-        const double rs = std::sqrt(r2);
-        const double grid_g = i == 0 ? fg[1] / r[1] : lerp_table(rs, r, fg) / rs;
-        t->g.push_back(compact(r2, softening) - grid_g);
-        t->potential.push_back(lerp_table(rs, r, pot));
-    }
-    t->g.push_back(0.0);
-    t->potential.push_back(0.0);
-    // The correction of a unit charge spread evenly over a cell: midpoint-rule
-    // box sums through a summed-area table (close_range.rs:195-219).
-    const int M = SQUARE_SAMPLES;
-    const double h = 1.0 / M;
-    t->half = std::ceil(cutoff) + 1.0;
-    const int cnt = (int)std::llround((2.0 * t->half + 1.0) * M);
-    std::vector<double> z(cnt);
-    for (int k = 0; k < cnt; k++) z[k] = -t->half - 0.5 + ((double)k + 0.5) * h;
-    const int W = cnt + 1;
-    std::vector<double> sat((size_t)W * W * 3, 0.0);
-    for (int a = 0; a < cnt; a++)
-        for (int b = 0; b < cnt; b++) {
-            const double r2 = z[a] * z[a] + z[b] * z[b];
-            const double gr = radial_at(t->g, r2, cutoff);
-            const double pt[3] = {radial_at(t->potential, r2, cutoff), gr * z[a], gr * z[b]};
-            for (int k = 0; k < 3; k++)
-                sat[((size_t)(a + 1) * W + b + 1) * 3 + k] =
-                    pt[k] + sat[((size_t)a * W + b + 1) * 3 + k] + sat[((size_t)(a + 1) * W + b) * 3 + k] - sat[((size_t)a * W + b) * 3 + k];
-        }
-    const int n = cnt - M + 1;
-    t->square_n = n;
-    t->square.resize((size_t)n * n * 3);
-    for (int i = 0; i < n; i++)
-        for (int j = 0; j < n; j++)
-            for (int k = 0; k < 3; k++)
-                t->square[((size_t)i * n + j) * 3 + k] =
-                    (sat[((size_t)(i + M) * W + j + M) * 3 + k] - sat[((size_t)i * W + j + M) * 3 + k] -
-                     sat[((size_t)(i + M) * W + j) * 3 + k] + sat[((size_t)i * W + j) * 3 + k]) * h * h;
-    // The square's potential averaged over each cell at offset (dc, dr): trapezoid
-    // rule on the table (close_range.rs:223-242).
-    const int R = t->reach, side = 2 * R + 1;
-    t->ion_stencil.assign((size_t)side * side, 0.0);
-    for (int i = 0; i < side; i++)
-        for (int j = 0; j < side; j++) {
-            const size_t a = (size_t)std::llround(((double)(i - R) - 0.5 + t->half) * M);
-            const size_t b = (size_t)std::llround(((double)(j - R) - 0.5 + t->half) * M);
-            double sum = 0.0;
-            for (int x = 0; x <= M; x++)
-                for (int y = 0; y <= M; y++) {
-                    const double wx = (x == 0 || x == M) ? 0.5 / M : 1.0 / M;
-                    const double wy = (y == 0 || y == M) ? 0.5 / M : 1.0 / M;
-                    sum += wx * t->square[((a + x) * n + (b + y)) * 3] * wy;
-                }
-            t->ion_stencil[(size_t)i * side + j] = sum;
-        }
-    // The force channels summed over every cell offset of the reach window: the
-    // close-range ion force of a uniform-density neighbourhood is one bilinear
-    // lookup in this table (the offsets shift the table index by whole multiples
-    // of SQUARE_SAMPLES, so every cell shares the electron's fractional offset).
-    t->ion_a0 = (int)std::llround((t->half - 0.5) * M);
-    t->ion_sum.assign((size_t)ION_SUM * ION_SUM * 2, 0.0);
-    for (int i = 0; i < ION_SUM; i++)
-        for (int j = 0; j < ION_SUM; j++)
-            for (int ox = -R; ox <= R; ox++)
-                for (int oy = -R; oy <= R; oy++) {
-                    const long long a = t->ion_a0 + i + (long long)M * ox, b = t->ion_a0 + j + (long long)M * oy;
-                    if (a < 0 || b < 0 || a >= n || b >= n) continue;
-                    for (int ch = 0; ch < 2; ch++)
-                        t->ion_sum[((size_t)i * ION_SUM + j) * 2 + ch] += t->square[((size_t)a * n + b) * 3 + 1 + ch];
-                }
-    return t;
-}
-
+std::mutex g_tables_lock;
+std::map<std::tuple<double, int, double>, std::shared_ptr<const HostTables>> g_tables;
 }  // namespace
 
-std::shared_ptr<const HostTables> host_tables(double softening, int passes, double cutoff) {
-    static std::mutex lock;
-    static std::map<std::tuple<double, int, double>, std::shared_ptr<const HostTables>> cache;
-    const auto key = std::make_tuple(softening, passes, cutoff);
-    {
-        std::lock_guard<std::mutex> g(lock);
-        auto it = cache.find(key);
-        if (it != cache.end()) return it->second;
-    }
-    auto t = build_tables(softening, passes, cutoff);
-    std::lock_guard<std::mutex> g(lock);
-    cache[key] = t;
-    return t;
+std::shared_ptr<const HostTables> find_tables(double softening, int passes, double cutoff) {
+    std::lock_guard<std::mutex> g(g_tables_lock);
+    auto it = g_tables.find(std::make_tuple(softening, passes, cutoff));
+    return it == g_tables.end() ? nullptr : it->second;
+}
+
+void store_tables(std::shared_ptr<const HostTables> t) {
+    std::lock_guard<std::mutex> g(g_tables_lock);
+    g_tables[std::make_tuple(t->softening, t->passes, t->cutoff)] = std::move(t);
 }
 
 double HostStack::emf_at(double t) const {
@@ -524,7 +297,8 @@ std::string prepare_scene(const EmbSettings &s, const EmbScene &sc, HostScene &o
     // Close-range potential of all ions per cell (close_range.rs:379-403).
     o.ion_pot.assign((size_t)cells, 0.0f);
     if (s.electron_cutoff > 0.0) {
-        auto t = host_tables(s.electron_softening, s.smoothing_passes, s.electron_cutoff);
+        auto t = find_tables(s.electron_softening, s.smoothing_passes, s.electron_cutoff);
+        if (!t) return "close-range tables not built for these settings";
         const int R = t->reach, side = 2 * R + 1;
         const double scale = s.coulomb_k * s.cell_charge;
         for (long long c = 0; c < cols; c++)

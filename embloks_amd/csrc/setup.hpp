@@ -2,8 +2,6 @@
 //
 // Everything here runs once per scene (or once per table parameter set) on the
 // host, in FP64, and is uploaded as FP32 / int64 arrays:
-//   * close-range tables: grid force, radial tables, square table, ion stencil
-//     (crates/embloks-core/src/close_range.rs:67-242);
 //   * islands and battery stacks (connectivity.rs:20-62, battery.rs:100-190);
 //   * per-cell drag, static charge (ions + fixed charges) in fixed point, the
 //     ions' close-range potential (physics.rs:17-21, fields.rs:91-112,
@@ -36,8 +34,11 @@ struct HostTables {
     int ion_a0 = 0;
 };
 
-// close_range::tables, cached per parameter set for the process.
-std::shared_ptr<const HostTables> host_tables(double softening, int passes, double cutoff);
+// close_range::tables per parameter set, cached for the process. They are built
+// on the device (tables.hip: ensure_tables) before a scene that needs them is
+// prepared; find_tables returns null for a set not built yet.
+std::shared_ptr<const HostTables> find_tables(double softening, int passes, double cutoff);
+void store_tables(std::shared_ptr<const HostTables> t);
 
 struct HostStack {
     std::vector<std::pair<int, int>> cells;

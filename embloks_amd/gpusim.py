@@ -56,6 +56,8 @@ SIGNATURES = {
     "emb_server_stats": (C.c_int, [VP, C.POINTER(abi.EmbServerStats)]),
     "emb_server_destroy": (C.c_int, [VP]),
     "emb_scene_footprint": (C.c_int64, [C.POINTER(abi.EmbSettings), C.c_int32, C.c_int32, C.c_int64]),
+    "emb_close_tables": (C.c_int, [VP, C.c_double, C.c_int32, C.c_double, D, D, D, C.POINTER(C.c_int32),
+                                   C.POINTER(C.c_double), D, C.POINTER(C.c_int32), C.POINTER(C.c_double)]),
     "emb_poisson_create": (C.c_int, [VP, C.c_int32, C.c_int32, C.POINTER(VP)]),
     "emb_poisson_set_rhs": (C.c_int, [VP, D]),
     "emb_poisson_solve": (C.c_int, [VP, C.c_int32, C.c_int32, C.c_int32, D]),
@@ -370,3 +372,22 @@ class Poisson:
         except Exception:
             pass
 
+
+
+def close_tables(softening: float, passes: int, cutoff: float, timed: bool = False, device: "Device | None" = None):
+    """close_range::tables (crates/embloks-core/src/close_range.rs:120-138) built on
+    the device: (g, potential, square (n, n, 3), half, ion_stencil), plus the
+    build's milliseconds when `timed` (a fresh build)."""
+    dev = (device or Device.default()).h
+    n, reach, half = C.c_int32(), C.c_int32(), C.c_double()
+    ms = C.c_double()
+    check(lib().emb_close_tables(dev, softening, passes, cutoff, D(), D(), D(), C.byref(n), C.byref(half), D(),
+                                 C.byref(reach), C.byref(ms) if timed else C.POINTER(C.c_double)()))
+    g, pot = np.zeros(4098), np.zeros(4098)   # PAIR_SAMPLES + 1 samples and the zero past the cutoff
+    square = np.zeros((n.value, n.value, 3))
+    size = 2 * reach.value + 1
+    stencil = np.zeros((size, size))
+    check(lib().emb_close_tables(dev, softening, passes, cutoff, _p(g), _p(pot), _p(square), C.byref(n),
+                                 C.byref(half), _p(stencil), C.byref(reach), C.POINTER(C.c_double)()))
+    out = (g, pot, square, half.value, stencil)
+    return out + (ms.value,) if timed else out

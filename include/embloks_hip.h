@@ -267,6 +267,23 @@ int emb_poisson_solve(EmbPoisson *p, int32_t precision, int32_t tile, int32_t re
 int emb_poisson_read(EmbPoisson *p, double *phi);
 int emb_poisson_destroy(EmbPoisson *p);
 
+/* --- The close-range tables -------------------------------------------------
+ * Replaces close_range::tables (crates/embloks-core/src/close_range.rs:120-138,
+ * built by build_tables, 140-146): the radial correction g and its potential
+ * (4098 samples each, uniform in r^2 up to the cutoff plus a zero sample), the
+ * square table (square_n x square_n x 3: potential, field x, field y over
+ * offsets in [-half, half]^2) and the ion stencil ((2 reach + 1)^2). Built on
+ * the device in FP64 (grid_force's 64 Poisson solves and gathers, then the
+ * radial, square and ion tables) and cached for the process; emb_sim_create
+ * and emb_server_submit build what their settings need. NULL arrays are
+ * skipped, so a first call with NULLs reads the sizes. If build_ms is not
+ * NULL the tables are built afresh and *build_ms gets the build's host-clock
+ * time (launches, kernels, copy back). cutoff must be in [0.02, 32] cells and
+ * passes in [0, 16] (the smoothed charge must stay inside the 16-node margin). */
+int emb_close_tables(EmbDevice *dev, double softening, int32_t passes, double cutoff, double *g, double *potential,
+                     double *square, int32_t *square_n, double *half, double *ion_stencil, int32_t *reach,
+                     double *build_ms);
+
 /* Footprint in bytes that a scene of this shape would occupy on the device. */
 int64_t emb_scene_footprint(const EmbSettings *settings, int32_t cols, int32_t rows, int64_t n_electrons);
 
