@@ -121,8 +121,9 @@ def measured_traffic(phase: str):
     for line in open(path):
         if line.startswith("#") or not line.strip():
             continue
-        name, rd, wr = line.split()
-        table[name] = int(rd) + int(wr)
+        parts = line.split()
+        if len(parts) == 3:
+            table[parts[0]] = int(parts[1]) + int(parts[2])
     parts = [table.get(k) for k in PHASE_KERNELS.get(phase, [])]
     return sum(parts) if parts and all(p is not None for p in parts) else None
 

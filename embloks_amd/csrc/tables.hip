@@ -24,8 +24,7 @@
 
 namespace emb {
 
-namespace {
-
+// Kernels and helpers of this file only (k_tab_*).
 constexpr double TPI = 3.14159265358979323846;
 constexpr int T_SQUARE = 32;       // SQUARE_SAMPLES, close_range.rs:32
 constexpr int T_SOURCES = 8;       // SOURCE_SAMPLES, close_range.rs:35
@@ -44,14 +43,14 @@ struct TabGeom {
     double *part;                 // T_SOURCES^2 x count: each source's radial force / DIRECTIONS
 };
 
-__device__ double compact_d(double r2, double eps) {   // close_range.rs:41-49
+static __device__ double compact_d(double r2, double eps) {   // close_range.rs:41-49
     const double x = r2 / (eps * eps);
     const double v = x < 1e-4 ? 2.0 * (1.0 - x / 2.0 + x * x / 6.0) : -2.0 * expm1(-x) / x;
     return v / (eps * eps);
 }
 
 // numpy.interp over ascending xs (close_range.rs:56-65).
-__device__ double interp_d(double x, const double *xs, const double *ys, int len) {
+static __device__ double interp_d(double x, const double *xs, const double *ys, int len) {
     if (x <= xs[0]) return ys[0];
     if (x >= xs[len - 1]) return ys[len - 1];
     int lo = 0, hi = len;   // first index with xs > x
@@ -64,7 +63,7 @@ __device__ double interp_d(double x, const double *xs, const double *ys, int len
     return ys[i] + (ys[i + 1] - ys[i]) * (x - xs[i]) / (xs[i + 1] - xs[i]);
 }
 
-__device__ double radial_d(const double *table, double r2, double cutoff) {   // close_range.rs:174-181
+static __device__ double radial_d(const double *table, double r2, double cutoff) {   // close_range.rs:174-181
     if (r2 >= cutoff * cutoff) return 0.0;
     const double x = fmin(r2 * ((double)PAIR_SAMPLES / (cutoff * cutoff)), (double)PAIR_SAMPLES);
     const int i = (int)x;
@@ -83,7 +82,7 @@ __global__ void k_tab_matrices(int N, double *S, double *L) {
 }
 
 // Z = X Y for N x N row-major matrices, each entry summed over k in order.
-__device__ void block_matmul(int N, const double *X, const double *Y, double *Z) {
+static __device__ void block_matmul(int N, const double *X, const double *Y, double *Z) {
     for (int q = threadIdx.x; q < N * N; q += T_THREADS) {
         const int i = q / N, j = q % N;
         double s = 0.0;
@@ -93,7 +92,7 @@ __device__ void block_matmul(int N, const double *X, const double *Y, double *Z)
     __syncthreads();
 }
 
-__device__ double bilin_d(const double *a, int w, int h, double gx, double gy) {   // Maxwell::gather's lerp
+static __device__ double bilin_d(const double *a, int w, int h, double gx, double gy) {   // Maxwell::gather's lerp
     const long long i = min(max((long long)floor(gx), 0LL), (long long)w - 2);
     const long long j = min(max((long long)floor(gy), 0LL), (long long)h - 2);
     const double fx = gx - (double)i, fy = gy - (double)j;
@@ -229,7 +228,7 @@ struct SquareArgs {
     double *square;   // n^2 x 3
 };
 
-__device__ double z_at(const SquareArgs &a, int k) {
+static __device__ double z_at(const SquareArgs &a, int k) {
     return -a.half - 0.5 + ((double)k + 0.5) * (1.0 / T_SQUARE);
 }
 
@@ -316,7 +315,7 @@ __global__ void k_tab_ion(IonArgs a) {
     a.ion_sum[r] = sum;
 }
 
-int cdiv_i(long long a, int b) { return (int)((a + b - 1) / b); }
+static int cdiv_i(long long a, int b) { return (int)((a + b - 1) / b); }
 
 // Device buffers of one build, freed on every exit.
 struct TabMem {
@@ -334,7 +333,7 @@ struct TabMem {
 };
 
 // Build the tables for (softening, passes, cutoff) on the device.
-int build_on_device(EmbDevice *dev, double softening, int passes, double cutoff, std::shared_ptr<HostTables> &out) {
+static int build_on_device(EmbDevice *dev, double softening, int passes, double cutoff, std::shared_ptr<HostTables> &out) {
     if (!(softening > 0.0) || !std::isfinite(softening)) return fail(EMB_E_INVALID, "electron_softening must be positive");
     // The smoothed charge lands inside the margin only while passes <= margin.
     if (passes < 0 || passes > T_MARGIN) return fail(EMB_E_INVALID, "smoothing_passes must be in [0, 16]");
@@ -415,8 +414,6 @@ int build_on_device(EmbDevice *dev, double softening, int passes, double cutoff,
     out = t;
     return EMB_OK;
 }
-
-}  // namespace
 
 int ensure_tables(EmbDevice *dev, double softening, int passes, double cutoff) {
     if (!(cutoff > 0.0)) return EMB_OK;   // no close-range forces, no tables
