@@ -1776,8 +1776,9 @@ __global__ __launch_bounds__(64 * FDTD_WAVES) void k_fdtd(Batch b, int f) {
                     const int ni = t - 2;
                     if (own_col && tp >= row_lo && tp < row_hi && ni >= 0 && ni < nx && nj >= 0 && nj < ny) {
                         const size_t node = (size_t)ni * ny + nj;
-                        w1[node] = -four_pi_k * v.rho;
-                        w1[(size_t)nx * ny + node] = div;
+                        const float src = -four_pi_k * v.rho;
+                        w1[node] = src;
+                        w1[(size_t)nx * ny + node] = div + src;
                     }
                 }
                 ey_prev = ey_f;
@@ -1804,14 +1805,21 @@ __global__ __launch_bounds__(256) void k_rhs(Batch b) {
         const int i = q / ny, j = q - (q / ny) * ny;
         const float div = (S.ex[(size_t)(i + 1) * ny + j] - S.ex[(size_t)i * ny + j]) +
                           (S.ey[(size_t)i * (ny + 1) + j + 1] - S.ey[(size_t)i * (ny + 1) + j]);
-        S.w1[q] = -S.four_pi_k * S.rho[q];
-        S.w1[(size_t)nx * ny + q] = div;
+        const float src = -S.four_pi_k * S.rho[q];
+        S.w1[q] = src;
+        S.w1[(size_t)nx * ny + q] = div + src;
     }
 }
 
 // E -= grad(phi_potential + phi_correction), phi zero beyond the grid
-// (maxwell.rs:176-193, 200-204; fields.wgsl:206-237). With `tick` set, the
-// scene's tick counter advances here, after everything that reads it.
+// (maxwell.rs:176-193, 200-204; fields.wgsl:206-237). The Poisson solve is
+// linear, so the sum is solved as one potential: the second right-hand side is
+// div E - 4 pi k rho (the Gauss residual), not div E. The two potentials of the
+// reference nearly cancel; solved apart in FP32, each one's rounding (relative
+// to the full potential) would stay in their small sum, while the residual's
+// solve carries only one rounding of its right-hand side. The first channel,
+// P(-4 pi k rho), is the stored potential. With `tick` set, the scene's tick
+// counter advances here, after everything that reads it.
 __global__ __launch_bounds__(256) void k_correct(Batch b, int tick) {
     for (int blk = blockIdx.x; blk < b.total[KC_NODES]; blk += gridDim.x) {
         const int s = scene_of(b, KC_NODES, blk);
@@ -1826,7 +1834,7 @@ __global__ __launch_bounds__(256) void k_correct(Batch b, int tick) {
         const float *ex = alt ? S.ex2 : S.ex, *ey = alt ? S.ey2 : S.ey;
         if (q >= (nx + 1) * (ny + 1)) continue;
         const int i = q / (ny + 1), j = q - (q / (ny + 1)) * (ny + 1);
-        auto p = [&](int a, int c) { return S.phi[nodes + (size_t)a * ny + c] + S.phi[(size_t)a * ny + c]; };
+        auto p = [&](int a, int c) { return S.phi[nodes + (size_t)a * ny + c]; };
         if (j < ny) {
             const float high = i < nx ? p(i, j) : 0.0f, low = i > 0 ? p(i - 1, j) : 0.0f;
             S.ex[(size_t)i * ny + j] = ex[(size_t)i * ny + j] - (high - low);
@@ -1911,10 +1919,17 @@ hipError_t setup_kernels(const Batch &b, hipStream_t stream) {
     hipError_t e = sort_kernels(b, stream);
     if (e != hipSuccess) return e;
     EMB_LAUNCH(k_deposit_smooth, KC_SMOOTH, DEP_THREADS, b, -1);
-    EMB_LAUNCH(k_rhs, KC_NODES, 256, b);
-    e = launch_dst_gemms(b, stream);
-    if (e != hipSuccess) return e;
-    EMB_LAUNCH(k_correct, KC_NODES, 256, b, 0);
+    // Gauss's law enforced twice (Maxwell::enforce_gauss, maxwell.rs:176-193).
+    // From fields that are far from it (a new scene starts at E = 0) the first
+    // pass solves a full potential, whose FP32 rounding stays in E; the second
+    // solves the small residual that rounding left, and is a no-op in exact
+    // arithmetic, as on fields that already satisfy the law.
+    for (int pass = 0; pass < 2; pass++) {
+        EMB_LAUNCH(k_rhs, KC_NODES, 256, b);
+        e = launch_dst_gemms(b, stream);
+        if (e != hipSuccess) return e;
+        EMB_LAUNCH(k_correct, KC_NODES, 256, b, 0);
+    }
     return hipGetLastError();
 }
 
