@@ -515,3 +515,29 @@ def test_full_size_scene_one_tick_matches_the_oracle(before):
                 assert abs(g - c) < 5e-5, (g, c)
     finally:
         orc.set_threads(1)
+
+
+@gpu
+@pytest.mark.parametrize("index", [0, 4])
+def test_scenes_without_close_range_forces_follow_the_oracle(index):
+    # electron_cutoff = 0 turns the close-range forces off (CloseRange::enabled,
+    # close_range.rs:300-302): every electron takes the field-only push, no
+    # tables are built, and the battery terminals read no pair potentials.
+    world = W.scene(index, electron_cutoff=0.0)
+    cpu, sim = reset_and_compare(world, 0)
+    ex, ey, bz = sim.read_yee()
+    cex, cey, cbz = cpu.yee()
+    pos, _, _ = sim.read_state()
+    err = {"ex": relative_error(ex, cex), "ey": relative_error(ey, cey),
+           "phi": relative_error(sim.read_potential(), cpu.potential_nodes()),
+           "pos": float(np.max(np.abs(pos - cpu.state()[0])))}
+    sim.run_ticks(9)
+    cpu.step(9)
+    gex, gey, _ = sim.read_yee()
+    cex, cey, _ = cpu.yee()
+    err["ex_10"], err["ey_10"] = relative_error(gex, cex), relative_error(gey, cey)
+    print("cutoff 0, scene", index, err)
+    for k in ("ex", "ey", "phi"):
+        assert err[k] < 1e-5, err
+    assert err["pos"] < 1e-4, err
+    assert err["ex_10"] < 1e-4 and err["ey_10"] < 1e-4, err
