@@ -165,11 +165,17 @@ def candidate_pairs(pos: np.ndarray, cols: int, rows: int, cutoff: float) -> flo
 
 def push_compute(world, ms: float):
     """The push is bound by its pair arithmetic, not by HBM: FP32 flops of the
-    candidate pairs per second against the FP32 peak."""
+    candidate pairs per second against the FP32 peak. That peak counts packed
+    FP32 (two lanes' worth per instruction); the library is built without packed
+    FP32, so one VALU instruction does one FP32 operation per lane and half the
+    peak is the issue limit, reported beside it."""
     pairs = candidate_pairs(world.pos, world.cols, world.rows, world.settings["electron_cutoff"])
     achieved = FLOP_PER_PAIR * pairs / (ms * 1e-3) / 1e12
+    unpacked = FP32_MFMA_PEAK_TFLOPS / 2.0
     return {"bound": "valu", "achieved": round(achieved, 2), "peak": FP32_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
-            "frac": round(achieved / FP32_MFMA_PEAK_TFLOPS, 4), "candidate_pairs": pairs,
+            "frac": round(achieved / FP32_MFMA_PEAK_TFLOPS, 4),
+            "peak_without_packed_fp32": unpacked, "frac_without_packed_fp32": round(achieved / unpacked, 4),
+            "candidate_pairs": pairs,
             "flop_per_candidate_pair": FLOP_PER_PAIR, "candidate_pairs_per_s": round(pairs / (ms * 1e-3), 1)}
 
 
