@@ -33,8 +33,13 @@ namespace emb {
 
 // Fixed-point scales of the deterministic deposit: electron charge in units of
 // 2^-44 of one electron (CIC weights carry 22 fractional bits per axis), q*v in
-// units of 2^-28 (charge velocity of one electron). With two smoothing passes
-// (x 256) a cell holds at most 2^(63-44-8) = 2048 electron charges of either sign.
+// units of 2^-28 (charge velocity of one electron). The x pass of the smoothing
+// multiplies by at most 4^passes and the y pass by as much again: with two
+// passes (x 256) a node holds at most 2^(63-44-8) = 2048 electron charges of
+// either sign. Three or four passes run the y pass in FP64 instead (the int64
+// grid itself stays exact); a node past the int64 headroom of the passes done
+// in integers sets SceneCounters::charge_overflow (EMB_E_NUMERIC at the next
+// sync) instead of wrapping silently.
 constexpr int W_BITS = 22;
 constexpr double RHO_SCALE = 17592186044416.0;  // 2^44
 constexpr double J_SCALE = 268435456.0;         // 2^28
@@ -113,7 +118,8 @@ struct SceneCounters {
     int32_t heavy_n;          // cells listed in heavy_cells by the last sort
     int32_t substeps;         // electron substeps of the current tick
     int32_t rest_error;       // set when a k_rest wait ran past its limit (the tick's later substeps were dropped)
-    uint32_t pad[2];
+    int32_t charge_overflow;  // set when a node's fixed-point charge came near the int64 range in the smoothing
+    uint32_t pad[1];
 };
 
 struct DevScene {

@@ -548,9 +548,12 @@ __device__ __forceinline__ void ion_cell(const DevScene &S, const IonWeights &w,
 // bilinear lookups is one lookup in the table summed over the cell offsets.
 __device__ __forceinline__ void ion_uniform(const DevScene &S, int home, float px, float py, float &fx, float &fy) {
     const float u = (px - 0.5f + S.half) * S.square_samples, w = (py - 0.5f + S.half) * S.square_samples;
-    const int a = (int)u, bb = (int)w;
-    const float fu = u - (float)a, fv = w - (float)bb;
-    const float *tab = S.ion_sum + ((size_t)(a - S.ion_a0) * ION_SUM + (bb - S.ion_a0)) * 2;
+    // A fraction within one FP32 step of 1 rounds u up to the table's last sample
+    // (row ION_SUM - 1): start the lerp one sample lower with weight 1 on that
+    // row, so the reads stay inside the ION_SUM x ION_SUM table.
+    const int a = min(max((int)u - S.ion_a0, 0), ION_SUM - 2), bb = min(max((int)w - S.ion_a0, 0), ION_SUM - 2);
+    const float fu = u - (float)(a + S.ion_a0), fv = w - (float)(bb + S.ion_a0);
+    const float *tab = S.ion_sum + ((size_t)a * ION_SUM + bb) * 2;
     const float coef = S.k_ion_q * S.density[home];
     const float w00 = (1.0f - fu) * (1.0f - fv), w10 = fu * (1.0f - fv), w01 = (1.0f - fu) * fv, w11 = fu * fv;
     fx += coef * (tab[0] * w00 + tab[ION_SUM * 2] * w10 + tab[2] * w01 + tab[ION_SUM * 2 + 2] * w11);
@@ -1608,6 +1611,14 @@ __global__ __launch_bounds__(DEP_THREADS) void k_deposit_smooth(Batch b, int bia
             bin[q] = 0;
             for (int z = q; z > 0; z--) bin[z] += bin[z - 1];
         }
+        // Headroom: a pass of weight 4^P stays inside int64 while every input it
+        // sums is at most (2^63 - 1) / 4^P; the value at the centre of each output's
+        // stencil is checked (every value some stencil reads is the centre of
+        // another's). The x pass always runs in int64; the y pass does for P <= 2
+        // and in FP64 beyond (its FP32 result is all that leaves the tile). A node
+        // past the bound raises the scene's overflow flag.
+        const long long lim = (long long)(0x7fffffffffffffffULL >> (2 * P));
+        bool over = false;
 #pragma unroll
         for (int ch = 0; ch < 3; ch++) {
             const float scale = ch == 0 ? S.rho_from_fixed : S.j_from_fixed;
@@ -1617,6 +1628,8 @@ __global__ __launch_bounds__(DEP_THREADS) void k_deposit_smooth(Batch b, int bia
                 const int i = q / W, j = q - (q / W) * W;
                 long long acc = 0;
                 for (int d = 0; d <= 2 * P; d++) acc += bin[d] * dep[ch][(i + d) * SW + j];
+                const long long mid = dep[ch][(i + P) * SW + j];
+                over |= mid > lim || mid < -lim;
                 tmp[i * SW + j] = acc;
             }
             __syncthreads();
@@ -1624,12 +1637,21 @@ __global__ __launch_bounds__(DEP_THREADS) void k_deposit_smooth(Batch b, int bia
                 const int i = q / ST, j = q % ST;
                 const int a = R0x + i, rr = R0y + j;
                 if (a >= region_c || rr >= region_r) continue;
-                long long acc = 0;
-                for (int d = 0; d <= 2 * P; d++) acc += bin[d] * tmp[i * SW + j + d];
                 const size_t node = (size_t)(a - P + S.margin) * S.ny + (rr - P + S.margin);
-                out[node] = (float)acc * scale;
+                if (P <= 2) {
+                    long long acc = 0;
+                    for (int d = 0; d <= 2 * P; d++) acc += bin[d] * tmp[i * SW + j + d];
+                    const long long mid = tmp[i * SW + j + P];
+                    over |= mid > lim || mid < -lim;
+                    out[node] = (float)acc * scale;
+                } else {
+                    double acc = 0.0;
+                    for (int d = 0; d <= 2 * P; d++) acc += (double)bin[d] * (double)tmp[i * SW + j + d];
+                    out[node] = (float)acc * scale;
+                }
             }
         }
+        if (over) __hip_atomic_store(&S.ctr->charge_overflow, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __syncthreads();
     }
 }

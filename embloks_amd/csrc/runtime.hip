@@ -429,7 +429,8 @@ struct EmbSim {
     hipStream_t stream = nullptr;
     Resident r;
     double *diag = nullptr;
-    float2 *force = nullptr;   // emb_sim_close_force output, allocated on first use
+    float2 *force = nullptr;   // emb_sim_close_force output, grown to the electron count on use
+    size_t force_cap = 0;      // electrons `force` holds (an upload may raise the count)
     float *probe = nullptr;    // emb_sim_probe output: 32 words + 3 x 256 counts, allocated on first use
 };
 
@@ -595,10 +596,12 @@ int emb_sim_step(EmbSim *sim, int ticks) {
 int emb_sim_sync(EmbSim *sim) {
     if (!sim) return fail(EMB_E_INVALID, "null sim");
     EMB_HIP(hipStreamSynchronize(sim->stream));
-    // k_rest's wait limit (a queue fault, never expected) leaves a mark instead of a hang.
-    int32_t rest_error = 0;
-    EMB_HIP(hipMemcpy(&rest_error, &sim->r.ds.ctr->rest_error, sizeof(rest_error), hipMemcpyDeviceToHost));
-    if (rest_error) return fail(EMB_E_NUMERIC, "electron substeps after the first were dropped: k_rest wait limit");
+    // k_rest's wait limit (a queue fault, never expected) leaves a mark instead of
+    // a hang; a charge grid past the fixed-point headroom leaves another.
+    int32_t flags[2] = {0, 0};
+    EMB_HIP(hipMemcpy(flags, &sim->r.ds.ctr->rest_error, sizeof(flags), hipMemcpyDeviceToHost));
+    const std::string why = numeric_fault(flags[0], flags[1]);
+    if (!why.empty()) return fail(EMB_E_NUMERIC, why);
     return EMB_OK;
 }
 
@@ -903,7 +906,15 @@ int emb_sim_close_force(EmbSim *sim, int32_t repeats, double *force, double *avg
         if (sim->r.ds.close_on == 0) return fail(EMB_E_STATE, "the close-range force is off (electron_cutoff = 0)");
         const size_t n = (size_t)sim->r.ds.n;
         EMB_HIP(hipSetDevice(sim->dev->ordinal));
-        if (!sim->force && n > 0) EMB_HIP(hipMalloc(&sim->force, n * sizeof(float2)));
+        if (n > sim->force_cap) {
+            // An upload can add electrons after an earlier call sized the buffer.
+            EMB_HIP(hipStreamSynchronize(sim->stream));
+            if (sim->force) EMB_HIP(hipFree(sim->force));
+            sim->force = nullptr;
+            sim->force_cap = 0;
+            EMB_HIP(hipMalloc(&sim->force, n * sizeof(float2)));
+            sim->force_cap = n;
+        }
         hipEvent_t e0, e1;
         EMB_HIP(hipEventCreate(&e0));
         EMB_HIP(hipEventCreate(&e1));

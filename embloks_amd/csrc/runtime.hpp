@@ -99,6 +99,14 @@ size_t scene_bytes(const EmbSettings &s, int cols, int rows, int64_t n, int n_st
 int make_resident(EmbDevice *dev, const EmbSettings &s, HostScene &&hs, hipStream_t stream, Resident &out);
 void free_resident(Resident &r);
 
+// The message of a scene's device-side numeric flags (SceneCounters::rest_error,
+// charge_overflow), empty when neither is set.
+inline std::string numeric_fault(int32_t rest_error, int32_t charge_overflow) {
+    if (rest_error) return "electron substeps after the first were dropped: k_rest wait limit";
+    if (charge_overflow) return "a node's charge passed the fixed-point range of the smoothing (too many electrons in one place)";
+    return std::string();
+}
+
 // Electron positions (n x 2, FP64) of a resident scene; synchronous.
 int read_positions(const DevScene &d, double *pos);
 

@@ -42,7 +42,7 @@ struct Request {
     std::vector<double> pos;                   // kept results
     std::vector<float> vel, ex, ey, bz;
     bool has_result = false;
-    bool rest_error = false;                   // k_rest's wait limit hit during the run
+    std::string fault;                         // a device-side numeric flag raised during the run
 };
 
 double now_s() {
@@ -142,12 +142,12 @@ int capture_graph(EmbServer *sv) {
 
 int retire(EmbServer *sv, Request *q) {
     Resident &r = q->r;
-    if (q->state == RUNNING) {   // a queue fault of k_rest, never expected, fails the request loudly
-        int32_t flag = 0;
-        EMB_HIP(hipMemcpy(&flag, &r.ds.ctr->rest_error, sizeof(flag), hipMemcpyDeviceToHost));
-        q->rest_error = flag != 0;
+    if (q->state == RUNNING) {   // a numeric flag (k_rest's wait limit, charge range) fails the request loudly
+        int32_t flags[2] = {0, 0};
+        EMB_HIP(hipMemcpy(flags, &r.ds.ctr->rest_error, sizeof(flags), hipMemcpyDeviceToHost));
+        q->fault = numeric_fault(flags[0], flags[1]);
     }
-    if (q->state == RUNNING && sv->cfg.keep_results && !q->rest_error) {
+    if (q->state == RUNNING && sv->cfg.keep_results && q->fault.empty()) {
         const DevScene &d = r.ds;
         const size_t n = (size_t)d.n, nx = d.nx, ny = d.ny;
         q->pos.resize(2 * n);
@@ -398,7 +398,7 @@ int emb_server_result(EmbServer *sv, int64_t id, double *pos, double *vel, doubl
     if (it == sv->all.end()) return fail(EMB_E_INVALID, "no such request");
     Request *q = it->second.get();
     if (q->state != DONE) return fail(EMB_E_STATE, "request not finished");
-    if (q->rest_error) return fail(EMB_E_NUMERIC, "electron substeps after the first were dropped: k_rest wait limit");
+    if (!q->fault.empty()) return fail(EMB_E_NUMERIC, q->fault);
     if (!q->has_result) return fail(EMB_E_STATE, "result not kept (keep_results = 0) or already read");
     auto out = [](double *dst, const auto &src) {
         if (dst)

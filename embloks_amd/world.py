@@ -81,18 +81,31 @@ class World:
         old = self.density.copy()
         c, r = cells[:, 0], cells[:, 1]
         conductive = material in (CONDUCTOR, BATTERY)
+        new_density = density if conductive else 0.0
+        # The islands before the paint decide where removed carriers come from
+        # (world.rs:80-91); only needed when some cell loses density.
+        old_labels = self.labels() if (old[c, r] > new_density).any() else None
         self.kind[c, r] = material
         self.resistance[c, r] = resistance if conductive else 0.0
-        self.density[c, r] = density if conductive else 0.0
+        self.density[c, r] = new_density
         self.emf[c, r] = emf if material == BATTERY else 0.0
         self.orientation[c, r] = orientation if material == BATTERY else 0
         self.frequency[c, r] = frequency if material == BATTERY else 0.0
         n = s["carriers_per_cell"]
         change = np.rint((self.density - old) * n).astype(np.int64)
         if (change < 0).any():
-            self._remove_carriers(np.maximum(-change, 0))
+            self._remove_carriers(np.maximum(-change, 0), old_labels)
         if (change > 0).any():
             self.spawn(np.maximum(change, 0), jitter)
+
+    def labels(self) -> np.ndarray:
+        """Island id per cell (-1 off conductors): conductive cells joined through
+        shared edges, never corners, numbered in scan order of their first cell
+        (columns outer, connectivity.rs:20-62)."""
+        from scipy import ndimage
+
+        lab, _ = ndimage.label(self.kind > 0)   # 4-connected; features numbered in C (scan) order
+        return lab.astype(np.int64) - 1
 
     def spawn(self, counts: np.ndarray, jitter=SPAWN_JITTER):
         """Add counts[c, r] electrons per cell, spread over it (world.rs:148-164)."""
@@ -108,15 +121,49 @@ class World:
             self.pos = np.concatenate([self.pos, added])
             self.vel = np.concatenate([self.vel, np.zeros_like(added)])
 
-    def _remove_carriers(self, counts: np.ndarray):
-        """Take away counts[c, r] electrons from the cells that lost density: those in
-        the cells themselves first (a simplified world.rs:180-238, enough for scenes
-        that overpaint a battery with wire)."""
+    def _remove_carriers(self, counts: np.ndarray, old_labels: np.ndarray):
+        """Take away counts[c, r] electrons for the cells that lost density
+        (world.rs:180-238). Per island of the old materials: the electrons in the
+        cells that lost density leave first, those in erased (now non-conductive)
+        cells before the others; if they fall short of the island's quota, the
+        electrons nearest to those cells' centres follow. Electrons still in
+        erased cells then move to the nearest remaining conductive cell of their
+        old island (with the spawn jitter and zero velocity), or vanish if nothing
+        of it is left."""
+        n = len(self.pos)
+        conductive = self.kind > 0
         cell = np.clip(np.floor(self.pos).astype(np.int64), 0, [self.cols - 1, self.rows - 1])
-        keep = np.ones(len(self.pos), bool)
-        for c, r in np.argwhere(counts > 0):
-            idx = np.nonzero((cell[:, 0] == c) & (cell[:, 1] == r) & keep)[0][: counts[c, r]]
-            keep[idx] = False
+        island = old_labels[cell[:, 0], cell[:, 1]]
+        lost = counts > 0
+        in_lost = lost[cell[:, 0], cell[:, 1]]
+        stranded = ~conductive[cell[:, 0], cell[:, 1]]
+        keep = np.ones(n, bool)
+        for isl in np.unique(old_labels[lost]):
+            lost_here = np.argwhere(lost & (old_labels == isl))   # scan order
+            quota = int(counts[lost_here[:, 0], lost_here[:, 1]].sum())
+            members = island == isl
+            inside = np.nonzero(members & in_lost)[0]
+            inside = inside[np.argsort(~stranded[inside], kind="stable")]   # erased cells first
+            keep[inside[:quota]] = False
+            if len(inside) < quota:
+                others = np.nonzero(members & ~in_lost)[0]
+                centres = lost_here + 0.5
+                d2 = ((self.pos[others, None, :] - centres[None, :, :]) ** 2).sum(axis=2)
+                dist = np.sqrt(d2.min(axis=1))
+                nearest = others[np.argsort(dist, kind="stable")]
+                keep[nearest[: quota - len(inside)]] = False
+            leftover = np.nonzero(keep & members & stranded)[0]
+            if len(leftover):
+                targets = np.argwhere(conductive & (old_labels == isl))
+                if len(targets) == 0:
+                    keep[leftover] = False
+                else:
+                    centres = targets + 0.5
+                    for i in leftover:
+                        d2 = ((centres - self.pos[i]) ** 2).sum(axis=1)
+                        j = self.rng.uniform(-SPAWN_JITTER, SPAWN_JITTER, size=2)
+                        self.pos[i] = centres[int(np.argmin(d2))] + j
+                        self.vel[i] = 0.0
         self.pos, self.vel = self.pos[keep], self.vel[keep]
 
     def place_charges(self, cells, charge):
