@@ -82,3 +82,56 @@ def test_painting_a_wire_with_carriers_then_ticking_follows_the_oracle():
            "pos": float(np.max(np.abs(gpos - cpu.state()[0])))}
     print(err)
     assert err["ex"] < 1e-5 and err["ey"] < 1e-5 and err["pos"] < 1e-4, err
+
+
+@gpu
+def test_erasing_part_of_a_wire_then_ticking_follows_the_oracle():
+    # World::paint with vacuum over a running circuit (world.rs:180-238: the
+    # erased cells' electrons leave by island quota, stranded ones move), then
+    # GpuSim::upload and one tick against the oracle from the uploaded state.
+    world = W.scene(0)
+    sim = gpusim()(world)
+    sim.run_ticks(30)
+    take_state(sim, world)
+    n0 = world.n_electrons
+    world.paint(W.line(20, 22, 24, 24), W.VACUUM)   # cut the loop's bottom wire
+    assert world.n_electrons == n0 - 3 * world.settings["carriers_per_cell"]
+    sim.upload(world, reset_display=False)
+    assert sim.dims().n_electrons == world.n_electrons
+    pos, vel, _ = sim.read_state()
+    cell = np.floor(pos).astype(np.int64)
+    assert np.all(world.kind[cell[:, 0], cell[:, 1]] > 0)
+    ex, ey, bz = sim.read_yee()
+    cpu = orc.Sim(world)
+    cpu.write_state(pos, vel, ex, ey, bz)
+    cpu.step(1)
+    sim.run_ticks(1)
+    gex, gey, _ = sim.read_yee()
+    cex, cey, _ = cpu.yee()
+    err = {"ex": relative_error(gex, cex), "ey": relative_error(gey, cey),
+           "pos": float(np.max(np.abs(sim.read_state()[0] - cpu.state()[0])))}
+    print(err)
+    assert err["ex"] < 1e-5 and err["ey"] < 1e-5 and err["pos"] < 1e-4, err
+
+
+@gpu
+def test_close_force_after_an_upload_that_adds_electrons():
+    # The close-force output grows with the electron count an upload brings.
+    from tests.test_gpu_p3m import FIXTURE_FORCE_TOL, check_forces
+
+    world = W.scene(4)
+    sim = gpusim()(world)
+    sim.close_force()
+    n0 = world.n_electrons
+    world.paint(W.block(2, 4, 2, 26), W.CONDUCTOR, resistance=0.5)
+    assert world.n_electrons > n0
+    sim.upload(world)
+    got, _ = sim.close_force()
+    assert got.shape == (world.n_electrons, 2)
+    pos = sim.read_state()[0]
+    w = world.copy()
+    w.pos = pos
+    want = orc.Sim(w).close_force()
+    err, edge = check_forces(w, got, want, tol=FIXTURE_FORCE_TOL)
+    print("force error", err, "boundary pairs", edge)
+    assert err < FIXTURE_FORCE_TOL, err

@@ -24,11 +24,16 @@ def force_error(got, want):
 # In neutral conductors an electron's close-range force is the small remainder
 # of its ions' attraction and its neighbours' repulsion (each ~10-100x larger),
 # summed over ~100-2000 FP32 pair terms: FP32 rounding of the parts, not of the
-# remainder, sets the error relative to max |F|.
-FORCE_TOL = 5e-5
+# remainder, sets the error relative to max |F|: 1.3e-6 to 1.9e-6 measured on the
+# clustered scenes and the 4M-electron sample, held to the field target 1e-5.
+FORCE_TOL = 1e-5
+# The 40 x 32 example scenes' battery cells hold 32 electrons on an ion of
+# density 4: there the remainder is ~100x smaller than its parts and FP32 rounding
+# of the parts reaches 0.9e-5 to 1.25e-5 of max |F| (measured); held to 2e-5.
+FIXTURE_FORCE_TOL = 2e-5
 
 
-def check_forces(world, got, want, idx=None):
+def check_forces(world, got, want, idx=None, tol=FORCE_TOL, max_edge=None):
     """Per-electron error against the oracle. The reference's pair force stops
     at r = cutoff with g(cutoff^2) != 0 (close_range.rs:359-372), so a pair
     within FP32 rounding of the cutoff is in the sum on one side and not the
@@ -38,11 +43,11 @@ def check_forces(world, got, want, idx=None):
     idx = np.arange(world.n_electrons) if idx is None else idx
     err = np.abs(got - want).max(axis=1) / np.abs(want).max()
     cut2 = world.settings["electron_cutoff"] ** 2
-    outliers = np.flatnonzero(err > FORCE_TOL)
+    outliers = np.flatnonzero(err > tol)
     for k in outliers:
         d2 = ((world.pos - world.pos[idx[k]]) ** 2).sum(axis=1)
         assert np.any(np.abs(d2 - cut2) < 1e-6 * cut2), (idx[k], err[k])
-    assert len(outliers) <= max(2, len(idx) // 1000), len(outliers)
+    assert len(outliers) <= (max(2, len(idx) // 1000) if max_edge is None else max_edge), len(outliers)
     rest = np.delete(err, outliers)
     return float(rest.max()), len(outliers)
 
@@ -53,9 +58,9 @@ def test_fixture_scenes_cutoff_3(index):
     world = W.scene(index)
     want = orc.Sim(world).close_force()
     got, _ = gpusim()(world).close_force()
-    err, edge = check_forces(world, got, want)
+    err, edge = check_forces(world, got, want, tol=FIXTURE_FORCE_TOL)
     print("scene", index, "force error", force_error(got, want), "without cutoff-boundary pairs", err, "boundary", edge)
-    assert err < FORCE_TOL, err
+    assert err < FIXTURE_FORCE_TOL, err
 
 
 @gpu
