@@ -9,6 +9,26 @@
 #include <cmath>
 #include <cstring>
 
+// Partial sums of emb_sim_diagnostics_static, one array per GPU for the life of
+// the process: DIAG_BLOCKS (residual max, energy) pairs from k_diagnostics, then
+// the two reduced values. Host code only ever uses its device address from
+// hipGetSymbolAddress (EmbDevice::scratch), never this name.
+constexpr int DIAG_BLOCKS = 1024;
+static __device__ __align__(256) double scratch[2 * DIAG_BLOCKS + 2];
+
+// The block partials reduced in block order, as emb_sim_diagnostics reduces them
+// on the host: residual max and energy sum into p[2 * blocks], p[2 * blocks + 1].
+__global__ void k_diag_reduce(double *p, int blocks) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double res = 0.0, en = 0.0;
+    for (int b = 0; b < blocks; b++) {
+        res = fmax(res, p[2 * b]);
+        en += p[2 * b + 1];
+    }
+    p[2 * blocks] = res;
+    p[2 * blocks + 1] = en;
+}
+
 namespace emb {
 
 static thread_local std::string g_error;
@@ -475,9 +495,13 @@ int emb_device_open(int ordinal, EmbDevice **out) {
         EMB_HIP(hipGetDeviceProperties(&p, ordinal));
         if (std::strncmp(p.gcnArchName, "gfx950", 6) != 0)
             return fail(EMB_E_DEVICE, std::string("built for gfx950, device is ") + p.gcnArchName);
+        // This device's address of the diagnostics scratch array (the device is current).
+        void *scr = nullptr;
+        EMB_HIP(hipGetSymbolAddress(&scr, HIP_SYMBOL(scratch)));
         auto *d = new EmbDevice();
         d->ordinal = ordinal;
         d->n_cu = p.multiProcessorCount;
+        d->scratch = static_cast<double *>(scr);
         *out = d;
         return EMB_OK;
     })
@@ -830,7 +854,7 @@ int emb_sim_diagnostics(EmbSim *sim, double *gauss_residual, double *electron_ch
     EMB_GUARD({
         if (!sim) return fail(EMB_E_INVALID, "null sim");
         Resident &r = sim->r;
-        const int blocks = 1024;
+        const int blocks = DIAG_BLOCKS;
         if (!sim->diag) EMB_HIP(hipMalloc(&sim->diag, 2 * blocks * sizeof(double)));
         EMB_HIP(diagnostics_kernel(r.d_scene, r.s.speed_of_light * r.s.speed_of_light, sim->diag, blocks, sim->stream));
         std::vector<double> out(2 * blocks);
@@ -856,6 +880,25 @@ int emb_sim_diagnostics(EmbSim *sim, double *gauss_residual, double *electron_ch
         }
         return EMB_OK;
     })
+}
+
+int emb_sim_diagnostics_static(EmbSim *sim, double *gauss_residual, double *field_energy) {
+    if (!sim) return fail(EMB_E_INVALID, "null sim");
+    EmbDevice *dev = sim->dev;
+    if (!dev->scratch) return fail(EMB_E_STATE, "device has no diagnostics scratch address");
+    // k_diagnostics is grid-stride over any grid with DIAG_BLOCKS blocks: every
+    // scene fits the fixed scratch, and every partial it holds is rewritten here.
+    EMB_HIP(hipSetDevice(dev->ordinal));
+    Resident &r = sim->r;
+    EMB_HIP(diagnostics_kernel(r.d_scene, r.s.speed_of_light * r.s.speed_of_light, dev->scratch, DIAG_BLOCKS, sim->stream));
+    hipLaunchKernelGGL(k_diag_reduce, dim3(1), dim3(64), 0, sim->stream, dev->scratch, DIAG_BLOCKS);
+    EMB_HIP(hipGetLastError());
+    double out[2] = {0.0, 0.0};
+    EMB_HIP(hipMemcpyAsync(out, dev->scratch + 2 * DIAG_BLOCKS, sizeof(out), hipMemcpyDeviceToHost, sim->stream));
+    EMB_HIP(hipStreamSynchronize(sim->stream));
+    if (gauss_residual) *gauss_residual = out[0];
+    if (field_energy) *field_energy = out[1];
+    return EMB_OK;
 }
 
 int emb_sim_kernel_times(EmbSim *sim, int ticks, double *avg_ms, int capacity) {

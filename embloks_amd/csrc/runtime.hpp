@@ -58,6 +58,7 @@ struct SineSet {
 struct EmbDevice {
     int ordinal = 0;
     int n_cu = 0;
+    double *scratch = nullptr;   // this device's copy of runtime.hip's `scratch` array (hipGetSymbolAddress)
     std::mutex lock;
     std::map<int, emb::SineSet> sine;   // DST-I matrices per length, shared by every scene
     std::map<std::tuple<double, int, double>, emb::DeviceTables> tables;

@@ -42,6 +42,7 @@ SIGNATURES = {
     "emb_sim_set_current": (C.c_int, [VP, D]),
     "emb_sim_read_charge_fixed": (C.c_int, [VP, I64]),
     "emb_sim_diagnostics": (C.c_int, [VP, D, D, D]),
+    "emb_sim_diagnostics_static": (C.c_int, [VP, D, D]),
     "emb_sim_kernel_times": (C.c_int, [VP, C.c_int, D, C.c_int]),
     "emb_sim_close_force": (C.c_int, [VP, C.c_int32, D, D]),
     "emb_tick_phase_name": (C.c_char_p, [C.c_int]),
@@ -239,6 +240,13 @@ class GpuSim:
         r, q, e = C.c_double(), C.c_double(), C.c_double()
         check(lib().emb_sim_diagnostics(self.h, C.byref(r), C.byref(q), C.byref(e)))
         return r.value, q.value, e.value
+
+    def diagnostics_static(self):
+        """(Gauss residual, field energy) as diagnostics() gives them, through the
+        entry point that allocates nothing (a per-GPU static scratch array)."""
+        r, e = C.c_double(), C.c_double()
+        check(lib().emb_sim_diagnostics_static(self.h, C.byref(r), C.byref(e)))
+        return r.value, e.value
 
     def close_force(self, repeats: int = 1):
         """CloseRange::force (close_range.rs:314-376) on every electron at its

@@ -178,6 +178,18 @@ int emb_sim_read_charge_fixed(EmbSim *sim, int64_t *rho);
  * total deposited electron charge (exact, in electron charges), field energy
  * sum(Ex^2) + sum(Ey^2) + c^2 sum(Bz^2). No counterpart in the reference. */
 int emb_sim_diagnostics(EmbSim *sim, double *gauss_residual, double *electron_charge, double *field_energy);
+/* emb_sim_diagnostics' Gauss residual and field energy (the same values, summed
+ * in the same order) without allocating anything: no hipMalloc, hipFree or
+ * allocating library call. No counterpart in the reference: for callers that
+ * cannot allocate (code under HIP graph capture, serving loops that check
+ * Gauss's law every N ticks, harnesses that forbid allocation). The per-block
+ * partials live in a file-scope device array of 2 x 1024 + 2 doubles (16 KiB)
+ * per GPU, held until the process ends; the reduction is grid-stride, so every
+ * scene fits it. The exact electron charge is not offered (it needs a deposit
+ * dump as large as the grid). Every call on a GPU shares that GPU's array:
+ * calls that overlap on one GPU (several threads, or simulations on different
+ * streams) race. */
+int emb_sim_diagnostics_static(EmbSim *sim, double *gauss_residual, double *field_energy);
 /* Run `ticks` more ticks with a HIP event between the kernel phases of each tick
  * on the sim's stream and write each phase's average duration (ms) to avg_ms.
  * Returns the number of phases (avg_ms NULL: just the count). Names from
