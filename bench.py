@@ -375,7 +375,11 @@ def cpu_baseline(world, budget_s: float = 20.0):
         return (time.perf_counter() - t0) / ticks, ticks
 
     one, n1 = timed(1, budget_s * 0.5)
-    cores = max(1, min(16, os.cpu_count() or 1))
+    # Every core of this process's share: the affinity mask and OMP_NUM_THREADS
+    # (a GPU host's share is set by its scheduler), not the machine's count.
+    share = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    omp = os.environ.get("OMP_NUM_THREADS")
+    cores = max(1, min(share, int(omp)) if omp and omp.isdigit() else share)
     many, nm = timed(cores, budget_s * 0.25)
     orc.set_threads(1)
     tables_ms = {}
@@ -383,19 +387,25 @@ def cpu_baseline(world, budget_s: float = 20.0):
         t0 = time.perf_counter()
         orc.tables(soft, passes, cutoff * (1.0 + 1e-12))   # a parameter set the oracle has not cached
         tables_ms[f"softening {soft}, passes {passes}, cutoff {cutoff}"] = round((time.perf_counter() - t0) * 1e3, 1)
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
+    except OSError:
+        pass
     return {"value": round(nodes / many, 1), "unit": "cell-steps/s", "cores": cores, "kind": "port",
+            "cpu_model": model, "host_cpus": os.cpu_count(), "cpu_share": share,
             "single_thread_value": round(nodes / one, 1), "close_range_tables_ms": tables_ms,
             "sample": (f"FP64 oracle, same 512x512 scene: {cores} threads {nm} ticks after 1 warm-up, "
                        f"{many * 1e3:.0f} ms/tick; 1 thread {n1} ticks, {one * 1e3:.0f} ms/tick")}
 
 
-def run_serving(n_requests: int, steps_scale: float, batched: bool):
+def run_serving(stream, batched: bool):
+    """Feed `stream` (arrival s, steps, world) to a server at its arrival times
+    (wall clock from the first submit) and poll until every request is done."""
     from embloks_amd import abi
-    from embloks_amd import world as W
     from embloks_amd.gpusim import Server
 
-    stream = W.serving_stream(n_requests=n_requests, seed=5,
-                              min_steps=max(1, int(1000 * steps_scale)), max_steps=max(2, int(20000 * steps_scale)))
     srv = Server(abi.settings(carriers_per_cell=1), hbm_budget_bytes=64 << 30, max_slots=256, batched=batched,
                  ticks_per_round=16, use_graph=batched)
     t0 = time.perf_counter()
@@ -435,8 +445,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
-    ap.add_argument("--serve-requests", type=int, default=48)
-    ap.add_argument("--serve-scale", type=float, default=0.02, help="fraction of each request's 1000-20000 steps")
+    ap.add_argument("--serve-requests", type=int, default=256)
+    ap.add_argument("--serve-scale", type=float, default=0.1, help="fraction of each request's 1000-20000 steps")
     ap.add_argument("--no-poisson", action="store_true", help="skip the isolated 2048^2 DST solve")
     ap.add_argument("--no-p3m", action="store_true", help="skip the 4096^2 clustered close-range force")
     ap.add_argument("--no-tables", action="store_true", help="skip the close-range table build")
@@ -494,16 +504,27 @@ def main():
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(world)
     if not args.no_serve:
+        from embloks_amd import world as W
+
         serve = {}
+        stream = W.serving_stream(n_requests=args.serve_requests, seed=5,
+                                  min_steps=max(1, int(1000 * args.serve_scale)),
+                                  max_steps=max(2, int(20000 * args.serve_scale)))
+        print(f"serving stream: {len(stream)} requests generated", file=sys.stderr, flush=True)
         for mode in ("batched", "plain"):
-            st = run_serving(args.serve_requests, args.serve_scale, mode == "batched")
+            st = run_serving(stream, mode == "batched")
             print(f"serving {mode}: {st['cell_steps_per_s'] / 1e9:.3f} G cell-steps/s", file=sys.stderr, flush=True)
             serve[mode] = {"cell_steps_per_s": round(st["cell_steps_per_s"], 1),
                            "p50_ms": round(st["step_latency_p50_ms"], 4), "p99_ms": round(st["step_latency_p99_ms"], 4),
+                           "request_p50_ms": round(st["request_latency_p50_ms"], 1),
+                           "request_p99_ms": round(st["request_latency_p99_ms"], 1),
+                           "queue_wait_p50_ms": round(st["queue_wait_p50_ms"], 1),
+                           "queue_wait_p99_ms": round(st["queue_wait_p99_ms"], 1),
                            "completed": st["completed"], "peak_resident": st["peak_resident"],
                            "elapsed_s": round(st["elapsed_s"], 3)}
         b, p = serve["batched"]["cell_steps_per_s"], serve["plain"]["cell_steps_per_s"]
         serve["speedup"] = round(b / p, 2) if p > 0 else None
+        serve["speedup_target"] = 5.0
         serve["stream"] = (f"{args.serve_requests} requests, Poisson arrivals mean 2 ms, sides log-uniform 64-1024, "
                            f"steps uniform {max(1, int(1000 * args.serve_scale))}-{max(2, int(20000 * args.serve_scale))}")
         out["serving"] = serve

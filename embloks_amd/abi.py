@@ -126,6 +126,10 @@ class EmbServerStats(C.Structure):
         ("step_samples", C.c_int64),
         ("peak_resident", C.c_int32),
         ("reserved0", C.c_int32),
+        ("request_latency_p50_ms", C.c_double),
+        ("request_latency_p99_ms", C.c_double),
+        ("queue_wait_p50_ms", C.c_double),
+        ("queue_wait_p99_ms", C.c_double),
     ]
 
 

@@ -1486,6 +1486,7 @@ __global__ __launch_bounds__(DEP_THREADS) void k_deposit_smooth(Batch b, int bia
             dep[1][i * SW + j] = 0;
             dep[2][i * SW + j] = 0;
         }
+        __syncthreads();   // every entry initialised before any wave adds to it
         // Electrons whose CIC corners reach the window live in cells [w0 - 1, w0 + W].
         // One thread per such source cell sums what the cell's electrons put on
         // the 3 x 3 nodes around it (CIC, fields.rs:27-40: the lower corner is

@@ -21,6 +21,7 @@
 struct EmbPoisson {
     EmbDevice *dev = nullptr;
     int nx = 0, ny = 0;
+    int sine_x = 0, sine_y = 0;   // sine lengths referenced (0: none)
     hipStream_t stream = nullptr;
     float *rhs = nullptr, *w1 = nullptr, *w2 = nullptr, *phi = nullptr, *inv_lap = nullptr;
     const float *sx = nullptr, *sy = nullptr;                                   // whole DST-I matrices
@@ -139,6 +140,8 @@ int emb_poisson_create(EmbDevice *dev, int32_t nx, int32_t ny, EmbPoisson **out)
             if (p->e0) (void)hipEventDestroy(p->e0);
             if (p->e1) (void)hipEventDestroy(p->e1);
             if (p->stream) (void)hipStreamDestroy(p->stream);
+            if (p->sine_x) emb::release_sine(p->dev, p->sine_x);
+            if (p->sine_y) emb::release_sine(p->dev, p->sine_y);
             delete p;
             return rc;
         };
@@ -150,8 +153,11 @@ int emb_poisson_create(EmbDevice *dev, int32_t nx, int32_t ny, EmbPoisson **out)
             return bail(fail(EMB_E_DEVICE, "stream or event creation failed"));
         const emb::SineSet *set_x = nullptr, *set_y = nullptr;
         int rc = emb::device_sine(dev, nx, &set_x);
-        if (rc == EMB_OK) rc = emb::device_sine(dev, ny, &set_y);
         if (rc != EMB_OK) return bail(rc);
+        p->sine_x = nx;   // the solver holds both lengths until destroyed
+        rc = emb::device_sine(dev, ny, &set_y);
+        if (rc != EMB_OK) return bail(rc);
+        p->sine_y = ny;
         p->sx = set_x->full;
         p->sy = set_y->full;
         p->sx_e = set_x->e;
@@ -244,6 +250,8 @@ int emb_poisson_destroy(EmbPoisson *p) {
         if (p->e0) (void)hipEventDestroy(p->e0);
         if (p->e1) (void)hipEventDestroy(p->e1);
         if (p->stream) (void)hipStreamDestroy(p->stream);
+        if (p->sine_x) emb::release_sine(p->dev, p->sine_x);
+        if (p->sine_y) emb::release_sine(p->dev, p->sine_y);
         delete p;
         return EMB_OK;
     })

@@ -7,6 +7,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 
 // Partial sums of emb_sim_diagnostics_static, one array per GPU for the life of
@@ -139,7 +140,7 @@ template <class T> hipError_t put(const T *dst, const std::vector<T> &src, hipSt
 }
 
 // The DST-I matrix of length n and its halves on the device, built once per
-// device; the caller holds dev->lock.
+// device and shared, with a reference taken; the caller holds dev->lock.
 int sine_locked(EmbDevice *dev, int n, const SineSet **out) {
     auto it = dev->sine.find(n);
     if (it == dev->sine.end()) {
@@ -159,22 +160,39 @@ int sine_locked(EmbDevice *dev, int n, const SineSet **out) {
             for (float *p : {set.full, set.e, set.e_t, set.o}) (void)hipFree(p);
             return fail(EMB_E_DEVICE, std::string("sine matrices: ") + hipGetErrorString(e));
         }
+        set.bytes = sine_set_bytes(n);
+        dev->sine_bytes += set.bytes;
         it = dev->sine.emplace(n, set).first;
     }
+    it->second.refs++;
+    it->second.last_use = ++dev->sine_clock;
     *out = &it->second;
     return EMB_OK;
 }
 
-int shared_tables(EmbDevice *dev, const EmbSettings &s, int nx, int ny, DevScene &d, DeviceTables &tab) {
-    std::lock_guard<std::mutex> guard(dev->lock);
-    const SineSet *sx = nullptr, *sy = nullptr;
-    int rc = sine_locked(dev, nx, &sx);
-    if (rc == EMB_OK) rc = sine_locked(dev, ny, &sy);
-    if (rc != EMB_OK) return rc;
-    d.sx_e = sx->e;
-    d.sx_o = sx->o;
-    d.sy_et = sy->e_t;
-    d.sy_o = sy->o;
+void release_locked(EmbDevice *dev, int n) {
+    auto it = dev->sine.find(n);
+    if (it == dev->sine.end() || it->second.refs <= 0) return;
+    it->second.refs--;
+    it->second.last_use = ++dev->sine_clock;
+}
+
+void trim_locked(EmbDevice *dev, size_t keep) {
+    while (dev->sine_bytes > keep) {
+        auto victim = dev->sine.end();
+        for (auto it = dev->sine.begin(); it != dev->sine.end(); ++it)
+            if (it->second.refs == 0 && (victim == dev->sine.end() || it->second.last_use < victim->second.last_use))
+                victim = it;
+        if (victim == dev->sine.end()) return;
+        for (float *p : {victim->second.full, victim->second.e, victim->second.e_t, victim->second.o}) (void)hipFree(p);
+        dev->sine_bytes -= victim->second.bytes;
+        dev->sine.erase(victim);
+    }
+}
+
+// The close-range tables of the settings on the device, uploaded once per
+// parameter set and kept; the caller holds dev->lock.
+int close_tables_locked(EmbDevice *dev, const EmbSettings &s, DeviceTables &tab) {
     const auto key = std::make_tuple(s.electron_softening, (int)s.smoothing_passes, s.electron_cutoff);
     auto it = dev->tables.find(key);
     if (it == dev->tables.end()) {
@@ -220,11 +238,65 @@ int shared_tables(EmbDevice *dev, const EmbSettings &s, int nx, int ny, DevScene
     return EMB_OK;
 }
 
+// Tables and sine sets of a scene of nx x ny nodes; on success the scene holds a
+// reference to both sine lengths.
+int shared_tables(EmbDevice *dev, const EmbSettings &s, int nx, int ny, DevScene &d, DeviceTables &tab) {
+    std::lock_guard<std::mutex> guard(dev->lock);
+    int rc = close_tables_locked(dev, s, tab);
+    if (rc != EMB_OK) return rc;
+    const SineSet *sx = nullptr, *sy = nullptr;
+    rc = sine_locked(dev, nx, &sx);
+    if (rc != EMB_OK) return rc;
+    rc = sine_locked(dev, ny, &sy);
+    if (rc != EMB_OK) {
+        release_locked(dev, nx);
+        return rc;
+    }
+    d.sx_e = sx->e;
+    d.sx_o = sx->o;
+    d.sy_et = sy->e_t;
+    d.sy_o = sy->o;
+    return EMB_OK;
+}
+
 }  // namespace
 
 int device_sine(EmbDevice *dev, int n, const SineSet **out) {
     std::lock_guard<std::mutex> guard(dev->lock);
     return sine_locked(dev, n, out);
+}
+
+void release_sine(EmbDevice *dev, int n) {
+    std::lock_guard<std::mutex> guard(dev->lock);
+    release_locked(dev, n);
+    // Idle sets stay for the next scene of the same length, up to a cap.
+    size_t idle = 0;
+    for (auto &kv : dev->sine)
+        if (kv.second.refs == 0) idle += kv.second.bytes;
+    if (idle > SINE_IDLE_BYTES) trim_locked(dev, dev->sine_bytes - (idle - SINE_IDLE_BYTES));
+}
+
+size_t sine_set_bytes(int n) {
+    const size_t h0 = (size_t)(n + 1) / 2, h1 = (size_t)n / 2;
+    return ((size_t)n * n + 2 * h0 * h0 + h1 * h1) * sizeof(float);
+}
+
+size_t sine_missing_bytes(EmbDevice *dev, int nx, int ny) {
+    std::lock_guard<std::mutex> guard(dev->lock);
+    size_t b = 0;
+    if (!dev->sine.count(nx)) b += sine_set_bytes(nx);
+    if (ny != nx && !dev->sine.count(ny)) b += sine_set_bytes(ny);
+    return b;
+}
+
+size_t sine_held_bytes(EmbDevice *dev) {
+    std::lock_guard<std::mutex> guard(dev->lock);
+    return dev->sine_bytes;
+}
+
+void trim_sine(EmbDevice *dev, size_t keep) {
+    std::lock_guard<std::mutex> guard(dev->lock);
+    trim_locked(dev, keep);
 }
 
 size_t scene_bytes(const EmbSettings &s, int cols, int rows, int64_t n, int n_stacks, int n_stack_cells, int n_bats,
@@ -295,6 +367,8 @@ int make_resident(EmbDevice *dev, const EmbSettings &s, HostScene &&hs_in, hipSt
     DeviceTables tables;
     int rc = shared_tables(dev, s, hs.nx, hs.ny, d, tables);
     if (rc != EMB_OK) return rc;
+    r.sine_nx = hs.nx;
+    r.sine_ny = hs.ny;
     d.close_on = s.electron_cutoff > 0.0 ? 1 : 0;
     d.reach = tables.reach;
     d.square_n = tables.square_n;
@@ -384,6 +458,11 @@ int make_resident(EmbDevice *dev, const EmbSettings &s, HostScene &&hs_in, hipSt
 }
 
 void free_resident(Resident &r) {
+    if (r.sine_nx > 0) {
+        release_sine(r.dev, r.sine_nx);
+        release_sine(r.dev, r.sine_ny);
+        r.sine_nx = r.sine_ny = 0;
+    }
     if (r.tab_event) (void)hipEventSynchronize(r.tab_event), (void)hipEventDestroy(r.tab_event);
     if (r.pin_tab) (void)hipHostFree(r.pin_tab);
     if (r.arena) (void)hipFree(r.arena);
@@ -452,7 +531,41 @@ struct EmbSim {
     float2 *force = nullptr;   // emb_sim_close_force output, grown to the electron count on use
     size_t force_cap = 0;      // electrons `force` holds (an upload may raise the count)
     float *probe = nullptr;    // emb_sim_probe output: 32 words + 3 x 256 counts, allocated on first use
+    // One tick of this scene captured as a HIP graph on first step (kernels only:
+    // the per-tick tables are copied before it), dropped when an upload replaces
+    // the resident scene.
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
 };
+
+namespace {
+
+void drop_tick_graph(EmbSim *sim) {
+    if (sim->graph_exec) (void)hipGraphExecDestroy(sim->graph_exec);
+    if (sim->graph) (void)hipGraphDestroy(sim->graph);
+    sim->graph_exec = nullptr;
+    sim->graph = nullptr;
+}
+
+int capture_tick_graph(EmbSim *sim) {
+    EMB_HIP(hipStreamBeginCapture(sim->stream, hipStreamCaptureModeThreadLocal));
+    const hipError_t e = tick_kernels(sim->r.batch, sim->r.ds.fdtd_substeps, sim->stream);
+    hipGraph_t g = nullptr;
+    const hipError_t e2 = hipStreamEndCapture(sim->stream, &g);
+    if (e != hipSuccess || e2 != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return fail(EMB_E_DEVICE, std::string("tick graph capture: ") + hipGetErrorString(e != hipSuccess ? e : e2));
+    }
+    sim->graph = g;
+    const hipError_t e3 = hipGraphInstantiate(&sim->graph_exec, g, nullptr, nullptr, 0);
+    if (e3 != hipSuccess) {
+        drop_tick_graph(sim);
+        return fail(EMB_E_DEVICE, std::string("tick graph instantiate: ") + hipGetErrorString(e3));
+    }
+    return EMB_OK;
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -517,6 +630,7 @@ int emb_device_close(EmbDevice *dev) {
         if (kv.second.square) (void)hipFree(kv.second.square);
         if (kv.second.ion_sum) (void)hipFree(kv.second.ion_sum);
         if (kv.second.g_lerp) (void)hipFree(kv.second.g_lerp);
+        if (kv.second.square_f2) (void)hipFree(kv.second.square_f2);
     }
     delete dev;
     return EMB_OK;
@@ -596,6 +710,7 @@ int emb_sim_upload(EmbSim *sim, const EmbScene *scene, int reset_display) {
         EMB_HIP(setup_kernels(fresh.batch, sim->stream));
         if (reset_display) EMB_HIP(finish_sync_kernel(fresh.batch, sim->stream));
         EMB_HIP(hipStreamSynchronize(sim->stream));
+        drop_tick_graph(sim);   // it holds the old scene's descriptors
         free_resident(sim->r);
         sim->r = std::move(fresh);
         return EMB_OK;
@@ -606,11 +721,24 @@ int emb_sim_step(EmbSim *sim, int ticks) {
     EMB_GUARD({
         if (!sim || ticks < 0) return fail(EMB_E_INVALID, "bad argument");
         EMB_HIP(hipSetDevice(sim->dev->ordinal));
+        // EMBLOKS_TICK_GRAPH=0 launches the same kernels one by one (for tools
+        // that trace launches but not graph replays).
+        static const bool graphs = [] {
+            const char *v = std::getenv("EMBLOKS_TICK_GRAPH");
+            return !(v && v[0] == '0');
+        }();
+        if (graphs && ticks > 0 && !sim->graph_exec) {
+            const int rc = capture_tick_graph(sim);
+            if (rc != EMB_OK) return rc;
+        }
         while (ticks > 0) {
             const int chunk = std::min(ticks, TICK_TABLE);
             int rc = enqueue_tick_tables(sim->r, chunk, sim->stream);
             if (rc != EMB_OK) return rc;
-            for (int t = 0; t < chunk; t++) EMB_HIP(tick_kernels(sim->r.batch, sim->r.ds.fdtd_substeps, sim->stream));
+            for (int t = 0; t < chunk; t++) {
+                if (graphs) EMB_HIP(hipGraphLaunch(sim->graph_exec, sim->stream));
+                else EMB_HIP(tick_kernels(sim->r.batch, sim->r.ds.fdtd_substeps, sim->stream));
+            }
             ticks -= chunk;
         }
         return EMB_OK;
@@ -987,6 +1115,7 @@ int emb_sim_destroy(EmbSim *sim) {
     if (!sim) return EMB_OK;
     (void)hipSetDevice(sim->dev->ordinal);
     (void)hipStreamSynchronize(sim->stream);
+    drop_tick_graph(sim);
     free_resident(sim->r);
     if (sim->diag) (void)hipFree(sim->diag);
     if (sim->probe) (void)hipFree(sim->probe);

@@ -52,7 +52,15 @@ namespace emb {
 // (sine_halves): the tick's products use the halves.
 struct SineSet {
     float *full = nullptr, *e = nullptr, *e_t = nullptr, *o = nullptr;
+    size_t bytes = 0;       // device bytes of the four matrices
+    int refs = 0;           // resident scenes and Poisson solvers using this length
+    uint64_t last_use = 0;  // EmbDevice::sine_clock when last taken or released
 };
+// Bytes of the sine set of length n (sine_matrix plus sine_halves).
+size_t sine_set_bytes(int n);
+// Unreferenced sine sets are kept for reuse up to this many bytes per device
+// (least recently used ones freed first).
+constexpr size_t SINE_IDLE_BYTES = (size_t)512 << 20;
 }  // namespace emb
 
 struct EmbDevice {
@@ -61,6 +69,8 @@ struct EmbDevice {
     double *scratch = nullptr;   // this device's copy of runtime.hip's `scratch` array (hipGetSymbolAddress)
     std::mutex lock;
     std::map<int, emb::SineSet> sine;   // DST-I matrices per length, shared by every scene
+    size_t sine_bytes = 0;              // device bytes of all sets in `sine`
+    uint64_t sine_clock = 0;
     std::map<std::tuple<double, int, double>, emb::DeviceTables> tables;
 };
 
@@ -84,14 +94,24 @@ struct Resident {
     uint64_t tick = 0;            // ticks enqueued
     int last_substeps = 0;
     long long *dep_dump = nullptr;
+    int sine_nx = 0, sine_ny = 0; // sine lengths this scene holds references to (0: none)
 };
 
 // The close-range tables of (softening, passes, cutoff), built on `dev` in FP64
 // and cached for the process (tables.hip); a no-op when cutoff <= 0 or built.
 int ensure_tables(EmbDevice *dev, double softening, int passes, double cutoff);
 
-// The FP32 DST-I matrix of length n on the device (built once per device).
+// The FP32 DST-I matrix of length n on the device (built once per device, then
+// shared), with a reference taken: release it with release_sine.
 int device_sine(EmbDevice *dev, int n, const SineSet **out);
+void release_sine(EmbDevice *dev, int n);
+// Device bytes that admitting a scene of nx x ny nodes adds in sine sets not yet
+// held, and the bytes held now (referenced or idle).
+size_t sine_missing_bytes(EmbDevice *dev, int nx, int ny);
+size_t sine_held_bytes(EmbDevice *dev);
+// Free idle (unreferenced) sine sets, least recently used first, until at most
+// `keep` bytes of sine sets remain or none is idle.
+void trim_sine(EmbDevice *dev, size_t keep);
 
 // Bytes a scene of this shape needs (arena only; shared tables excluded).
 size_t scene_bytes(const EmbSettings &s, int cols, int rows, int64_t n, int n_stacks, int n_stack_cells, int n_bats, int n_ac);

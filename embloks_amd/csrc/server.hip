@@ -37,6 +37,7 @@ struct Request {
     int64_t steps = 0, done = 0;
     size_t footprint = 0;
     double time0 = 0.0;
+    double t_submit = 0.0, t_admit = 0.0, t_done = 0.0;   // host clock (s): request latency and queue wait
     ReqState state = QUEUED;
     Resident r;
     std::vector<double> pos;                   // kept results
@@ -78,7 +79,9 @@ struct EmbServer {
     std::vector<DevScene> host_scenes;         // sources of the scene-table copies
     std::vector<int32_t> host_prefix;
     // stats
-    std::vector<float> latencies_ms;   // one sample per scene-step
+    std::vector<float> latencies_ms;   // one sample per scene-step: the batch tick it took part in
+    std::vector<float> request_ms;     // per completed request: submit -> done
+    std::vector<float> wait_ms;        // per admitted request: submit -> admitted
     int64_t cell_steps = 0, completed = 0;
     double t_first = -1.0, t_last = 0.0;
     int peak = 0;
@@ -213,11 +216,14 @@ int emb_server_submit(EmbServer *sv, const EmbScene *scene, int64_t steps, int64
         if (!why.empty()) return fail(EMB_E_INVALID, why);
         q->footprint = scene_bytes(sv->s, q->hs.cols, q->hs.rows, q->hs.n, (int)q->hs.stacks.size(),
                                    (int)q->hs.stack_cells.size(), (int)q->hs.bats.size(), (int)q->hs.ac_freq.size());
-        if ((int64_t)q->footprint > sv->cfg.hbm_budget_bytes)
+        // Its arena and its DST-I matrices must fit the budget on their own.
+        const size_t sines = sine_set_bytes(q->hs.nx) + (q->hs.ny != q->hs.nx ? sine_set_bytes(q->hs.ny) : 0);
+        if ((int64_t)(q->footprint + sines) > sv->cfg.hbm_budget_bytes)
             return fail(EMB_E_NOMEM, "scene footprint exceeds the HBM budget");
         q->steps = steps;
         q->time0 = scene->time;
         q->id = sv->next_id++;
+        q->t_submit = now_s();
         *id = q->id;
         sv->queue.push_back(q.get());
         sv->all[q->id] = std::move(q);
@@ -238,7 +244,14 @@ int admit(EmbServer *sv) {
     bool any = false;
     while (!sv->queue.empty() && (int)sv->active.size() < sv->cfg.max_slots) {
         Request *q = sv->queue.front();
-        if ((int64_t)(sv->used_bytes + q->footprint) > sv->cfg.hbm_budget_bytes) break;
+        // The budget covers the resident arenas and every sine set on the device;
+        // idle sets (no resident scene) are freed to make room.
+        const int64_t budget = sv->cfg.hbm_budget_bytes;
+        const size_t need = q->footprint + sine_missing_bytes(sv->dev, q->hs.nx, q->hs.ny);
+        if ((int64_t)(sv->used_bytes + sine_held_bytes(sv->dev) + need) > budget &&
+            (int64_t)(sv->used_bytes + need) <= budget)
+            trim_sine(sv->dev, (size_t)(budget - (int64_t)(sv->used_bytes + need)));
+        if ((int64_t)(sv->used_bytes + sine_held_bytes(sv->dev) + need) > budget) break;
         sv->queue.pop_front();
         int rc = make_resident(sv->dev, sv->s, std::move(q->hs), sv->up_stream, q->r);
         if (rc != EMB_OK) {
@@ -251,11 +264,15 @@ int admit(EmbServer *sv) {
         EMB_HIP(finish_sync_kernel(q->r.batch, sv->up_stream));
         sv->used_bytes += q->footprint;
         q->state = RUNNING;
+        q->t_admit = now_s();
+        sv->wait_ms.push_back((float)((q->t_admit - q->t_submit) * 1e3));
         any = true;
         if (q->steps == 0) {   // nothing to run: finished on admission
             EMB_HIP(hipStreamSynchronize(sv->up_stream));
             int rc2 = retire(sv, q);
             q->state = DONE;
+            q->t_done = now_s();
+            sv->request_ms.push_back((float)((q->t_done - q->t_submit) * 1e3));
             sv->completed++;
             if (rc2 != EMB_OK) return rc2;
             continue;
@@ -286,8 +303,11 @@ int finish_round(EmbServer *sv) {
     }
     sv->inflight.clear();
     std::vector<Request *> keep;
+    const double t_round = now_s();   // the round is complete on the device by now
     for (Request *q : sv->active) {
         if (q->done >= q->steps) {
+            q->t_done = t_round;
+            sv->request_ms.push_back((float)((q->t_done - q->t_submit) * 1e3));
             int rc = retire(sv, q);
             q->state = DONE;
             sv->completed++;
@@ -423,16 +443,19 @@ int emb_server_stats(EmbServer *sv, EmbServerStats *out) {
     out->elapsed_s = sv->t_first < 0.0 ? 0.0 : std::max(0.0, sv->t_last - sv->t_first);
     out->cell_steps_per_s = out->elapsed_s > 0.0 ? (double)sv->cell_steps / out->elapsed_s : 0.0;
     out->step_samples = (int64_t)sv->latencies_ms.size();
-    if (!sv->latencies_ms.empty()) {
-        std::vector<float> v = sv->latencies_ms;
-        auto pct = [&](double p) {
-            const size_t k = std::min(v.size() - 1, (size_t)std::floor(p / 100.0 * (double)(v.size() - 1) + 0.5));
-            std::nth_element(v.begin(), v.begin() + (long)k, v.end());
-            return (double)v[k];
-        };
-        out->step_latency_p50_ms = pct(50.0);
-        out->step_latency_p99_ms = pct(99.0);
-    }
+    // Nearest-rank percentile of a sample (0 when empty).
+    auto pct = [](std::vector<float> v, double p) {
+        if (v.empty()) return 0.0;
+        const size_t k = std::min(v.size() - 1, (size_t)std::floor(p / 100.0 * (double)(v.size() - 1) + 0.5));
+        std::nth_element(v.begin(), v.begin() + (long)k, v.end());
+        return (double)v[k];
+    };
+    out->step_latency_p50_ms = pct(sv->latencies_ms, 50.0);
+    out->step_latency_p99_ms = pct(sv->latencies_ms, 99.0);
+    out->request_latency_p50_ms = pct(sv->request_ms, 50.0);
+    out->request_latency_p99_ms = pct(sv->request_ms, 99.0);
+    out->queue_wait_p50_ms = pct(sv->wait_ms, 50.0);
+    out->queue_wait_p99_ms = pct(sv->wait_ms, 99.0);
     out->peak_resident = sv->peak;
     return EMB_OK;
 }

@@ -230,11 +230,15 @@ typedef struct EmbServerStats {
     int64_t cell_steps;         /* nx * ny * ticks summed over completed work */
     double elapsed_s;
     double cell_steps_per_s;
-    double step_latency_p50_ms;
+    double step_latency_p50_ms;   /* one tick of a resident scene: the batch tick it took part in */
     double step_latency_p99_ms;
     int64_t step_samples;
     int32_t peak_resident;
     int32_t reserved0;
+    double request_latency_p50_ms;   /* per completed request: submit -> finished (host clock) */
+    double request_latency_p99_ms;
+    double queue_wait_p50_ms;        /* per admitted request: submit -> admitted */
+    double queue_wait_p99_ms;
 } EmbServerStats;
 
 typedef struct EmbServer EmbServer;

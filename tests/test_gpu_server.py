@@ -88,3 +88,80 @@ def test_cancel_a_running_request():
     with pytest.raises(EmbError):
         srv.done(ids[1])
     srv.destroy()
+
+
+def drain(srv, limit=200000):
+    for _ in range(limit):
+        if not srv.poll():
+            return True
+    return False
+
+
+@gpu
+def test_admission_evicts_idle_sine_matrices_to_fit_the_budget():
+    # The budget holds the largest scene with its two DST-I sets and little
+    # more: every later size needs the idle sets of the earlier ones freed
+    # (least recently used first) to be admitted.
+    from embloks_amd.gpusim import footprint
+
+    scenes, steps = mix()
+    order = [0, 2, 1, 3, 4]   # sizes 48, 40 x 32, 96, 40 x 32, 160
+    s = abi.settings()
+
+    def sine(n):
+        return (n * n + 2 * ((n + 1) // 2) ** 2 + (n // 2) ** 2) * 4
+
+    need = [footprint(s, w.cols, w.rows, w.n_electrons) + sine(w.cols + 32) + sine(w.rows + 32) for w in scenes]
+    srv = Server(s, hbm_budget_bytes=max(need) + (1 << 20), max_slots=4, ticks_per_round=4, keep_results=True)
+    ids = [srv.submit(scenes[k], steps[k]) for k in order]
+    assert drain(srv), "a request never fit the budget"
+    assert all(srv.done(i) for i in ids)
+    st = srv.stats()
+    assert st["completed"] == len(ids)
+    assert st["queue_wait_p99_ms"] >= st["queue_wait_p50_ms"] >= 0.0
+    assert st["request_latency_p99_ms"] >= st["request_latency_p50_ms"] > 0.0
+    # Each still equals its standalone run.
+    for k, i in zip(order, ids):
+        sim = GpuSim(scenes[k])
+        sim.run_ticks(steps[k])
+        pos, vel, _ = sim.read_state()
+        got = srv.result(i)
+        for x, y in zip((pos, vel) + sim.read_yee(), got):
+            np.testing.assert_array_equal(x, y)
+        sim.destroy()
+    srv.destroy()
+
+
+@gpu
+def test_serving_stream_of_256_requests_each_equal_to_its_standalone_run():
+    # The serving workload: 256 requests, sides log-uniform 64-1024, conductor
+    # density of the 512^2 circuit, one carrier per cell, Poisson arrivals
+    # (all submitted at once here), steps cut to 1% (10-200 per request).
+    stream = W.serving_stream(n_requests=256, seed=5, min_steps=10, max_steps=200)
+    s = abi.settings(carriers_per_cell=1)
+    srv = Server(s, hbm_budget_bytes=64 << 30, max_slots=256, batched=True, ticks_per_round=16, use_graph=True,
+                 keep_results=True)
+    ids = [srv.submit(w, n) for _, n, w in stream]
+    print("submitted 256 requests", flush=True)
+    assert drain(srv)
+    st = srv.stats()
+    print("stream of 256:", {k: st[k] for k in ("completed", "cell_steps_per_s", "step_latency_p50_ms",
+                                                 "step_latency_p99_ms", "request_latency_p50_ms",
+                                                 "request_latency_p99_ms", "queue_wait_p99_ms", "peak_resident")})
+    assert st["completed"] == 256
+    assert st["cell_steps"] == sum((w.cols + 32) * (w.rows + 32) * n for _, n, w in stream)
+    sides = np.array([w.cols for _, _, w in stream])
+    pick = {int(np.argmin(sides)), int(np.argmax(sides))}
+    pick |= set(int(k) for k in np.random.default_rng(1).choice(256, 6, replace=False))
+    for k in sorted(pick):
+        _, n, w = stream[k]
+        sim = GpuSim(w)
+        sim.run_ticks(n)
+        pos, vel, _ = sim.read_state()
+        got = srv.result(ids[k])
+        for x, y in zip((pos, vel) + sim.read_yee(), got):
+            np.testing.assert_array_equal(x, y)
+        print("request", k, "side", w.cols, "steps", n, "electrons", w.n_electrons, "equal to its standalone run",
+              flush=True)
+        sim.destroy()
+    srv.destroy()
