@@ -80,6 +80,12 @@ __host__ __device__ inline int rest_sync_len(int max_substeps) {
     return 1 + REST_PHASES * (max_substeps > 1 ? max_substeps - 1 : 0);
 }
 constexpr int ION_SUM = 33;       // samples per axis of the summed ion table
+// Row pitch of the Poisson arrays of a scene with ny nodes per row: a multiple of
+// four floats, so every row starts 16-byte aligned (the padding stays zero).
+__host__ __device__ inline int poisson_pitch(int ny) { return (ny + 3) & ~3; }
+// Side of a stored half of the DST-I matrix (sine_halves) of m rows: zero-padded
+// to a multiple of 16, the products' K slice.
+__host__ __device__ inline int sine_half_pitch(int m) { return (m + 15) & ~15; }
 constexpr int PAIR_SAMPLES = 4096;   // samples of the radial pair table (close_range.rs:30)
 
 // Cell flags (CellInfo.flags).
@@ -132,7 +138,8 @@ struct DevScene {
     int32_t reach;            // ceil(cutoff)
     int32_t square_n;
     int32_t close_on;
-    int32_t pad0[2];
+    int32_t pny;              // row pitch of the Poisson arrays (w1, w2, phi, inv_lap): ny rounded up to 4
+    int32_t pad0;
     // --- constants (FP32)
     float dt, max_step, restitution;
     float qm;                 // q / m

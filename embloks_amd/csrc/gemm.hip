@@ -80,7 +80,10 @@ __device__ __forceinline__ float4 load4(const float *p, int avail, bool vec) {
 // of B past N are clamped to the last valid ones, since they only reach output
 // rows and columns that are never stored.
 
-template <int BM, int BN, int GM = 2, int GN = 2, int GK = 16, int PF = 1, bool FAST = false, int FOLD = 0>
+// MVEC (FAST + FOLD 2): the mirror columns of A are loaded as a float4 (fold_n
+// a multiple of 4, so the mirror is aligned) or as four floats.
+template <int BM, int BN, int GM = 2, int GN = 2, int GK = 16, int PF = 1, bool FAST = false, int FOLD = 0,
+          bool MVEC = true>
 __device__ __forceinline__ void tile_f32(const GemmArgs &g, int m0, int n0, float *lds) {
     constexpr int NT = 64 * GM * GN;
     constexpr int LA = BM + 4, LB = BN + 4, KV = GK / 4;
@@ -121,7 +124,7 @@ __device__ __forceinline__ void tile_f32(const GemmArgs &g, int m0, int n0, floa
         const int idx = tid + NT * v;
         rb_k[v] = idx / (BN / 4);
         rb_col[v] = (idx % (BN / 4)) * 4;
-        const int gc = FAST ? min(n0 + rb_col[v], g.N - 4) : n0 + rb_col[v];
+        const int gc = FAST ? min(n0 + rb_col[v], (g.Np ? g.Np : g.N) - 4) : n0 + rb_col[v];
         pb[v] = g.B + (size_t)rb_k[v] * g.ldb + gc;
         if (FAST && FOLD == 1) pb2[v] = g.B + (size_t)(fn - 1 - rb_k[v]) * g.ldb + gc;   // moves up as k grows
     }
@@ -139,7 +142,14 @@ __device__ __forceinline__ void tile_f32(const GemmArgs &g, int m0, int n0, floa
         for (int v = 0; v < AV; v++) {
             if (FAST) {
                 ra[slot][v] = *reinterpret_cast<const floatx4 *>(pa[v] + k0);
-                if (FOLD == 2) ra2[slot][v] = *reinterpret_cast<const floatx4 *>(pa2[v] - k0);
+                if (FOLD == 2) {
+                    if (MVEC) {
+                        ra2[slot][v] = *reinterpret_cast<const floatx4 *>(pa2[v] - k0);
+                    } else {
+                        const float *m = pa2[v] - k0;
+                        ra2[slot][v] = floatx4{m[0], m[1], m[2], m[3]};
+                    }
+                }
             } else {
                 const int gr = m0 + ra_row[v], gk = k0 + ra_k[v];
                 if (FOLD == 2) {
@@ -179,7 +189,15 @@ __device__ __forceinline__ void tile_f32(const GemmArgs &g, int m0, int n0, floa
             floatx4 x = ra[slot][v];
             if (FAST && FOLD == 2) {   // the mirror float4 runs backwards
                 const floatx4 y = ra2[slot][v];
-                x = x + sgn * floatx4{y[3], y[2], y[1], y[0]};
+                if (MVEC) {   // fold_n even: no column is its own mirror
+                    x = x + sgn * floatx4{y[3], y[2], y[1], y[0]};
+                } else {      // the middle column of an odd fold_n is taken once
+#pragma unroll
+                    for (int e = 0; e < 4; e++) {
+                        const int c = k0 + kk + e;
+                        x[e] += (fn - 1 - c == c) ? 0.0f : sgn * y[3 - e];
+                    }
+                }
             }
             a[(kk + 0) * LA + row] = x[0];
             a[(kk + 1) * LA + row] = x[1];
@@ -265,13 +283,17 @@ __device__ __forceinline__ void tile_f32(const GemmArgs &g, int m0, int n0, floa
             }
 }
 
-// Whether a product can take tile_f32's FAST path for slices of GK.
-// A column fold (2) also needs the mirror float4s aligned: fold_n a multiple of 4.
-__host__ __device__ inline bool gemm_fast(const GemmArgs &g, int gk) {
-    return g.K % gk == 0 && g.N % 4 == 0 && g.N >= 4 && g.lda % 4 == 0 && g.ldb % 4 == 0 &&
-           (reinterpret_cast<uintptr_t>(g.A) & 15) == 0 && (reinterpret_cast<uintptr_t>(g.B) & 15) == 0 &&
-           (g.fold != 2 || g.fold_n % 4 == 0);
+// Whether a product can take tile_f32's FAST path for slices of GK: 0 no,
+// 1 yes, 2 yes with the mirror columns of a column fold loaded as single
+// floats (fold_n not a multiple of 4, so the mirror float4 is not aligned).
+__host__ __device__ inline int gemm_fast(const GemmArgs &g, int gk) {
+    const int np = g.Np ? g.Np : g.N;
+    const bool fast = g.K % gk == 0 && np % 4 == 0 && np >= 4 && g.lda % 4 == 0 && g.ldb % 4 == 0 &&
+                      (reinterpret_cast<uintptr_t>(g.A) & 15) == 0 && (reinterpret_cast<uintptr_t>(g.B) & 15) == 0;
+    if (!fast) return 0;
+    return (g.fold != 2 || g.fold_n % 4 == 0) ? 1 : 2;
 }
+
 
 // ---------------------------------------------------------------------------------------
 // The tick's solve over a batch of scenes: blocks walk (scene, channel, half,
@@ -315,41 +337,48 @@ __global__ __launch_bounds__(256) void k_dst_gemm(Batch b) {
         const int rem0 = local - ch * (te + to);
         const int odd = rem0 >= te ? 1 : 0;
         const int rem = rem0 - odd * te;
-        const size_t nodes = (size_t)nx * ny;
-        const float *in = (LEFT ? S.w1 : S.w2) + ch * nodes;
-        float *out = (STAGE == 3 ? S.phi : (STAGE == 1 ? S.w1 : S.w2)) + ch * nodes;
+        // The Poisson arrays' rows are padded to pny (a multiple of 4) and the
+        // sine halves to whole slices, so every scene takes the FAST tiles.
+        const int pny = S.pny;
+        const size_t pnodes = (size_t)nx * pny;
+        const float *in = (LEFT ? S.w1 : S.w2) + ch * pnodes;
+        float *out = (STAGE == 3 ? S.phi : (STAGE == 1 ? S.w1 : S.w2)) + ch * pnodes;
         GemmArgs g;
         g.C = out;
-        g.ldc = ny;
+        g.ldc = pny;
         g.scale = STAGE == 1 ? S.inv_lap : nullptr;
         g.odd = odd;
         if (LEFT) {   // out = Sx . in: rows 2 i + odd from the half of Sx and the folded rows of in
             const int mh = odd ? nx / 2 : (nx + 1) / 2;
             g.A = odd ? S.sx_o : S.sx_e;
-            g.lda = mh;
+            g.lda = sine_half_pitch(mh);
             g.B = in;
-            g.ldb = ny;
+            g.ldb = pny;
             g.M = mh;
             g.N = ny;
-            g.K = mh;
+            g.Np = pny;
+            g.K = fold_k(mh, nx);
             g.fold = 1;
             g.fold_n = nx;
         } else {      // out = in . Sy: columns 2 j + odd from the folded columns of in
             const int mh = odd ? ny / 2 : (ny + 1) / 2;
             g.A = in;
-            g.lda = ny;
+            g.lda = pny;
             g.B = odd ? S.sy_o : S.sy_et;
-            g.ldb = mh;
+            g.ldb = sine_half_pitch(mh);
             g.M = nx;
             g.N = mh;
-            g.K = mh;
+            g.Np = sine_half_pitch(mh);
+            g.K = fold_k(mh, ny);
             g.fold = 2;
             g.fold_n = ny;
         }
         const int tn = (g.N + TICK_BN - 1) / TICK_BN;
         const int m0 = (rem / tn) * TICK_BM, n0 = (rem % tn) * TICK_BN;
         constexpr int F = LEFT ? 1 : 2;
-        if (gemm_fast(g, 16)) tile_f32<TICK_BM, TICK_BN, 2, 2, 16, 4, true, F>(g, m0, n0, lds);
+        const int fast = gemm_fast(g, 16);
+        if (fast == 1) tile_f32<TICK_BM, TICK_BN, 2, 2, 16, 4, true, F>(g, m0, n0, lds);
+        else if (F == 2 && fast == 2) tile_f32<TICK_BM, TICK_BN, 2, 2, 16, 4, true, F, false>(g, m0, n0, lds);
         else tile_f32<TICK_BM, TICK_BN, 2, 2, 16, 4, false, F>(g, m0, n0, lds);
         __syncthreads();
     }
@@ -375,7 +404,7 @@ hipError_t launch_dst_gemms(const Batch &b, hipStream_t stream) {
 // ---------------------------------------------------------------------------------------
 // One product on plain pointers (the isolated solve).
 
-template <int BM, int BN, int PF, bool FAST, int FOLD>
+template <int BM, int BN, int PF, bool FAST, int FOLD, bool MVEC = true>
 __global__ __launch_bounds__(256) void k_gemm_f32(GemmArgs g0, GemmArgs g1) {
     __shared__ __align__(16) float lds[2 * 16 * (BM + 4 + BN + 4)];
     const int tn0 = (g0.N + BN - 1) / BN, tiles0 = ((g0.M + BM - 1) / BM) * tn0;
@@ -384,7 +413,7 @@ __global__ __launch_bounds__(256) void k_gemm_f32(GemmArgs g0, GemmArgs g1) {
         const bool second = blk >= tiles0;
         const GemmArgs &g = second ? g1 : g0;
         const int t = second ? blk - tiles0 : blk, tn = second ? tn1 : tn0;
-        tile_f32<BM, BN, 2, 2, 16, PF, FAST, FOLD>(g, (t / tn) * BM, (t % tn) * BN, lds);
+        tile_f32<BM, BN, 2, 2, 16, PF, FAST, FOLD, MVEC>(g, (t / tn) * BM, (t % tn) * BN, lds);
         __syncthreads();
     }
 }
@@ -398,9 +427,10 @@ static hipError_t launch_gemm_tiles(const GemmArgs &g0, const GemmArgs &g1, hipS
     const long long total = tiles(g0) + tiles(g1);
     const int grid = (int)(total < 8192 ? total : 8192);
     if (grid == 0) return hipSuccess;
-    const bool fast = gemm_fast(g0, 16) && (g1.M == 0 || gemm_fast(g1, 16));
-    if (fast) k_gemm_f32<BM, BN, 4, true, FOLD><<<grid, 256, 0, stream>>>(g0, g1);
-    else k_gemm_f32<BM, BN, 4, false, FOLD><<<grid, 256, 0, stream>>>(g0, g1);
+    const int f0 = gemm_fast(g0, 16), f1 = g1.M == 0 ? 1 : gemm_fast(g1, 16);
+    if (f0 == 0 || f1 == 0) k_gemm_f32<BM, BN, 4, false, FOLD><<<grid, 256, 0, stream>>>(g0, g1);
+    else if (FOLD == 2 && (f0 == 2 || f1 == 2)) k_gemm_f32<BM, BN, 4, true, FOLD, false><<<grid, 256, 0, stream>>>(g0, g1);
+    else k_gemm_f32<BM, BN, 4, true, FOLD><<<grid, 256, 0, stream>>>(g0, g1);
     return hipGetLastError();
 }
 

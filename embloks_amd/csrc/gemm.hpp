@@ -17,13 +17,27 @@ namespace emb {
 // 2 i + odd of C; with fold = 2, column c of A is read as A[.][c] + s
 // A[.][fold_n - 1 - c] and result column j goes to column 2 j + odd. s = +1 for
 // the even half (odd = 0), -1 for the odd half; `scale` is read at C's position.
+//
+// Np (when set) is how many columns of B the loads may touch, a multiple of 4
+// >= N: B's columns past N up to Np are padding (their results are not
+// stored). K may include zero padding of the sine half (its rows or columns
+// past the half's size are zero), so that K is a whole number of slices.
 struct GemmArgs {
     const float *A = nullptr, *B = nullptr;
     float *C = nullptr;
     const float *scale = nullptr;
     int M = 0, N = 0, K = 0, lda = 0, ldb = 0, ldc = 0;
     int fold = 0, fold_n = 0, odd = 0;
+    int Np = 0;
 };
+
+// K of a folded half product over a half of side mh: padded to whole slices
+// when the padding stays inside the folded operand (it multiplies the sine
+// half's zero padding).
+__host__ __device__ inline int fold_k(int mh, int fold_n) {
+    const int kp = sine_half_pitch(mh);
+    return kp <= fold_n ? kp : mh;
+}
 
 // One FP32 product with 64 x 64 (tile 64) or 128 x 64 (tile 128) output tiles.
 hipError_t launch_gemm(const GemmArgs &g, int tile, hipStream_t stream);

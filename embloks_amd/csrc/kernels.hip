@@ -330,7 +330,7 @@ __global__ __launch_bounds__(TRIM_THREADS) void k_trim(Batch b) {
             float v = 0.0f;
             if (sub == 0) {
                 if (part == 0) {
-                    v = S.phi[(size_t)(c + S.margin) * S.ny + (r + S.margin)];
+                    v = S.phi[(size_t)(c + S.margin) * S.pny + (r + S.margin)];
                     if (S.close_on) v += S.ion_pot[cell];
                 }
             } else {
@@ -1700,7 +1700,7 @@ __global__ __launch_bounds__(64 * FDTD_WAVES) void k_fdtd(Batch b, int f) {
         const int s = scene_of(b, KC_FDTD, blk);
         const DevScene &S = b.scenes[s];
         if (f >= S.fdtd_substeps) continue;
-        const int nx = S.nx, ny = S.ny;
+        const int nx = S.nx, ny = S.ny, pny = S.pny;
         const int local = block_in_scene(b, KC_FDTD, s, blk);
         const int strips = (ny + 1 + FDTD_STRIP - 1) / FDTD_STRIP;
         const int sblocks = (strips + FDTD_WAVES - 1) / FDTD_WAVES;
@@ -1798,10 +1798,10 @@ __global__ __launch_bounds__(64 * FDTD_WAVES) void k_fdtd(Batch b, int f) {
                     const float div = (lane_prev(exd) - lane_prev(exd_prev)) + (eyd_prev - lane_prev(eyd_prev));
                     const int ni = t - 2;
                     if (own_col && tp >= row_lo && tp < row_hi && ni >= 0 && ni < nx && nj >= 0 && nj < ny) {
-                        const size_t node = (size_t)ni * ny + nj;
+                        const size_t node = (size_t)ni * pny + nj;   // the Poisson arrays' padded rows
                         const float src = -four_pi_k * v.rho;
                         w1[node] = src;
-                        w1[(size_t)nx * ny + node] = div + src;
+                        w1[(size_t)nx * pny + node] = div + src;
                     }
                 }
                 ey_prev = ey_f;
@@ -1829,8 +1829,9 @@ __global__ __launch_bounds__(256) void k_rhs(Batch b) {
         const float div = (S.ex[(size_t)(i + 1) * ny + j] - S.ex[(size_t)i * ny + j]) +
                           (S.ey[(size_t)i * (ny + 1) + j + 1] - S.ey[(size_t)i * (ny + 1) + j]);
         const float src = -S.four_pi_k * S.rho[q];
-        S.w1[q] = src;
-        S.w1[(size_t)nx * ny + q] = div + src;
+        const size_t node = (size_t)i * S.pny + j;   // the Poisson arrays' padded rows
+        S.w1[node] = src;
+        S.w1[(size_t)nx * S.pny + node] = div + src;
     }
 }
 
@@ -1850,14 +1851,14 @@ __global__ __launch_bounds__(256) void k_correct(Batch b, int tick) {
         const int local = block_in_scene(b, KC_NODES, s, blk);
         const int q = local * 256 + (int)threadIdx.x;
         const int nx = S.nx, ny = S.ny;
-        const size_t nodes = (size_t)nx * ny;
         if (tick && local == 0 && threadIdx.x == 0) S.ctr->tick += 1;
         // After an odd number of FDTD substeps the fields are in the second buffer.
         const bool alt = tick && (S.fdtd_substeps & 1);
         const float *ex = alt ? S.ex2 : S.ex, *ey = alt ? S.ey2 : S.ey;
         if (q >= (nx + 1) * (ny + 1)) continue;
         const int i = q / (ny + 1), j = q - (q / (ny + 1)) * (ny + 1);
-        auto p = [&](int a, int c) { return S.phi[nodes + (size_t)a * ny + c]; };
+        const size_t pnodes = (size_t)nx * S.pny;
+        auto p = [&](int a, int c) { return S.phi[pnodes + (size_t)a * S.pny + c]; };
         if (j < ny) {
             const float high = i < nx ? p(i, j) : 0.0f, low = i > 0 ? p(i - 1, j) : 0.0f;
             S.ex[(size_t)i * ny + j] = ex[(size_t)i * ny + j] - (high - low);
@@ -2111,7 +2112,7 @@ __global__ __launch_bounds__(256) void k_probe(const DevScene *scene, int cell, 
     out[0] = 1.0f;
     out[1] = (float)(S.cstart[cell + 1] - S.cstart[cell]);
     // potential_at (common.wgsl:104-125): node potential, ions, electrons nearby
-    float pot = S.phi[(size_t)(c + S.margin) * S.ny + (r + S.margin)];
+    float pot = S.phi[(size_t)(c + S.margin) * S.pny + (r + S.margin)];
     if (S.close_on) {
         pot += S.ion_pot[cell];
         const int R = S.reach;

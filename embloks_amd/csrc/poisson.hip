@@ -42,12 +42,13 @@ hipError_t left_product(const EmbPoisson &p, const float *in, float *out, int ti
         GemmArgs &g = h[odd];
         const int mh = odd ? p.nx / 2 : (p.nx + 1) / 2;
         g.A = odd ? p.sx_o : p.sx_e;
-        g.lda = mh;
+        g.lda = sine_half_pitch(mh);
         g.B = in;
         g.ldb = p.ny;
         g.C = out;
         g.ldc = p.ny;
-        g.M = g.K = mh;
+        g.M = mh;
+        g.K = fold_k(mh, p.nx);
         g.N = p.ny;
         g.fold = 1;
         g.fold_n = p.nx;
@@ -65,12 +66,14 @@ hipError_t right_product(const EmbPoisson &p, const float *in, float *out, const
         g.A = in;
         g.lda = p.ny;
         g.B = odd ? p.sy_o : p.sy_et;
-        g.ldb = mh;
+        g.ldb = sine_half_pitch(mh);
         g.C = out;
         g.ldc = p.ny;
         g.scale = scale;
         g.M = p.nx;
-        g.N = g.K = mh;
+        g.N = mh;
+        g.Np = sine_half_pitch(mh);
+        g.K = fold_k(mh, p.ny);
         g.fold = 2;
         g.fold_n = p.ny;
         g.odd = odd;

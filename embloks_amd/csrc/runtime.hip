@@ -67,6 +67,7 @@ struct Counts {
 void layout(Bump &a, const Counts &c, DevScene &d, DevScene **d_scene, int32_t **d_prefix, SceneCounters **ctr,
             float **tab) {
     const size_t nx = c.nx, ny = c.ny, nodes = nx * ny, cells = (size_t)c.cols * c.rows, n = (size_t)c.n;
+    const size_t pnodes = nx * (size_t)poisson_pitch(c.ny);   // the Poisson arrays' rows are padded
     d.ex = a.take<float>((nx + 1) * ny);
     d.ey = a.take<float>(nx * (ny + 1));
     d.bz = a.take<float>((nx + 1) * (ny + 1));
@@ -76,16 +77,16 @@ void layout(Bump &a, const Counts &c, DevScene &d, DevScene **d_scene, int32_t *
     d.rho = a.take<float>(nodes);
     d.jx = a.take<float>(nodes);
     d.jy = a.take<float>(nodes);
-    d.w1 = a.take<float>(2 * nodes);
-    d.w2 = a.take<float>(2 * nodes);
-    d.phi = a.take<float>(2 * nodes);
+    d.w1 = a.take<float>(2 * pnodes);
+    d.w2 = a.take<float>(2 * pnodes);
+    d.phi = a.take<float>(2 * pnodes);
     d.damp_ex_x = a.take<float>(nx + 1);
     d.damp_ex_y = a.take<float>(ny);
     d.damp_ey_x = a.take<float>(nx);
     d.damp_ey_y = a.take<float>(ny + 1);
     d.damp_bz_x = a.take<float>(nx + 1);
     d.damp_bz_y = a.take<float>(ny + 1);
-    d.inv_lap = a.take<float>(nodes);
+    d.inv_lap = a.take<float>(pnodes);
     d.cxy = a.take<int32_t>(n);
     d.fx = a.take<float>(n);
     d.fy = a.take<float>(n);
@@ -277,7 +278,7 @@ void release_sine(EmbDevice *dev, int n) {
 }
 
 size_t sine_set_bytes(int n) {
-    const size_t h0 = (size_t)(n + 1) / 2, h1 = (size_t)n / 2;
+    const size_t h0 = (size_t)sine_half_pitch((n + 1) / 2), h1 = (size_t)sine_half_pitch(n / 2);
     return ((size_t)n * n + 2 * h0 * h0 + h1 * h1) * sizeof(float);
 }
 
@@ -357,6 +358,7 @@ int make_resident(EmbDevice *dev, const EmbSettings &s, HostScene &&hs_in, hipSt
     d.rows = hs.rows;
     d.nx = hs.nx;
     d.ny = hs.ny;
+    d.pny = poisson_pitch(hs.ny);
     d.margin = hs.margin;
     d.n = (int32_t)hs.n;
     d.n_stacks = (int32_t)hs.stacks.size();
@@ -845,7 +847,12 @@ int emb_sim_read_potential(EmbSim *sim, double *potential) {
         if (!sim || !potential) return fail(EMB_E_INVALID, "null argument");
         EMB_HIP(hipStreamSynchronize(sim->stream));
         const DevScene &d = sim->r.ds;
-        return copy_back(sim, d.phi, (size_t)d.nx * d.ny, potential);
+        // Rows of phi are padded to d.pny floats.
+        std::vector<float> tmp((size_t)d.nx * d.pny);
+        EMB_HIP(hipMemcpy(tmp.data(), d.phi, tmp.size() * sizeof(float), hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < (size_t)d.nx; i++)
+            for (size_t j = 0; j < (size_t)d.ny; j++) potential[i * d.ny + j] = tmp[i * d.pny + j];
+        return EMB_OK;
     })
 }
 

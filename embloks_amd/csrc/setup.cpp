@@ -52,20 +52,22 @@ std::vector<float> sine_matrix(int n) {
 
 SineHalves sine_halves(int n) {
     const int m = (n + 1) / 2, h = n / 2;
+    const int pm = sine_half_pitch(m), ph = sine_half_pitch(h);
     auto S = [n](long long k, long long j) {
         return (float)std::sin(PI * (double)(((k + 1) * (j + 1)) % (2 * (n + 1))) / (double)(n + 1));
     };
+    // Each half square, zero-padded to a side of sine_half_pitch.
     SineHalves o;
-    o.se.resize((size_t)m * m);
-    o.se_t.resize((size_t)m * m);
-    o.so.resize((size_t)h * h);
+    o.se.assign((size_t)pm * pm, 0.0f);
+    o.se_t.assign((size_t)pm * pm, 0.0f);
+    o.so.assign((size_t)ph * ph, 0.0f);
     for (int i = 0; i < m; i++)
         for (int j = 0; j < m; j++) {
-            o.se[(size_t)i * m + j] = S(2 * i, j);
-            o.se_t[(size_t)j * m + i] = S(2 * i, j);
+            o.se[(size_t)i * pm + j] = S(2 * i, j);
+            o.se_t[(size_t)j * pm + i] = S(2 * i, j);
         }
     for (int i = 0; i < h; i++)
-        for (int j = 0; j < h; j++) o.so[(size_t)i * h + j] = S(2 * i + 1, j);
+        for (int j = 0; j < h; j++) o.so[(size_t)i * ph + j] = S(2 * i + 1, j);
     return o;
 }
 
@@ -340,9 +342,10 @@ std::string prepare_scene(const EmbSettings &s, const EmbScene &sc, HostScene &o
         for (int k = 1; k <= o.nx; k++) kx[k - 1] = 2.0 * std::cos(PI * (double)k / (double)(o.nx + 1)) - 2.0;
         for (int k = 1; k <= o.ny; k++) ky[k - 1] = 2.0 * std::cos(PI * (double)k / (double)(o.ny + 1)) - 2.0;
         const double norm = 4.0 / (double)((o.nx + 1) * (o.ny + 1));
-        o.inv_lap.resize((size_t)o.nx * o.ny);
+        const int pny = poisson_pitch(o.ny);
+        o.inv_lap.assign((size_t)o.nx * pny, 0.0f);
         for (int i = 0; i < o.nx; i++)
-            for (int j = 0; j < o.ny; j++) o.inv_lap[(size_t)i * o.ny + j] = (float)(norm / (kx[i] + ky[j]));
+            for (int j = 0; j < o.ny; j++) o.inv_lap[(size_t)i * pny + j] = (float)(norm / (kx[i] + ky[j]));
     }
     // Gains (NaN = no entry) and electrons.
     o.gains.assign(o.stacks.size(), 1.0f);

@@ -82,7 +82,8 @@ std::vector<float> sine_matrix(int n);
 // transpose, and the odd rows so[i][j] = S[2i+1][j] over j < h (h x h,
 // h = floor(n/2); so is symmetric). S x is then se p (rows 2i) and so q (rows
 // 2i+1) with p_j = x_j + x_{n-1-j}, q_j = x_j - x_{n-1-j} (j < h) and, for odd n,
-// p_h = x_h.
+// p_h = x_h. Each is stored square and zero-padded to sine_half_pitch of its
+// side (row-major, that pitch), so a product's K can run in whole slices.
 struct SineHalves {
     std::vector<float> se, se_t, so;
 };

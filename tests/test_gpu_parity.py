@@ -85,27 +85,32 @@ def test_one_tick_matches_the_oracle(name, before):
             assert abs(g - c) < 1e-5
 
 
-def odd_world():
-    """A 31 x 32 conductor ring with one battery: 63 x 64 Yee nodes, so the
-    DST's row fold has a middle row (odd length) while its halves still take
-    the unguarded tile path where they can."""
-    w = W.World(31, 32, seed=7)
-    ring = [(c, r) for c in range(4, 27) for r in range(6, 24) if c in (4, 26) or r in (6, 23)]
+def odd_world(cols=31, rows=32):
+    """A conductor ring with one battery on a cols x rows grid. 31 x 32 gives 63
+    x 64 Yee nodes, so the DST's row fold has a middle row (odd length); 32 x 31
+    gives a middle column (the column fold of an odd row length, whose rows are
+    padded and whose mirror columns are not 16-byte aligned); 33 x 34 gives row
+    lengths of 66, even but not a multiple of 4."""
+    w = W.World(cols, rows, seed=7)
+    ring = [(c, r) for c in range(4, cols - 4) for r in range(6, rows - 8)
+            if c in (4, cols - 5) or r in (6, rows - 9)]
     w.paint(ring, W.CONDUCTOR, resistance=1.0)
     w.paint([(15, 6)], W.BATTERY, emf=1.5, orientation=W.RIGHT)
     return w
 
 
 @gpu
+@pytest.mark.parametrize("shape", [(31, 32), (32, 31), (33, 34)])
 @pytest.mark.parametrize("before", [0, 5])
-def test_odd_node_count_one_tick_matches_the_oracle(before):
-    cpu, sim = reset_and_compare(odd_world(), before)
+def test_odd_node_count_one_tick_matches_the_oracle(shape, before):
+    cpu, sim = reset_and_compare(odd_world(*shape), before)
     ex, ey, bz = sim.read_yee()
     cex, cey, cbz = cpu.yee()
-    assert ex.shape == (64, 64) and ey.shape == (63, 65)
+    nx, ny = shape[0] + 32, shape[1] + 32
+    assert ex.shape == (nx + 1, ny) and ey.shape == (nx, ny + 1)
     err = {"ex": relative_error(ex, cex), "ey": relative_error(ey, cey),
            "phi": relative_error(sim.read_potential(), cpu.potential_nodes())}
-    print(before, err)
+    print(shape, before, err)
     for k in ("ex", "ey", "phi"):
         assert err[k] < 1e-5, err
 
