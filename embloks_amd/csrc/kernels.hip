@@ -102,12 +102,22 @@ __global__ __launch_bounds__(256) void k_scan(Batch b) {
             run += v[q];
         }
         if (threadIdx.x == 0) S.toff[tile] = total;
+        // This scan tile's share of the push's dense-tile flags, cleared for
+        // k_canon to set, four flags per store (the flag array's allocation is
+        // padded to 256 bytes, so its last word is whole).
+        const int ntiles = (cells + SCAN_TILE - 1) / SCAN_TILE;
+        const int ptiles = ((S.cols + PUSH_TILE - 1) / PUSH_TILE) * ((S.rows + PUSH_TILE - 1) / PUSH_TILE);
+        const int words = (ptiles + 3) / 4, per = (words + ntiles - 1) / ntiles;
+        uint32_t *flags = reinterpret_cast<uint32_t *>(S.tile_dense);
+        for (int q = tile * per + (int)threadIdx.x; q < min(words, (tile + 1) * per); q += 256) flags[q] = 0u;
         __syncthreads();
     }
 }
 
-// One block per scene: the scan tiles' totals become their offsets, and the
-// dense-tile flags of the push are cleared for k_canon to set.
+// One block per scene: the scan tiles' totals become their offsets. Each
+// thread takes TOP_RUN consecutive totals per round (8192 per round), so a
+// 4096^2 grid's 16k tiles take two rounds of the block scan.
+constexpr int TOP_RUN = 32;
 __global__ __launch_bounds__(256) void k_scan_top(Batch b) {
     __shared__ int wsum[4];
     for (int blk = blockIdx.x; blk < b.total[KC_SCENE]; blk += gridDim.x) {
@@ -115,16 +125,23 @@ __global__ __launch_bounds__(256) void k_scan_top(Batch b) {
         const DevScene &S = b.scenes[s];
         const int ntiles = (S.cols * S.rows + SCAN_TILE - 1) / SCAN_TILE;
         int carry = 0;
-        for (int c0 = 0; c0 < ntiles; c0 += 256) {
-            const int k = c0 + (int)threadIdx.x;
-            const int t = k < ntiles ? S.toff[k] : 0;
+        for (int c0 = 0; c0 < ntiles; c0 += 256 * TOP_RUN) {
+            const int k0 = c0 + (int)threadIdx.x * TOP_RUN;
+            int v[TOP_RUN], sum = 0;
+#pragma unroll
+            for (int q = 0; q < TOP_RUN; q++) {
+                v[q] = k0 + q < ntiles ? S.toff[k0 + q] : 0;
+                sum += v[q];
+            }
             int chunk_total;
-            const int ex = block_scan_256(t, wsum, chunk_total);
-            if (k < ntiles) S.toff[k] = carry + ex;
+            int run = carry + block_scan_256(sum, wsum, chunk_total);
+#pragma unroll
+            for (int q = 0; q < TOP_RUN; q++) {
+                if (k0 + q < ntiles) S.toff[k0 + q] = run;
+                run += v[q];
+            }
             carry += chunk_total;
         }
-        const int ptiles = ((S.cols + PUSH_TILE - 1) / PUSH_TILE) * ((S.rows + PUSH_TILE - 1) / PUSH_TILE);
-        for (int q = threadIdx.x; q < ptiles; q += 256) S.tile_dense[q] = 0;
         if (threadIdx.x == 0) S.ctr->heavy_n = 0;
         __syncthreads();
     }
