@@ -199,7 +199,7 @@ struct DevScene {
     const float *density;     // per cell (ion density)
     const float *ion_pot;     // per cell: close-range potential of all ions
     const float *g_table;     // PAIR_SAMPLES + 4: the reference's PAIR_SAMPLES + 2 samples, then two zeros
-    const lerp2 *g_lerp;      // PAIR_SAMPLES + 4: (g[i], g[i+1] - g[i]) below the cutoff, then (0, 0)
+    const lerp2 *g_lerp;      // PAIR_SAMPLES + 4: (g[i] - i d, d), d = g[i+1] - g[i], below the cutoff, then (0, 0)
     const float *square;      // square_n^2 * 3
     const lerp2 *square_f2;   // square_n^2: the force channels (1, 2) of the square table as pairs
     const float *ion_sum;     // ION_SUM^2 * 2: the square table's force channels summed over

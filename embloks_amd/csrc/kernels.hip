@@ -595,13 +595,14 @@ __device__ __forceinline__ void pair_term_c(const lerp2 *gt, float gs, float dx,
     const float d2 = (dx * dx + dy * dy) * gs;
     const float tt = fminf(d2, (float)(PAIR_SAMPLES + 1));
     const lerp2 g = gt[(int)tt];
-    const float l = g.x + g.y * __builtin_amdgcn_fractf(tt);
+    const float l = g.x + g.y * tt;
     sx += l * dx;
     sy += l * dy;
 }
 
 // One pair term g(r^2) d, g lerped from the r^2-uniform table
-// (close_range.rs:359-372). The offset (ix, iy) is in 2^-24 cells; s2 =
+// (close_range.rs:359-372), each segment held as intercept and slope in the
+// table coordinate. The offset (ix, iy) is in 2^-24 cells; s2 =
 // g_scale 2^-48 turns its square into the table coordinate r^2 PAIR_SAMPLES /
 // cutoff^2. A pair at or beyond the cutoff (d2 >= PAIR_SAMPLES) is clamped to
 // entries PAIR_SAMPLES or PAIR_SAMPLES + 1, which are both (0, 0), and the
@@ -612,7 +613,7 @@ __device__ __forceinline__ void pair_term(const lerp2 *gt, float s2, int ix, int
     const float d2 = (dx * dx + dy * dy) * s2;
     const float tt = fminf(d2, (float)(PAIR_SAMPLES + 1));
     const lerp2 g = gt[(int)tt];
-    const float l = g.x + g.y * __builtin_amdgcn_fractf(tt);
+    const float l = g.x + g.y * tt;
     sx += l * dx;
     sy += l * dy;
 }

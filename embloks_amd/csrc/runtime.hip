@@ -208,16 +208,22 @@ int close_tables_locked(EmbDevice *dev, const EmbSettings &s, DeviceTables &tab)
             EMB_HIP(hipMalloc(&t.square, sq.size() * sizeof(float)));
             EMB_HIP(hipMemcpy(t.g, g.data(), g.size() * sizeof(float), hipMemcpyHostToDevice));
             EMB_HIP(hipMemcpy(t.square, sq.data(), sq.size() * sizeof(float), hipMemcpyHostToDevice));
-            // (g[i], g[i+1] - g[i]) for every sample the lerp can start at below
-            // the cutoff; entries PAIR_SAMPLES and up are (0, 0), where pairs at
-            // or beyond the cutoff are clamped.
+            // Segment i of the lerp as intercept and slope in the table
+            // coordinate t: g(t) = (g[i] - i d) + d t with d = g[i+1] - g[i], so
+            // the pair loop needs no fraction of t (formed in FP64, rounded once;
+            // within 1.6e-7 of max |g r| of the exact lerp, as the fraction form).
+            // Entries PAIR_SAMPLES and up are (0, 0), where pairs at or beyond the
+            // cutoff are clamped.
             std::vector<double> gp(h->g.begin(), h->g.end());
             gp.push_back(0.0);
             gp.push_back(0.0);
             gp.push_back(0.0);
             std::vector<float2> gl(gp.size() - 1);
-            for (size_t i = 0; i + 1 < gp.size(); i++)
-                gl[i] = i < (size_t)PAIR_SAMPLES ? make_float2((float)gp[i], (float)(gp[i + 1] - gp[i])) : make_float2(0.0f, 0.0f);
+            for (size_t i = 0; i + 1 < gp.size(); i++) {
+                const double d = gp[i + 1] - gp[i];
+                gl[i] = i < (size_t)PAIR_SAMPLES ? make_float2((float)(gp[i] - (double)i * d), (float)d)
+                                                 : make_float2(0.0f, 0.0f);
+            }
             EMB_HIP(hipMalloc(&t.g_lerp, gl.size() * sizeof(float2)));
             EMB_HIP(hipMemcpy(t.g_lerp, gl.data(), gl.size() * sizeof(float2), hipMemcpyHostToDevice));
             std::vector<float2> sf(h->square.size() / 3);
