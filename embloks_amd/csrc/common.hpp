@@ -18,6 +18,7 @@
 #include <hip/hip_runtime.h>
 typedef float2 lerp2;
 typedef float2 force2;
+typedef float4 snap4;
 #else
 #define __host__
 #define __device__
@@ -26,6 +27,9 @@ struct lerp2 {
 };
 struct force2 {
     float x, y;
+};
+struct snap4 {
+    float x, y, z, w;
 };
 #endif
 
@@ -178,7 +182,7 @@ struct DevScene {
     float *fx, *fy, *vx, *vy;
     // --- cell list
     int32_t *scell;           // per sorted slot: flat cell index (c * rows + r)
-    float *sfx, *sfy, *svx, *svy; // electrons in sorted order (snapshot of the substep start)
+    snap4 *snap;                  // electrons in sorted order (snapshot of the substep start): x, y, vx, vy
     force2 *pforce;           // per sorted slot: close-range + ion force of the cell-mode electrons (k_push_tiled)
     int32_t *order;           // electron index per sorted slot
     int32_t *ecell, *erank;   // per electron: cell and rank within it

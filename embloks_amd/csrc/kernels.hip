@@ -23,6 +23,17 @@
 
 namespace emb {
 
+// The sorted snapshot holds (x, y, vx, vy) per electron in one 16-byte entry:
+// the deposit reads an electron with one load, a pair loop its position half.
+__device__ __forceinline__ float2 snap_pos(const DevScene &S, int j) {
+    const float4 e = S.snap[j];
+    return make_float2(e.x, e.y);
+}
+__device__ __forceinline__ float2 snap_vel(const DevScene &S, int j) {
+    const float4 e = S.snap[j];
+    return make_float2(e.z, e.w);
+}
+
 // ---------------------------------------------------------------------------------------
 // Counting sort by clamped cell (close_range.rs:258-286; particles.wgsl:23-85).
 // Ranks within a cell come from atomics; k_canon then orders each cell's slots by
@@ -172,10 +183,7 @@ __device__ __forceinline__ void canon_cell(const DevScene &S, int c, int s0, int
     for (int a = s0; a < s1; a++) {
         const int e = order[a];
         S.scell[a] = c;
-        S.sfx[a] = S.fx[e];
-        S.sfy[a] = S.fy[e];
-        S.svx[a] = S.vx[e];
-        S.svy[a] = S.vy[e];
+        S.snap[a] = make_float4(S.fx[e], S.fy[e], S.vx[e], S.vy[e]);
     }
 }
 
@@ -264,10 +272,7 @@ __global__ __launch_bounds__(256) void k_canon(Batch b) {
                 if (q >= S1) continue;
                 S.order[q] = e[u];
                 S.scell[q] = cel[u];
-                S.sfx[q] = fxv[u];
-                S.sfy[q] = fyv[u];
-                S.svx[q] = vxv[u];
-                S.svy[q] = vyv[u];
+                S.snap[q] = make_float4(fxv[u], fyv[u], vxv[u], vyv[u]);
             }
         }
         __syncthreads();
@@ -277,25 +282,20 @@ __global__ __launch_bounds__(256) void k_canon(Batch b) {
 // ---------------------------------------------------------------------------------------
 // Close-range tables (close_range.rs:244-254, common.wgsl:88-100).
 
-// Channel k of the square table at fractional indices, `a`/`b` from truncation
-// as in CloseRange::force (close_range.rs:337-347).
-__device__ __forceinline__ float square_trunc(const DevScene &S, int k, float u, float v) {
-    const int n = S.square_n;
-    const int a = clampi((int)u, 0, n - 2), b = clampi((int)v, 0, n - 2);
-    const float fu = u - (float)a, fv = v - (float)b;
-    const float *t = S.square + ((size_t)a * n + b) * 3 + k;
-    return t[0] * (1.0f - fu) * (1.0f - fv) + t[(size_t)n * 3] * fu * (1.0f - fv) + t[3] * (1.0f - fu) * fv +
-           t[(size_t)n * 3 + 3] * fu * fv;
-}
-
-// The same with floor, as table_bilinear (close_range.rs:245-254).
+// Channel k of the square table at fractional indices, as table_bilinear
+// (close_range.rs:245-254). The blend's products and sums are written as
+// explicit FMAs, so every kernel that reads the table rounds it the same way,
+// whichever products the compiler would otherwise fuse.
 __device__ __forceinline__ float square_floor(const DevScene &S, int k, float u, float v) {
     const int n = S.square_n;
     const int a = clampi((int)floorf(u), 0, n - 2), b = clampi((int)floorf(v), 0, n - 2);
-    const float fu = u - (float)a, fv = v - (float)b;
+    const float fu = u - (float)a, fv = v - (float)b, gu = 1.0f - fu, gv = 1.0f - fv;
     const float *t = S.square + ((size_t)a * n + b) * 3 + k;
-    return t[0] * (1.0f - fu) * (1.0f - fv) + t[(size_t)n * 3] * fu * (1.0f - fv) + t[3] * (1.0f - fu) * fv +
-           t[(size_t)n * 3 + 3] * fu * fv;
+    const size_t n3 = (size_t)n * 3;
+    float r = __fmul_rn(__fmul_rn(t[0], gu), gv);
+    r = __fmaf_rn(__fmul_rn(t[n3], fu), gv, r);
+    r = __fmaf_rn(__fmul_rn(t[3], gu), fv, r);
+    return __fmaf_rn(__fmul_rn(t[n3 + 3], fu), fv, r);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -357,8 +357,9 @@ __global__ __launch_bounds__(TRIM_THREADS) void k_trim(Batch b) {
                     // Offset of the electron from the centre of cell (c, r).
                     const float ox = (float)(cc - c) - 0.5f + S.half, oy = (float)(rr - r) - 0.5f + S.half;
                     for (int k = S.cstart[nb] + part; k < S.cstart[nb + 1]; k += TRIM_SPLIT) {
-                        const float u = (ox + S.sfx[k]) * S.square_samples;
-                        const float w = (oy + S.sfy[k]) * S.square_samples;
+                        const float2 f = snap_pos(S, k);
+                        const float u = (ox + f.x) * S.square_samples;
+                        const float w = (oy + f.y) * S.square_samples;
                         v += square_floor(S, 0, u, w);
                     }
                     v *= S.k_q;
@@ -497,7 +498,10 @@ struct GlobalNeighbours {
         b = S.cstart[cell];
         e = S.cstart[cell + 1];
     }
-    __device__ int2 p(int j) const { return fixed_pos(0, 0, S.sfx[j], S.sfy[j]); }
+    __device__ int2 p(int j) const {
+        const float2 f = snap_pos(S, j);
+        return fixed_pos(0, 0, f.x, f.y);
+    }
     __device__ int2 origin(int hc, int hr, float px, float py, int cc, int rr) const {
         return fixed_pos(hc - cc, hr - rr, px, py);
     }
@@ -650,8 +654,9 @@ __device__ void push_core(const DevScene &S, const NB &nb, const lerp2 *gt, int 
     const int id = S.order[slot];
     const int home = S.scell[slot];
     const int hc = home / S.rows, hr = home - (home / S.rows) * S.rows;
-    const float px = S.sfx[slot], py = S.sfy[slot];   // fractions within the home cell
-    const float vx = S.svx[slot], vy = S.svy[slot];
+    const float4 e4 = S.snap[slot];
+    const float px = e4.x, py = e4.y;   // fractions within the home cell
+    const float vx = e4.z, vy = e4.w;
     const CellInfo info = S.info[home];
     float ax, ay;
     field_accel(S, hc, hr, px, py, vx, vy, info.flags, k, h, t, ax, ay);
@@ -700,8 +705,9 @@ __device__ __forceinline__ void kick_core(const DevScene &S, int slot, int k, fl
     const int id = S.order[slot];
     const int home = S.scell[slot];
     const int hc = home / S.rows, hr = home - (home / S.rows) * S.rows;
-    const float px = S.sfx[slot], py = S.sfy[slot];
-    const float vx = S.svx[slot], vy = S.svy[slot];
+    const float4 e4 = S.snap[slot];
+    const float px = e4.x, py = e4.y;
+    const float vx = e4.z, vy = e4.w;
     const CellInfo info = S.info[home];
     float ax, ay;
     field_accel(S, hc, hr, px, py, vx, vy, info.flags, k, h, t, ax, ay);
@@ -757,7 +763,8 @@ struct GlobalRuns {
     int yc;
     __device__ float2 at(int j, int cc) const {
         const int rr = S.scell[j] - cc * S.rows;
-        return cell_pos(rr, yc, S.sfx[j], S.sfy[j]);
+        const float2 f = snap_pos(S, j);
+        return cell_pos(rr, yc, f.x, f.y);
     }
 };
 
@@ -797,8 +804,9 @@ __device__ __forceinline__ void cell_item(const DevScene &S, const RUNS &runs, c
         const int ei = chunk * E * I + e * I + i;
         live[e] = ei < n;
         slot[e] = e0 + (live[e] ? ei : n - 1);
-        px[e] = S.sfx[slot[e]];
-        by[e] = cell_pos(hr, yc, px[e], S.sfy[slot[e]]).y;
+        const float2 f = snap_pos(S, slot[e]);
+        px[e] = f.x;
+        by[e] = cell_pos(hr, yc, f.x, f.y).y;
         sx[e] = 0.0f;
         sy[e] = 0.0f;
     }
@@ -834,7 +842,7 @@ __device__ __forceinline__ void cell_item(const DevScene &S, const RUNS &runs, c
     // sl < E, finishes it.
     const int me = E == 2 ? (sl & 1) : 0;
     // (the y fraction is read again rather than kept through the pair loop)
-    const float mpx = E == 2 && me ? px[E - 1] : px[0], mpy = S.sfy[E == 2 && me ? slot[E - 1] : slot[0]];
+    const float mpx = E == 2 && me ? px[E - 1] : px[0], mpy = snap_pos(S, E == 2 && me ? slot[E - 1] : slot[0]).y;
     const CellInfo info = S.info[home];
     const bool uniform = (info.flags & F_ION_UNIFORM) != 0u;
     float fx = 0.0f, fy = 0.0f;
@@ -869,7 +877,8 @@ __device__ __forceinline__ void cell_item(const DevScene &S, const RUNS &runs, c
         } else if (MODE == ITEM_STORE) {
             S.pforce[mslot] = make_float2(fx, fy);
         } else {
-            const float vx = S.svx[mslot], vy = S.svy[mslot];
+            const float2 v = snap_vel(S, mslot);
+            const float vx = v.x, vy = v.y;
             float ax, ay;
             field_accel(S, hc, hr, mpx, mpy, vx, vy, info.flags, k, h, t, ax, ay);
             ax += fx * S.inv_mass;
@@ -888,7 +897,8 @@ template <class NB>
 __device__ void force_core(const DevScene &S, const NB &nb, const lerp2 *gt, int slot, float2 *force_out) {
     const int home = S.scell[slot];
     const int hc = home / S.rows, hr = home - (home / S.rows) * S.rows;
-    const float px = S.sfx[slot], py = S.sfy[slot];
+    const float2 f = snap_pos(S, slot);
+    const float px = f.x, py = f.y;
     const CellInfo info = S.info[home];
     float fx = 0.0f, fy = 0.0f, sx = 0.0f, sy = 0.0f;
     const int R = S.reach;
@@ -1047,7 +1057,8 @@ __device__ __forceinline__ void push_tile(const DevScene &S, PushTileLds &L, int
                     if (q + step < cb && col_base[q + step] - base <= j) q += step;
                 const int g = col_g0[q] + (j - (col_base[q] - base));
                 const int rr = S.scell[g] - (wc0 + q) * S.rows;
-                lp[j] = cell_pos(rr, wr0 + (W >> 1), S.sfx[g], S.sfy[g]);
+                const float2 f = snap_pos(S, g);
+                lp[j] = cell_pos(rr, wr0 + (W >> 1), f.x, f.y);
             }
         }
         __syncthreads();
@@ -1231,7 +1242,8 @@ __global__ __launch_bounds__(HEAVY_THREADS) void k_push_heavy(Batch b, int k, fl
                         if (q + step < W && col_base[q + step] <= j) q += step;
                     const int g = col_g0[q] + (j - col_base[q]);
                     const int rr = S.scell[g] - (wc0 + q) * S.rows;
-                    lp[j] = cell_pos(rr, wr0 + (W >> 1), S.sfx[g], S.sfy[g]);
+                    const float2 f = snap_pos(S, g);
+                    lp[j] = cell_pos(rr, wr0 + (W >> 1), f.x, f.y);
                 }
             }
             __syncthreads();
@@ -1520,7 +1532,7 @@ __global__ __launch_bounds__(DEP_THREADS) void k_deposit_smooth(Batch b, int bia
         const int ncy = max(sr1 - sr0 + 1, 0), ncell = max(sc1 - sc0 + 1, 0) * ncy;
         // The scene's arrays in registers: the LDS atomics would otherwise make the
         // compiler reload them from the scene table every iteration.
-        const float *sfx = S.sfx, *sfy = S.sfy, *svx = S.svx, *svy = S.svy;
+        const float4 *snap = S.snap;
         const int32_t *cstart = S.cstart;
         const int rows = S.rows, cols = S.cols;
         float vmax = 0.0f;
@@ -1544,11 +1556,11 @@ __global__ __launch_bounds__(DEP_THREADS) void k_deposit_smooth(Batch b, int bia
 #pragma unroll
                 for (int u = 0; u < 4; u++) {
                     const bool on = kb + u < k1;
-                    const int k = on ? kb + u : k0;
-                    xs[u] = sfx[k];
-                    ys[u] = sfy[k];
-                    vxs[u] = on ? svx[k] : 0.0f;
-                    vys[u] = on ? svy[k] : 0.0f;
+                    const float4 e4 = snap[on ? kb + u : k0];
+                    xs[u] = e4.x;
+                    ys[u] = e4.y;
+                    vxs[u] = on ? e4.z : 0.0f;
+                    vys[u] = on ? e4.w : 0.0f;
                 }
 #pragma unroll
                 for (int u = 0; u < 4; u++) {
@@ -2139,8 +2151,10 @@ __global__ __launch_bounds__(256) void k_probe(const DevScene *scene, int cell, 
             for (int rr = max(r - R, 0); rr <= min(r + R, S.rows - 1); rr++) {
                 const int nb = cc * S.rows + rr;
                 const float ox = (float)(cc - c) - 0.5f + S.half, oy = (float)(rr - r) - 0.5f + S.half;
-                for (int k = S.cstart[nb]; k < S.cstart[nb + 1]; k++)
-                    near += square_floor(S, 0, (ox + S.sfx[k]) * S.square_samples, (oy + S.sfy[k]) * S.square_samples);
+                for (int k = S.cstart[nb]; k < S.cstart[nb + 1]; k++) {
+                    const float2 f = snap_pos(S, k);
+                    near += square_floor(S, 0, (ox + f.x) * S.square_samples, (oy + f.y) * S.square_samples);
+                }
             }
         pot += S.k_q * near;
     }

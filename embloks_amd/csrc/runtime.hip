@@ -93,10 +93,7 @@ void layout(Bump &a, const Counts &c, DevScene &d, DevScene **d_scene, int32_t *
     d.vx = a.take<float>(n);
     d.vy = a.take<float>(n);
     d.scell = a.take<int32_t>(n);
-    d.sfx = a.take<float>(n);
-    d.sfy = a.take<float>(n);
-    d.svx = a.take<float>(n);
-    d.svy = a.take<float>(n);
+    d.snap = a.take<snap4>(n);
     d.pforce = a.take<float2>(n);
     d.order = a.take<int32_t>(n);
     d.ecell = a.take<int32_t>(n);
