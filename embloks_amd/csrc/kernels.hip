@@ -791,7 +791,12 @@ __device__ __forceinline__ void cell_item(const DevScene &S, const RUNS &runs, c
     const int lane = threadIdx.x & 63;
     const int R = S.reach;
     const int home = hc * S.rows + hr;
-    const int I = item_lanes(n, E, E == 2 ? min(imax, 32) : imax), shift = __builtin_ctz(I), SL = 64 >> shift;
+    // Every chunk but a cell's last holds E Ifull electrons; the last takes as
+    // many lanes as its own electrons need.
+    // (read as wave-uniform: the loop bounds below derive from I)
+    const int imx = E == 2 ? min(imax, 32) : imax;
+    const int base = __builtin_amdgcn_readfirstlane(chunk * E * item_lanes(n, E, imx));
+    const int I = __builtin_amdgcn_readfirstlane(item_lanes(n - base, E, imx)), shift = __builtin_ctz(I), SL = 64 >> shift;
     const int i = lane & (I - 1), sl = lane >> shift;
     int slot[E];
     bool live[E];
@@ -801,7 +806,7 @@ __device__ __forceinline__ void cell_item(const DevScene &S, const RUNS &runs, c
     float sx[E], sy[E];
 #pragma unroll
     for (int e = 0; e < E; e++) {
-        const int ei = chunk * E * I + e * I + i;
+        const int ei = base + e * I + i;
         live[e] = ei < n;
         slot[e] = e0 + (live[e] ? ei : n - 1);
         const float2 f = snap_pos(S, slot[e]);
@@ -887,6 +892,11 @@ __device__ __forceinline__ void cell_item(const DevScene &S, const RUNS &runs, c
         }
     }
 }
+
+// Lanes per electron pair of a cell-mode item at most: a cell of more than
+// 2 CELL_IMAX electrons splits into items of 2 CELL_IMAX and a last one sized to
+// the rest (17 electrons: 8 + 1 lanes' walks, not 16).
+constexpr int CELL_IMAX = 8;
 
 // Electrons per lane in the cell mode: two share each neighbour read, except in
 // a cell of one.
@@ -1011,7 +1021,7 @@ __device__ __forceinline__ void push_tile(const DevScene &S, PushTileLds &L, int
         }
         cell_e0[lane] = first;
         cell_n[lane] = m;
-        const int E = item_e(m), I = item_lanes(m, E, E == 2 ? 32 : 64);
+        const int E = item_e(m), I = item_lanes(m, E, E == 2 ? CELL_IMAX : 64);
         const int items = (m > 0 && (HEAVY || m < HEAVY_CELL)) ? (m + E * I - 1) / (E * I) : 0;
         const int item_incl = wave_scan_incl(items);
         item_pref[lane] = item_incl - items;
@@ -1076,15 +1086,15 @@ __device__ __forceinline__ void push_tile(const DevScene &S, PushTileLds &L, int
             if (staged) {
                 const LdsRuns runs{lp, lb, le, wc0, wr0, W};
                 if (item_e(n) == 2)
-                    cell_item<MODE, 2>(S, runs, gt, dens_w, hc, hr, e0, n, chunk, wc0, wr0, W, k, h, t, force_out);
+                    cell_item<MODE, 2>(S, runs, gt, dens_w, hc, hr, e0, n, chunk, wc0, wr0, W, k, h, t, force_out, CELL_IMAX);
                 else
-                    cell_item<MODE, 1>(S, runs, gt, dens_w, hc, hr, e0, n, chunk, wc0, wr0, W, k, h, t, force_out);
+                    cell_item<MODE, 1>(S, runs, gt, dens_w, hc, hr, e0, n, chunk, wc0, wr0, W, k, h, t, force_out, CELL_IMAX);
             } else {
                 const GlobalRuns runs{S, wc0, wr0, wr0 + (W >> 1)};
                 if (item_e(n) == 2)
-                    cell_item<MODE, 2>(S, runs, gt, dens_w, hc, hr, e0, n, chunk, wc0, wr0, W, k, h, t, force_out);
+                    cell_item<MODE, 2>(S, runs, gt, dens_w, hc, hr, e0, n, chunk, wc0, wr0, W, k, h, t, force_out, CELL_IMAX);
                 else
-                    cell_item<MODE, 1>(S, runs, gt, dens_w, hc, hr, e0, n, chunk, wc0, wr0, W, k, h, t, force_out);
+                    cell_item<MODE, 1>(S, runs, gt, dens_w, hc, hr, e0, n, chunk, wc0, wr0, W, k, h, t, force_out, CELL_IMAX);
             }
         }
         __syncthreads();
