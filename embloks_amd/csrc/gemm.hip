@@ -322,7 +322,7 @@ int dst_gemm_blocks(int nx, int ny) {
 }
 
 template <int STAGE>
-__global__ __launch_bounds__(256) void k_dst_gemm(Batch b) {
+__global__ __launch_bounds__(256) void k_dst_gemm(Batch b, int chans) {
     constexpr bool LEFT = (STAGE == 0 || STAGE == 2);
     __shared__ __align__(16) float lds[2 * 16 * (TICK_BM + 4 + TICK_BN + 4)];
     for (int blk = blockIdx.x; blk < b.total[KC_GEMM]; blk += gridDim.x) {
@@ -334,6 +334,10 @@ __global__ __launch_bounds__(256) void k_dst_gemm(Batch b) {
         const int local = block_in_scene(b, KC_GEMM, s, blk);
         const int ch = local / (te + to);
         if (ch >= 2) continue;   // the scene's block count covers the larger stage
+        // Channel 0 (the charge's potential) only where it is wanted: every
+        // tick in a scene with battery stacks (the trim reads it), otherwise on
+        // demand (DST_POT) when the potential is read.
+        if (chans == DST_POT ? ch != 0 : (chans == DST_TICK && ch == 0 && S.n_stacks == 0)) continue;
         const int rem0 = local - ch * (te + to);
         const int odd = rem0 >= te ? 1 : 0;
         const int rem = rem0 - odd * te;
@@ -384,20 +388,20 @@ __global__ __launch_bounds__(256) void k_dst_gemm(Batch b) {
     }
 }
 
-template __global__ void k_dst_gemm<0>(Batch);
-template __global__ void k_dst_gemm<1>(Batch);
-template __global__ void k_dst_gemm<2>(Batch);
-template __global__ void k_dst_gemm<3>(Batch);
+template __global__ void k_dst_gemm<0>(Batch, int);
+template __global__ void k_dst_gemm<1>(Batch, int);
+template __global__ void k_dst_gemm<2>(Batch, int);
+template __global__ void k_dst_gemm<3>(Batch, int);
 
 // The four products of one Poisson solve for every scene of the batch.
-hipError_t launch_dst_gemms(const Batch &b, hipStream_t stream) {
+hipError_t launch_dst_gemms(const Batch &b, hipStream_t stream, int chans) {
     const int total = b.total[KC_GEMM];
     if (total == 0) return hipSuccess;
     const int grid = total < 8192 ? total : 8192;
-    k_dst_gemm<0><<<grid, 256, 0, stream>>>(b);
-    k_dst_gemm<1><<<grid, 256, 0, stream>>>(b);
-    k_dst_gemm<2><<<grid, 256, 0, stream>>>(b);
-    k_dst_gemm<3><<<grid, 256, 0, stream>>>(b);
+    k_dst_gemm<0><<<grid, 256, 0, stream>>>(b, chans);
+    k_dst_gemm<1><<<grid, 256, 0, stream>>>(b, chans);
+    k_dst_gemm<2><<<grid, 256, 0, stream>>>(b, chans);
+    k_dst_gemm<3><<<grid, 256, 0, stream>>>(b, chans);
     return hipGetLastError();
 }
 

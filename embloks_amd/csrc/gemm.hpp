@@ -58,7 +58,11 @@ hipError_t launch_gemm_nt_bf16x3(const __bf16 *ah, const __bf16 *al, int lda, co
 
 // The four products of the tick's solve for every scene of a batch, and the
 // blocks per scene they need (KC_GEMM).
-hipError_t launch_dst_gemms(const Batch &b, hipStream_t stream);
+// Which channels a solve runs: both (scene set-up), the tick's (the residual
+// always, the charge's potential where the scene has battery stacks), or the
+// potential alone (read on demand).
+enum { DST_BOTH = 0, DST_TICK = 1, DST_POT = 2 };
+hipError_t launch_dst_gemms(const Batch &b, hipStream_t stream, int chans);
 int dst_gemm_blocks(int nx, int ny);
 
 }  // namespace emb

@@ -20,6 +20,7 @@
 //   dst GEMMs   (gemm.hip) and k_correct: E -= grad(phi) (maxwell.rs:176-204)
 
 #include "device.hpp"
+#include "gemm.hpp"
 
 namespace emb {
 
@@ -1958,7 +1959,7 @@ __global__ __launch_bounds__(256) void k_diagnostics(const DevScene *scene, doub
 // ---------------------------------------------------------------------------------------
 // Kernel sequences (host side).
 
-hipError_t launch_dst_gemms(const Batch &b, hipStream_t stream);
+
 
 static inline int grid_of(int total) { return total < 8192 ? total : 8192; }
 
@@ -1990,7 +1991,7 @@ hipError_t setup_kernels(const Batch &b, hipStream_t stream) {
     // arithmetic, as on fields that already satisfy the law.
     for (int pass = 0; pass < 2; pass++) {
         EMB_LAUNCH(k_rhs, KC_NODES, 256, b);
-        e = launch_dst_gemms(b, stream);
+        e = launch_dst_gemms(b, stream, DST_BOTH);
         if (e != hipSuccess) return e;
         EMB_LAUNCH(k_correct, KC_NODES, 256, b, 0);
     }
@@ -2064,7 +2065,7 @@ hipError_t tick_kernels(const Batch &b, int fdtd_substeps, hipStream_t stream, h
     EMB_MARK(8);
     for (int f = 0; f < fdtd_substeps; f++) EMB_LAUNCH(k_fdtd, KC_FDTD, 64 * FDTD_WAVES, b, f);
     EMB_MARK(9);
-    hipError_t e = launch_dst_gemms(b, stream);
+    hipError_t e = launch_dst_gemms(b, stream, DST_TICK);
     if (e != hipSuccess) return e;
     EMB_MARK(10);
     EMB_LAUNCH(k_correct, KC_NODES, 256, b, 1);

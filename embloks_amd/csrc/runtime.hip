@@ -541,7 +541,22 @@ struct EmbSim {
     // the resident scene.
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
+    // The charge's potential (phi's channel 0) is solved every tick only in a
+    // scene with battery stacks; otherwise on the first read after a tick
+    // (read_potential, the probe), from the right-hand side the tick left.
+    bool pot_fresh = true;
 };
+
+namespace {
+
+int fresh_potential(EmbSim *sim) {
+    if (sim->pot_fresh) return EMB_OK;
+    EMB_HIP(launch_dst_gemms(sim->r.batch, sim->stream, DST_POT));
+    sim->pot_fresh = true;
+    return EMB_OK;
+}
+
+}  // namespace
 
 namespace {
 
@@ -718,6 +733,7 @@ int emb_sim_upload(EmbSim *sim, const EmbScene *scene, int reset_display) {
         drop_tick_graph(sim);   // it holds the old scene's descriptors
         free_resident(sim->r);
         sim->r = std::move(fresh);
+        sim->pot_fresh = true;   // set-up solved both channels
         return EMB_OK;
     })
 }
@@ -744,6 +760,7 @@ int emb_sim_step(EmbSim *sim, int ticks) {
                 if (graphs) EMB_HIP(hipGraphLaunch(sim->graph_exec, sim->stream));
                 else EMB_HIP(tick_kernels(sim->r.batch, sim->r.ds.fdtd_substeps, sim->stream));
             }
+            if (chunk > 0) sim->pot_fresh = sim->r.ds.n_stacks > 0;
             ticks -= chunk;
         }
         return EMB_OK;
@@ -848,6 +865,9 @@ int emb_sim_write_yee(EmbSim *sim, const double *ex, const double *ey, const dou
 int emb_sim_read_potential(EmbSim *sim, double *potential) {
     EMB_GUARD({
         if (!sim || !potential) return fail(EMB_E_INVALID, "null argument");
+        EMB_HIP(hipSetDevice(sim->dev->ordinal));
+        const int rc = fresh_potential(sim);
+        if (rc != EMB_OK) return rc;
         EMB_HIP(hipStreamSynchronize(sim->stream));
         const DevScene &d = sim->r.ds;
         // Rows of phi are padded to d.pny floats.
@@ -959,6 +979,8 @@ int emb_sim_probe(EmbSim *sim, int32_t c, int32_t r, int32_t arrow_stride, EmbPr
         const double c2 = res.s.speed_of_light * res.s.speed_of_light;
         const double inv_mu0 = c2 / (4.0 * M_PI * res.s.coulomb_k);   // sim.rs:262
         unsigned *hist = reinterpret_cast<unsigned *>(sim->probe + WORDS);
+        const int rc = fresh_potential(sim);   // the probe reads the potential
+        if (rc != EMB_OK) return rc;
         EMB_HIP(probe_kernel(res.d_scene, cell, arrow_stride, (float)c2, (float)inv_mu0, sim->probe, hist, sim->stream));
         std::vector<float> w(WORDS);
         std::memset(out, 0, sizeof(*out));
@@ -1056,6 +1078,7 @@ int emb_sim_kernel_times(EmbSim *sim, int ticks, double *avg_ms, int capacity) {
                 if (e != hipSuccess) rc = fail(EMB_E_DEVICE, hipGetErrorString(e));
             }
             done += chunk;
+            sim->pot_fresh = sim->r.ds.n_stacks > 0;
         }
         if (rc == EMB_OK) {
             hipError_t e = hipStreamSynchronize(sim->stream);

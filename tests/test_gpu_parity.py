@@ -40,6 +40,29 @@ def test_static_fields_match_the_oracle(name):
     assert relative_error(sim.read_current(), cpu.current()) < 1e-6
 
 
+@gpu
+def test_potential_solved_on_read_is_the_same_whenever_it_is_read():
+    # A scene without battery stacks solves the charge's potential only when it
+    # is read (read_potential, the probe), from the right-hand side the last
+    # tick left: reading it after every tick, twice in a row or after a probe
+    # gives the same bits. (The one-tick comparisons hold it to the oracle.)
+    world = W.random_wire(40, 32)
+    assert not np.any(world.kind == W.BATTERY)
+    a, b = gpusim()(world), gpusim()(world)
+    for _ in range(3):
+        a.run_ticks(1)
+        a.read_potential()
+    b.run_ticks(3)
+    want = b.read_potential()
+    np.testing.assert_array_equal(a.read_potential(), want)
+    np.testing.assert_array_equal(b.read_potential(), want)
+    p = b.read_probe((10, 10))["potential"]
+    np.testing.assert_array_equal(b.read_potential(), want)
+    assert p == a.read_probe((10, 10))["potential"]
+    a.destroy()
+    b.destroy()
+
+
 def reset_and_compare(world, ticks_before=0):
     """Both backends from the same state; compare after one tick."""
     cpu = orc.Sim(world)
