@@ -61,6 +61,7 @@ constexpr int MAX_STACKS_PER_BLOCK = 64;
 constexpr int SCAN_TILE = 1024;   // cells per block of the cell-list scan
 constexpr int TICK_TABLE = 64;    // ticks of host-computed AC phases / EMF targets
 constexpr int PUSH_TILE = 8;      // cells per side of a push tile
+constexpr int PUSH_BLOCKS_MAX = 2048;   // tiled-push blocks per scene: four rounds of the chip's 512 resident blocks
 constexpr int DEP_TILE = 24;      // cells per side of a deposit + smoothing tile
 constexpr int CELL_MODE_MIN = 6;  // a push tile with a cell this crowded runs the cell mode
 constexpr int HEAVY_CELL = 64;    // a cell this crowded is pushed by k_push_heavy, spread over blocks
@@ -129,7 +130,7 @@ struct SceneCounters {
     int32_t substeps;         // electron substeps of the current tick
     int32_t rest_error;       // set when a k_rest wait ran past its limit (the tick's later substeps were dropped)
     int32_t charge_overflow;  // set when a node's fixed-point charge came near the int64 range in the smoothing
-    uint32_t pad[1];
+    int32_t dense_n;          // tiles listed in dense_tiles by the last sort
 };
 
 struct DevScene {
@@ -191,6 +192,7 @@ struct DevScene {
     int32_t *toff;            // per scan tile: offset of the tile
     int32_t *cstart;          // per cell + 1: absolute start in the sorted order
     uint8_t *tile_dense;      // per push tile: a cell holds >= CELL_MODE_MIN electrons
+    int32_t *dense_tiles;     // the tiles with tile_dense set (ctr->dense_n of them), in no particular order
     int32_t *heavy_cells;     // cells holding >= HEAVY_CELL electrons (ctr->heavy_n of them)
     // --- deposit
     long long *cur;           // [2 * cells]: cell-binned sum of v, units 2^-28 (x q_e)

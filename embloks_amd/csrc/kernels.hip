@@ -154,7 +154,10 @@ __global__ __launch_bounds__(256) void k_scan_top(Batch b) {
             }
             carry += chunk_total;
         }
-        if (threadIdx.x == 0) S.ctr->heavy_n = 0;
+        if (threadIdx.x == 0) {
+            S.ctr->heavy_n = 0;
+            S.ctr->dense_n = 0;
+        }
         __syncthreads();
     }
 }
@@ -209,15 +212,35 @@ __global__ __launch_bounds__(256) void k_canon(Batch b) {
         const int c = c0 + (int)threadIdx.x;
         const bool mine = c < c1;
         const int s0 = mine ? scan_start(S, c) : 0, s1 = mine ? scan_start(S, c + 1) : 0;
+        // A push tile with a crowded cell runs the cell mode (k_push_tiled);
+        // the others go electron by electron (k_push_sparse). The first cell to
+        // flag its tile lists it (the flag's byte set in its word by an atomic
+        // OR; k_scan cleared the words). Consecutive lanes hold consecutive
+        // rows of a column, so only the first crowded lane of each run of one
+        // tile sets the flag: one atomic per tile column, not per cell.
+        {
+            int tile = -1;
+            if (mine && s1 - s0 >= CELL_MODE_MIN) {
+                const int cc = c / S.rows, rr = c - cc * S.rows;
+                tile = (cc / PUSH_TILE) * ((S.rows + PUSH_TILE - 1) / PUSH_TILE) + rr / PUSH_TILE;
+            }
+            const int lane = threadIdx.x & 63;
+            const int prev = __shfl_up(tile, 1, 64);
+            bool first = false;
+            if (tile >= 0 && (lane == 0 || prev != tile)) {
+                const unsigned bit = 1u << (8 * (tile & 3));
+                first = (atomicOr(reinterpret_cast<unsigned *>(S.tile_dense + (tile & ~3)), bit) & bit) == 0;
+            }
+            // the wave's new tiles take consecutive list places: one counter atomic per wave
+            const unsigned long long m = __ballot(first);
+            int base = 0;
+            if (lane == 0 && m) base = atomicAdd(&S.ctr->dense_n, __popcll(m));
+            base = __shfl(base, 0, 64);
+            if (first) S.dense_tiles[base + __popcll(m & ((1ull << lane) - 1ull))] = tile;
+        }
         if (mine) {
             S.cstart[c] = s0;
             if (c == cells - 1) S.cstart[cells] = S.n;
-            // A push tile with a crowded cell runs the cell mode (k_push_tiled);
-            // the others go electron by electron (k_push_sparse).
-            if (s1 - s0 >= CELL_MODE_MIN) {
-                const int cc = c / S.rows, rr = c - cc * S.rows;
-                S.tile_dense[(cc / PUSH_TILE) * ((S.rows + PUSH_TILE - 1) / PUSH_TILE) + rr / PUSH_TILE] = 1;
-            }
             // Crowded cells get their own blocks (k_push_heavy); the list's order
             // does not matter, each cell's result depends on the cell alone.
             if (s1 - s0 >= HEAVY_CELL) S.heavy_cells[atomicAdd(&S.ctr->heavy_n, 1)] = c;
@@ -1121,12 +1144,17 @@ __device__ __forceinline__ void push_tiled_body(Batch b, int k, float2 *force_ou
         const float h = S.dt / (float)sub;
         const int local = block_in_scene(b, KC_PUSH, s, blk);
         if (!FORCE && k == 0 && threadIdx.x == 0 && local == 0) S.ctr->substeps = sub;
-        if (!S.close_on || !S.tile_dense[local]) continue;
-        if (!table_loaded) {   // the pair table is shared by every scene
-            for (int q = threadIdx.x; q < PAIR_SAMPLES + 4; q += THREADS) L.gt[q] = S.g_lerp[q];
-            table_loaded = true;
+        if (!S.close_on) continue;
+        // The scene's blocks share out the sort's list of dense tiles.
+        const int *pre = b.prefix + KC_PUSH * (b.n_scenes + 1);
+        const int nblk = pre[s + 1] - pre[s], dense_n = S.ctr->dense_n;
+        for (int q = local; q < dense_n; q += nblk) {
+            if (!table_loaded) {   // the pair table is shared by every scene
+                for (int j = threadIdx.x; j < PAIR_SAMPLES + 4; j += THREADS) L.gt[j] = S.g_lerp[j];
+                table_loaded = true;
+            }
+            push_tile<MODE, THREADS, false>(S, L, S.dense_tiles[q], k, h, t, force_out);
         }
-        push_tile<MODE, THREADS, false>(S, L, local, k, h, t, force_out);
     }
 }
 

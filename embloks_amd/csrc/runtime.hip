@@ -102,7 +102,9 @@ void layout(Bump &a, const Counts &c, DevScene &d, DevScene **d_scene, int32_t *
     d.lstart = a.take<int32_t>(cells);
     d.toff = a.take<int32_t>((cells + SCAN_TILE - 1) / SCAN_TILE);
     d.cstart = a.take<int32_t>(cells + 1);
-    d.tile_dense = a.take<uint8_t>((size_t)((c.cols + PUSH_TILE - 1) / PUSH_TILE) * ((c.rows + PUSH_TILE - 1) / PUSH_TILE));
+    const size_t ptiles = (size_t)((c.cols + PUSH_TILE - 1) / PUSH_TILE) * ((c.rows + PUSH_TILE - 1) / PUSH_TILE);
+    d.tile_dense = a.take<uint8_t>(ptiles);
+    d.dense_tiles = a.take<int32_t>(ptiles);
     d.heavy_cells = a.take<int32_t>(n / HEAVY_CELL + 1);
     d.cur = a.take<long long>(2 * cells);
     d.cur_smooth = a.take<float>(2 * cells);
@@ -329,7 +331,9 @@ void scene_blocks(const DevScene &d, int out[KC_COUNT]) {
     out[KC_NODES] = cdiv((long long)(d.nx + 1) * (d.ny + 1), 256);
     out[KC_STACKS] = std::max(1, d.n_stacks) * TRIM_BLOCKS;
     out[KC_SCENE] = 1;
-    out[KC_PUSH] = cdiv(d.cols, PUSH_TILE) * cdiv(d.rows, PUSH_TILE);
+    // (the tiled push walks the sort's list of dense tiles: blocks enough for
+    // four rounds of the chip's resident blocks, never more than tiles)
+    out[KC_PUSH] = std::min(PUSH_BLOCKS_MAX, cdiv(d.cols, PUSH_TILE) * cdiv(d.rows, PUSH_TILE));
     out[KC_HEAVY] = (int)std::min<long long>(256, std::max<long long>(1, d.n / 16384));
     out[KC_REST] = (int)std::min<long long>(256, std::max<long long>(1, cdiv(d.n, 1024)));
 }
