@@ -104,7 +104,7 @@ def run_single(args):
 
 
 # Kernels that carry each tick phase (the rocprofv3 names without the namespace).
-PHASE_KERNELS = {"trim": ["k_trim"], "push": ["k_push_tiled", "k_push_sparse<false>", "k_push_heavy<false>"], "rest": ["k_rest"],
+PHASE_KERNELS = {"trim": ["k_trim", "k_trim_gain"], "push": ["k_push_tiled", "k_push_sparse<false>", "k_push_heavy<false>"], "rest": ["k_rest"],
                  "count": ["k_count"], "scan": ["k_scan", "k_scan_top"], "scatter": ["k_scatter"], "canon": ["k_canon"],
                  "deposit_smooth": ["k_deposit_smooth"], "fdtd": ["k_fdtd"], "correct": ["k_correct"],
                  "finish": ["k_finish"],

@@ -218,7 +218,6 @@ struct DevScene {
     int32_t *has_gain;        // per stack
     float *trim_part;         // per stack and trim block: terminal potential sums (low, high)
     float *stack_read;        // per stack: the last trim's v_low, v_high, and 1 if it trimmed (else 0)
-    int32_t *trim_done;       // per stack: trim blocks finished this tick (back to 0 by the last)
     int32_t *rest_sync;       // [rest_sync_len(max_substeps)]: k_rest's work ticket, then items done per
                               // (substep, phase); zeroed by k_trim at the start of every tick
     const float *phase_tab;   // [TICK_TABLE][n_ac]: frac(f t) per AC cell and tick

@@ -332,7 +332,6 @@ constexpr int TRIM_THREADS = 512;
 
 __global__ __launch_bounds__(TRIM_THREADS) void k_trim(Batch b) {
     __shared__ float part[2][TRIM_THREADS / 64];
-    __shared__ bool last;
     for (int blk = blockIdx.x; blk < b.total[KC_STACKS]; blk += gridDim.x) {
         const int s = scene_of(b, KC_STACKS, blk);
         const DevScene &S = b.scenes[s];
@@ -408,44 +407,53 @@ __global__ __launch_bounds__(TRIM_THREADS) void k_trim(Batch b) {
                 hi += part[1][w];
             }
             float *mine = S.trim_part + 2 * ((size_t)st * TRIM_BLOCKS + tb);
-            __hip_atomic_store(mine, lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(mine + 1, hi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            // The block that finishes the stack's last part adds the parts in
-            // block order and updates the gain; the release/acquire pair on the
-            // counter makes every part's store visible to it.
-            const int done = __hip_atomic_fetch_add(S.trim_done + st, 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-            last = done == TRIM_BLOCKS - 1;
+            mine[0] = lo;
+            mine[1] = hi;
         }
         __syncthreads();
-        if (last && threadIdx.x == 0) {
-            float lo = 0.0f, hi = 0.0f;
-            const float *parts = S.trim_part + 2 * (size_t)st * TRIM_BLOCKS;
-            for (int q = 0; q < TRIM_BLOCKS; q++) {
-                lo += __hip_atomic_load(parts + 2 * q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                hi += __hip_atomic_load(parts + 2 * q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            }
-            __hip_atomic_store(S.trim_done + st, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            const float v_low = lo / (float)I.low_n, v_high = hi / (float)I.high_n;
-            const int cells = S.cols * S.rows;
-            float isum = 0.0f;
-            for (int k = 0; k < I.cur_n; k++) {
-                const int cell = S.stack_cells[I.cur_off + k];
-                const float ix = (float)S.cur[cell] * S.i_from_fixed;
-                const float iy = (float)S.cur[cells + cell] * S.i_from_fixed;
-                isum += ix * I.dir_x + iy * I.dir_y;
-            }
-            const float current = isum / (float)I.cur_n;
-            const float target = S.target_tab[(size_t)(t % TICK_TABLE) * S.n_stacks + st];
-            const float delivered = v_high - v_low + current * I.internal_resistance;
-            float gain = S.has_gain[st] ? S.gains[st] : 1.0f;
-            gain += S.trim_rate * (target - delivered) * target / (I.amplitude * I.amplitude);
-            S.gains[st] = fminf(fmaxf(gain, S.gain_low), S.gain_high);
-            S.has_gain[st] = 1;
-            S.stack_read[3 * st] = v_low;
-            S.stack_read[3 * st + 1] = v_high;
-            S.stack_read[3 * st + 2] = 1.0f;
+    }
+}
+
+// The stack's gain from its blocks' parts (k_trim, the kernel before), added in
+// block order, so the result does not depend on which block ended last. A
+// kernel of its own rather than a last-block handshake: the release/acquire of
+// a device-scope counter writes back and invalidates the L2 of the block's XCD,
+// once per block of k_trim.
+__global__ __launch_bounds__(64) void k_trim_gain(Batch b) {
+    for (int blk = blockIdx.x; blk < b.total[KC_STACKS]; blk += gridDim.x) {
+        const int s = scene_of(b, KC_STACKS, blk);
+        const DevScene &S = b.scenes[s];
+        const int local = block_in_scene(b, KC_STACKS, s, blk);
+        const int st = local / TRIM_BLOCKS, tb = local - st * TRIM_BLOCKS;
+        if (tb != 0 || st >= S.n_stacks || threadIdx.x != 0) continue;
+        const StackInfo I = S.stacks[st];
+        if (!I.both_wired || I.amplitude == 0.0f) continue;   // k_trim set the stack's readings
+        const uint32_t t = S.ctr->tick;
+        float lo = 0.0f, hi = 0.0f;
+        const float *parts = S.trim_part + 2 * (size_t)st * TRIM_BLOCKS;
+        for (int q = 0; q < TRIM_BLOCKS; q++) {
+            lo += parts[2 * q];
+            hi += parts[2 * q + 1];
         }
-        __syncthreads();
+        const float v_low = lo / (float)I.low_n, v_high = hi / (float)I.high_n;
+        const int cells = S.cols * S.rows;
+        float isum = 0.0f;
+        for (int k = 0; k < I.cur_n; k++) {
+            const int cell = S.stack_cells[I.cur_off + k];
+            const float ix = (float)S.cur[cell] * S.i_from_fixed;
+            const float iy = (float)S.cur[cells + cell] * S.i_from_fixed;
+            isum += ix * I.dir_x + iy * I.dir_y;
+        }
+        const float current = isum / (float)I.cur_n;
+        const float target = S.target_tab[(size_t)(t % TICK_TABLE) * S.n_stacks + st];
+        const float delivered = v_high - v_low + current * I.internal_resistance;
+        float gain = S.has_gain[st] ? S.gains[st] : 1.0f;
+        gain += S.trim_rate * (target - delivered) * target / (I.amplitude * I.amplitude);
+        S.gains[st] = fminf(fmaxf(gain, S.gain_low), S.gain_high);
+        S.has_gain[st] = 1;
+        S.stack_read[3 * st] = v_low;
+        S.stack_read[3 * st + 1] = v_high;
+        S.stack_read[3 * st + 2] = 1.0f;
     }
 }
 
@@ -2073,6 +2081,7 @@ hipError_t finish_sync_kernel(const Batch &b, hipStream_t stream) {
 hipError_t tick_kernels(const Batch &b, int fdtd_substeps, hipStream_t stream, hipEvent_t *marks) {
     EMB_MARK(0);
     EMB_LAUNCH(k_trim, KC_STACKS, TRIM_THREADS, b);
+    EMB_LAUNCH(k_trim_gain, KC_STACKS, 64, b);
     EMB_MARK(1);
     EMB_LAUNCH(k_push_tiled, KC_PUSH, PUSH_THREADS, b, 0);
     EMB_LAUNCH(k_push_sparse<false>, KC_ELECTRONS, 256, b, 0, nullptr);

@@ -119,7 +119,6 @@ void layout(Bump &a, const Counts &c, DevScene &d, DevScene **d_scene, int32_t *
     d.gains = a.take<float>(c.n_stacks);
     d.has_gain = a.take<int32_t>(c.n_stacks);
     d.trim_part = a.take<float>(2 * (size_t)c.n_stacks * TRIM_BLOCKS);
-    d.trim_done = a.take<int32_t>(c.n_stacks);
     d.stack_read = a.take<float>(3 * (size_t)c.n_stacks);
     d.rest_sync = a.take<int32_t>(rest_sync_len(c.max_substeps));
     *tab = a.take<float>((size_t)TICK_TABLE * (c.n_ac + c.n_stacks));
