@@ -235,6 +235,7 @@ enum KernelClass {
     KC_SMOOTH,          // 32 x 32 tiles of the smoothing region
     KC_FDTD,            // FDTD_WAVES strips x fdtd_rows() rows of the (nx+1) x (ny+1) corner space
     KC_GEMM,            // 2 channels x 64 x 64 output tiles of nx x ny
+    KC_GEMM_SMALL,      // the same with small tiles, for scenes with few 64 x 64 tiles (dst_small)
     KC_NODES,           // 256 corners of (nx+1) x (ny+1) per block
     KC_STACKS,          // TRIM_BLOCKS blocks per battery stack (at least one stack per scene)
     KC_SCENE,           // one block per scene

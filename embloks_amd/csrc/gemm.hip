@@ -28,8 +28,9 @@
 //     its error is measured against FP32 and FP64. The operands are split (and
 //     transposed where the contraction runs over rows) by an O(n^2) pass before
 //     each product, so the O(n^3) kernel only loads bf16 and multiplies.
-// Both use 256 threads as a 2 x 2 grid of waves, operands staged through LDS
-// in K slices with the next slice prefetched into registers.
+// Both stage operands through LDS in K slices with the next slices prefetched
+// into registers; the tick's products of small scenes run 32 x 32 tiles on two
+// waves, the others 64 x 64 tiles (or larger) on a 2 x 2 grid of waves.
 
 #include "device.hpp"
 #include "gemm.hpp"
@@ -42,6 +43,15 @@ namespace emb {
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __attribute__((address_space(1))) const float gfloat;
+typedef __attribute__((address_space(1))) const floatx4 gfloatx4;
+
+// Operand loads through global-memory pointers: global_load instructions, which
+// count in vmcnt alone. Through a generic pointer they are flat_loads, which
+// also count in lgkmcnt, so the wait before the next LDS read of the slice being
+// multiplied would also wait for every prefetch still in flight.
+__device__ __forceinline__ floatx4 gload4(const float *p) { return *(gfloatx4 *)p; }
+__device__ __forceinline__ float gload1(const float *p) { return *(gfloat *)p; }
 
 // f(integral_constant<int, 0>) ... f(integral_constant<int, N - 1>), so that an
 // index into a register array stays a compile-time constant.
@@ -141,13 +151,13 @@ __device__ __forceinline__ void tile_f32(const GemmArgs &g, int m0, int n0, floa
 #pragma unroll
         for (int v = 0; v < AV; v++) {
             if (FAST) {
-                ra[slot][v] = *reinterpret_cast<const floatx4 *>(pa[v] + k0);
+                ra[slot][v] = gload4(pa[v] + k0);
                 if (FOLD == 2) {
                     if (MVEC) {
-                        ra2[slot][v] = *reinterpret_cast<const floatx4 *>(pa2[v] - k0);
+                        ra2[slot][v] = gload4(pa2[v] - k0);
                     } else {
                         const float *m = pa2[v] - k0;
-                        ra2[slot][v] = floatx4{m[0], m[1], m[2], m[3]};
+                        ra2[slot][v] = floatx4{gload1(m), gload1(m + 1), gload1(m + 2), gload1(m + 3)};
                     }
                 }
             } else {
@@ -169,8 +179,8 @@ __device__ __forceinline__ void tile_f32(const GemmArgs &g, int m0, int n0, floa
 #pragma unroll
         for (int v = 0; v < BV; v++) {
             if (FAST) {
-                rb[slot][v] = *reinterpret_cast<const floatx4 *>(pb[v] + (size_t)k0 * g.ldb);
-                if (FOLD == 1) rb2[slot][v] = *reinterpret_cast<const floatx4 *>(pb2[v] - (size_t)k0 * g.ldb);
+                rb[slot][v] = gload4(pb[v] + (size_t)k0 * g.ldb);
+                if (FOLD == 1) rb2[slot][v] = gload4(pb2[v] - (size_t)k0 * g.ldb);
             } else {
                 const int gk = k0 + rb_k[v], gc = n0 + rb_col[v];
                 floatx4 r = gk < g.K ? vec4(load4(pb[v] + (size_t)k0 * g.ldb, g.N - gc, bvec))
@@ -216,19 +226,23 @@ __device__ __forceinline__ void tile_f32(const GemmArgs &g, int m0, int n0, floa
     };
     auto multiply = [&](int buf) {
         const float *a = As + buf * GK * LA, *b = Bs + buf * GK * LB;
+        // every fragment of the slice read before the first MFMA: one wait for
+        // the LDS, not one per step of 4
+        float af[GK / 4][TM], bf[GK / 4][TN];
 #pragma unroll
-        for (int kk = 0; kk < GK; kk += 4) {
-            float af[TM], bf[TN];
+        for (int q = 0; q < GK / 4; q++) {
 #pragma unroll
-            for (int x = 0; x < TM; x++) af[x] = a[(kk + kq) * LA + wm + x * 16 + l16];
+            for (int x = 0; x < TM; x++) af[q][x] = a[(4 * q + kq) * LA + wm + x * 16 + l16];
 #pragma unroll
-            for (int y = 0; y < TN; y++) bf[y] = b[(kk + kq) * LB + wn + y * 16 + l16];
+            for (int y = 0; y < TN; y++) bf[q][y] = b[(4 * q + kq) * LB + wn + y * 16 + l16];
+        }
+#pragma unroll
+        for (int q = 0; q < GK / 4; q++)
 #pragma unroll
             for (int x = 0; x < TM; x++)
 #pragma unroll
                 for (int y = 0; y < TN; y++)
-                    acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[x], bf[y], acc[x][y], 0, 0, 0);
-        }
+                    acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[q][x], bf[q][y], acc[x][y], 0, 0, 0);
     };
     // Slices past the last are clamped to it, so every load and store runs
     // unconditionally in the unrolled body (a repeated slice is never multiplied).
@@ -302,84 +316,108 @@ __host__ __device__ inline int gemm_fast(const GemmArgs &g, int gk) {
 
 constexpr int TICK_BM = 64, TICK_BN = 64;
 
-// Output tiles of the even and odd halves of one stage's product.
-__host__ __device__ inline void dst_half_tiles(bool left, int nx, int ny, int &te, int &to) {
+// Output tiles of the even and odd halves of one stage's product, BM x BN each.
+__host__ __device__ inline void dst_half_tiles(bool left, int nx, int ny, int bm, int bn, int &te, int &to) {
     auto cdiv = [](int a, int b) { return (a + b - 1) / b; };
     if (left) {   // rows of Sx: ceil(nx/2) even, floor(nx/2) odd
-        te = cdiv((nx + 1) / 2, TICK_BM) * cdiv(ny, TICK_BN);
-        to = cdiv(nx / 2, TICK_BM) * cdiv(ny, TICK_BN);
+        te = cdiv((nx + 1) / 2, bm) * cdiv(ny, bn);
+        to = cdiv(nx / 2, bm) * cdiv(ny, bn);
     } else {      // columns of Sy
-        te = cdiv(nx, TICK_BM) * cdiv((ny + 1) / 2, TICK_BN);
-        to = cdiv(nx, TICK_BM) * cdiv(ny / 2, TICK_BN);
+        te = cdiv(nx, bm) * cdiv((ny + 1) / 2, bn);
+        to = cdiv(nx, bm) * cdiv(ny / 2, bn);
     }
 }
 
-int dst_gemm_blocks(int nx, int ny) {
+__host__ __device__ inline int dst_blocks(int nx, int ny, int bm, int bn) {
     int le, lo, re, ro;
-    dst_half_tiles(true, nx, ny, le, lo);
-    dst_half_tiles(false, nx, ny, re, ro);
-    return 2 * std::max(le + lo, re + ro);
+    dst_half_tiles(true, nx, ny, bm, bn, le, lo);
+    dst_half_tiles(false, nx, ny, bm, bn, re, ro);
+    return 2 * (le + lo > re + ro ? le + lo : re + ro);
+}
+
+// Scenes with at most DST_SPLIT_TILES output tiles of 64 x 64 per stage (both
+// channels) do not fill the chip's 256 CUs with them: their products run
+// SMALL_BM x SMALL_BN tiles on blocks of SMALL_GM x SMALL_GN waves instead
+// (KC_GEMM_KS). At 544 nodes the four products took 51 us against 54.5 us with
+// 64 x 64 tiles; 32 x 64 tiles (69 us) and 64 x 64 tiles with K split over the
+// block's four waves and the partial tiles summed in LDS (72 us) were slower.
+constexpr int DST_SPLIT_TILES = 512;
+constexpr int SMALL_BM = 32, SMALL_BN = 32, SMALL_GM = 2, SMALL_GN = 1;
+
+bool dst_small(int nx, int ny) { return dst_blocks(nx, ny, TICK_BM, TICK_BN) <= DST_SPLIT_TILES; }
+
+int dst_gemm_blocks(int nx, int ny) {
+    return dst_small(nx, ny) ? dst_blocks(nx, ny, SMALL_BM, SMALL_BN) : dst_blocks(nx, ny, TICK_BM, TICK_BN);
+}
+
+// Block `local` of a scene's stage STAGE: its product's arguments and output
+// tile origin, or false when the block has nothing to do.
+template <int STAGE, int BM, int BN>
+__device__ __forceinline__ bool dst_tile(const DevScene &S, int local, int chans, GemmArgs &g, int &m0, int &n0) {
+    constexpr bool LEFT = (STAGE == 0 || STAGE == 2);
+    const int nx = S.nx, ny = S.ny;
+    int te, to;
+    dst_half_tiles(LEFT, nx, ny, BM, BN, te, to);
+    const int ch = local / (te + to);
+    if (ch >= 2) return false;   // the scene's block count covers the larger stage
+    // Channel 0 (the charge's potential) only where it is wanted: every
+    // tick in a scene with battery stacks (the trim reads it), otherwise on
+    // demand (DST_POT) when the potential is read.
+    if (chans == DST_POT ? ch != 0 : (chans == DST_TICK && ch == 0 && S.n_stacks == 0)) return false;
+    const int rem0 = local - ch * (te + to);
+    const int odd = rem0 >= te ? 1 : 0;
+    const int rem = rem0 - odd * te;
+    // The Poisson arrays' rows are padded to pny (a multiple of 4) and the
+    // sine halves to whole slices, so every scene takes the FAST tiles.
+    const int pny = S.pny;
+    const size_t pnodes = (size_t)nx * pny;
+    const float *in = (LEFT ? S.w1 : S.w2) + ch * pnodes;
+    float *out = (STAGE == 3 ? S.phi : (STAGE == 1 ? S.w1 : S.w2)) + ch * pnodes;
+    g = GemmArgs{};
+    g.C = out;
+    g.ldc = pny;
+    g.scale = STAGE == 1 ? S.inv_lap : nullptr;
+    g.odd = odd;
+    if (LEFT) {   // out = Sx . in: rows 2 i + odd from the half of Sx and the folded rows of in
+        const int mh = odd ? nx / 2 : (nx + 1) / 2;
+        g.A = odd ? S.sx_o : S.sx_e;
+        g.lda = sine_half_pitch(mh);
+        g.B = in;
+        g.ldb = pny;
+        g.M = mh;
+        g.N = ny;
+        g.Np = pny;
+        g.K = fold_k(mh, nx);
+        g.fold = 1;
+        g.fold_n = nx;
+    } else {      // out = in . Sy: columns 2 j + odd from the folded columns of in
+        const int mh = odd ? ny / 2 : (ny + 1) / 2;
+        g.A = in;
+        g.lda = pny;
+        g.B = odd ? S.sy_o : S.sy_et;
+        g.ldb = sine_half_pitch(mh);
+        g.M = nx;
+        g.N = mh;
+        g.Np = sine_half_pitch(mh);
+        g.K = fold_k(mh, ny);
+        g.fold = 2;
+        g.fold_n = ny;
+    }
+    const int tn = (g.N + BN - 1) / BN;
+    m0 = (rem / tn) * BM;
+    n0 = (rem % tn) * BN;
+    return true;
 }
 
 template <int STAGE>
 __global__ __launch_bounds__(256) void k_dst_gemm(Batch b, int chans) {
-    constexpr bool LEFT = (STAGE == 0 || STAGE == 2);
+    constexpr int F = (STAGE == 0 || STAGE == 2) ? 1 : 2;
     __shared__ __align__(16) float lds[2 * 16 * (TICK_BM + 4 + TICK_BN + 4)];
     for (int blk = blockIdx.x; blk < b.total[KC_GEMM]; blk += gridDim.x) {
         const int s = scene_of(b, KC_GEMM, blk);
-        const DevScene &S = b.scenes[s];
-        const int nx = S.nx, ny = S.ny;
-        int te, to;
-        dst_half_tiles(LEFT, nx, ny, te, to);
-        const int local = block_in_scene(b, KC_GEMM, s, blk);
-        const int ch = local / (te + to);
-        if (ch >= 2) continue;   // the scene's block count covers the larger stage
-        // Channel 0 (the charge's potential) only where it is wanted: every
-        // tick in a scene with battery stacks (the trim reads it), otherwise on
-        // demand (DST_POT) when the potential is read.
-        if (chans == DST_POT ? ch != 0 : (chans == DST_TICK && ch == 0 && S.n_stacks == 0)) continue;
-        const int rem0 = local - ch * (te + to);
-        const int odd = rem0 >= te ? 1 : 0;
-        const int rem = rem0 - odd * te;
-        // The Poisson arrays' rows are padded to pny (a multiple of 4) and the
-        // sine halves to whole slices, so every scene takes the FAST tiles.
-        const int pny = S.pny;
-        const size_t pnodes = (size_t)nx * pny;
-        const float *in = (LEFT ? S.w1 : S.w2) + ch * pnodes;
-        float *out = (STAGE == 3 ? S.phi : (STAGE == 1 ? S.w1 : S.w2)) + ch * pnodes;
         GemmArgs g;
-        g.C = out;
-        g.ldc = pny;
-        g.scale = STAGE == 1 ? S.inv_lap : nullptr;
-        g.odd = odd;
-        if (LEFT) {   // out = Sx . in: rows 2 i + odd from the half of Sx and the folded rows of in
-            const int mh = odd ? nx / 2 : (nx + 1) / 2;
-            g.A = odd ? S.sx_o : S.sx_e;
-            g.lda = sine_half_pitch(mh);
-            g.B = in;
-            g.ldb = pny;
-            g.M = mh;
-            g.N = ny;
-            g.Np = pny;
-            g.K = fold_k(mh, nx);
-            g.fold = 1;
-            g.fold_n = nx;
-        } else {      // out = in . Sy: columns 2 j + odd from the folded columns of in
-            const int mh = odd ? ny / 2 : (ny + 1) / 2;
-            g.A = in;
-            g.lda = pny;
-            g.B = odd ? S.sy_o : S.sy_et;
-            g.ldb = sine_half_pitch(mh);
-            g.M = nx;
-            g.N = mh;
-            g.Np = sine_half_pitch(mh);
-            g.K = fold_k(mh, ny);
-            g.fold = 2;
-            g.fold_n = ny;
-        }
-        const int tn = (g.N + TICK_BN - 1) / TICK_BN;
-        const int m0 = (rem / tn) * TICK_BM, n0 = (rem % tn) * TICK_BN;
-        constexpr int F = LEFT ? 1 : 2;
+        int m0, n0;
+        if (!dst_tile<STAGE, TICK_BM, TICK_BN>(b.scenes[s], block_in_scene(b, KC_GEMM, s, blk), chans, g, m0, n0)) continue;
         const int fast = gemm_fast(g, 16);
         if (fast == 1) tile_f32<TICK_BM, TICK_BN, 2, 2, 16, 4, true, F>(g, m0, n0, lds);
         else if (F == 2 && fast == 2) tile_f32<TICK_BM, TICK_BN, 2, 2, 16, 4, true, F, false>(g, m0, n0, lds);
@@ -388,20 +426,51 @@ __global__ __launch_bounds__(256) void k_dst_gemm(Batch b, int chans) {
     }
 }
 
+// The scenes of dst_small: SMALL_BM x SMALL_BN tiles.
+constexpr int SMALL_THREADS = 64 * SMALL_GM * SMALL_GN;
+template <int STAGE>
+__global__ __launch_bounds__(SMALL_THREADS) void k_dst_gemm_small(Batch b, int chans) {
+    constexpr int F = (STAGE == 0 || STAGE == 2) ? 1 : 2;
+    constexpr int GM = SMALL_GM, GN = SMALL_GN;
+    __shared__ __align__(16) float lds[2 * 16 * (SMALL_BM + 4 + SMALL_BN + 4)];
+    for (int blk = blockIdx.x; blk < b.total[KC_GEMM_SMALL]; blk += gridDim.x) {
+        const int s = scene_of(b, KC_GEMM_SMALL, blk);
+        GemmArgs g;
+        int m0, n0;
+        if (!dst_tile<STAGE, SMALL_BM, SMALL_BN>(b.scenes[s], block_in_scene(b, KC_GEMM_SMALL, s, blk), chans, g, m0,
+                                                 n0))
+            continue;
+        const int fast = gemm_fast(g, 16);
+        if (fast == 1) tile_f32<SMALL_BM, SMALL_BN, GM, GN, 16, 4, true, F>(g, m0, n0, lds);
+        else if (F == 2 && fast == 2) tile_f32<SMALL_BM, SMALL_BN, GM, GN, 16, 4, true, F, false>(g, m0, n0, lds);
+        else tile_f32<SMALL_BM, SMALL_BN, GM, GN, 16, 4, false, F>(g, m0, n0, lds);
+        __syncthreads();
+    }
+}
+
 template __global__ void k_dst_gemm<0>(Batch, int);
 template __global__ void k_dst_gemm<1>(Batch, int);
 template __global__ void k_dst_gemm<2>(Batch, int);
 template __global__ void k_dst_gemm<3>(Batch, int);
+template __global__ void k_dst_gemm_small<0>(Batch, int);
+template __global__ void k_dst_gemm_small<1>(Batch, int);
+template __global__ void k_dst_gemm_small<2>(Batch, int);
+template __global__ void k_dst_gemm_small<3>(Batch, int);
 
-// The four products of one Poisson solve for every scene of the batch.
+// The four products of one Poisson solve for every scene of the batch: each
+// stage's 64 x 64 tiles and small tiles in a launch of their own (either may be
+// empty), in stage order.
 hipError_t launch_dst_gemms(const Batch &b, hipStream_t stream, int chans) {
-    const int total = b.total[KC_GEMM];
-    if (total == 0) return hipSuccess;
-    const int grid = total < 8192 ? total : 8192;
-    k_dst_gemm<0><<<grid, 256, 0, stream>>>(b, chans);
-    k_dst_gemm<1><<<grid, 256, 0, stream>>>(b, chans);
-    k_dst_gemm<2><<<grid, 256, 0, stream>>>(b, chans);
-    k_dst_gemm<3><<<grid, 256, 0, stream>>>(b, chans);
+    const int whole = std::min(b.total[KC_GEMM], 8192), split = std::min(b.total[KC_GEMM_SMALL], 8192);
+    if (whole + split == 0) return hipSuccess;
+#define EMB_DST_STAGE(st)                                                                         \
+    if (whole) k_dst_gemm<st><<<whole, 256, 0, stream>>>(b, chans);                               \
+    if (split) k_dst_gemm_small<st><<<split, SMALL_THREADS, 0, stream>>>(b, chans);
+    EMB_DST_STAGE(0)
+    EMB_DST_STAGE(1)
+    EMB_DST_STAGE(2)
+    EMB_DST_STAGE(3)
+#undef EMB_DST_STAGE
     return hipGetLastError();
 }
 
@@ -540,13 +609,13 @@ __global__ __launch_bounds__(256) void k_gemm_nt_bf16x3(NtArgs g) {
         auto load = [&](auto slot, int k0) {
 #pragma unroll
             for (int v = 0; v < CA; v++) {
-                rah[slot][v] = *reinterpret_cast<const u32x4 *>(g.ah + offa[v] + k0);
-                ral[slot][v] = *reinterpret_cast<const u32x4 *>(g.al + offa[v] + k0);
+                rah[slot][v] = *(__attribute__((address_space(1))) const u32x4 *)(g.ah + offa[v] + k0);
+                ral[slot][v] = *(__attribute__((address_space(1))) const u32x4 *)(g.al + offa[v] + k0);
             }
 #pragma unroll
             for (int v = 0; v < CB; v++) {
-                rbh[slot][v] = *reinterpret_cast<const u32x4 *>(g.bh + offb[v] + k0);
-                rbl[slot][v] = *reinterpret_cast<const u32x4 *>(g.bl + offb[v] + k0);
+                rbh[slot][v] = *(__attribute__((address_space(1))) const u32x4 *)(g.bh + offb[v] + k0);
+                rbl[slot][v] = *(__attribute__((address_space(1))) const u32x4 *)(g.bl + offb[v] + k0);
             }
         };
         auto store = [&](auto slot, int buf) {
