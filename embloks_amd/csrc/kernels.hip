@@ -518,27 +518,6 @@ __device__ __forceinline__ int2 fixed_pos(int dc, int dr, float fx, float fy) {
     return make_int2(dc * (1 << 24) + fix24(fx), dr * (1 << 24) + fix24(fy));
 }
 
-// Neighbour access of the per-electron close-range loop: the whole sorted
-// snapshot in global memory (positions relative to their own cell), or a
-// window of it staged in LDS (positions relative to the window's corner cell).
-// `origin` gives the electron's position in the frame of neighbour cell
-// (cc, rr)'s positions.
-struct GlobalNeighbours {
-    const DevScene &S;
-    __device__ void range(int cc, int rr, int &b, int &e) const {
-        const int cell = cc * S.rows + rr;
-        b = S.cstart[cell];
-        e = S.cstart[cell + 1];
-    }
-    __device__ int2 p(int j) const {
-        const float2 f = snap_pos(S, j);
-        return fixed_pos(0, 0, f.x, f.y);
-    }
-    __device__ int2 origin(int hc, int hr, float px, float py, int cc, int rr) const {
-        return fixed_pos(hc - cc, hr - rr, px, py);
-    }
-};
-
 // Lorentz force of the gathered fields plus the battery's impressed field, as an
 // acceleration (physics.rs:49-51, 139-149; battery.rs:210-228).
 __device__ __forceinline__ void field_accel(const DevScene &S, int hc, int hr, float px, float py, float vx, float vy,
@@ -679,10 +658,47 @@ __device__ __forceinline__ void finish(const DevScene &S, int id, int hc, int hr
     S.vy[id] = nvy;
 }
 
+// CloseRange::force (close_range.rs:314-376) on one electron at fractions
+// (px, py) of its home cell (hc, hr), every pair seen from both sides, one
+// thread per electron over the sorted snapshot. A window column's cells are
+// contiguous in the sorted order, so the column's electrons are walked as one
+// run (two cell starts per column, not two per cell; a neighbour's row comes
+// with it): a pair at or past the cutoff reads a zero table entry and adds
+// exactly nothing, so the sum is the one over the cells within reach. The ion
+// terms of the column's conductive cells are summed before its pairs, in
+// separate accumulators, in cell order.
+__device__ __forceinline__ float2 sparse_close_force(const DevScene &S, const lerp2 *gt, int hc, int hr, float px,
+                                                     float py, bool uniform, int home) {
+    float fx = 0.0f, fy = 0.0f;
+    const int R = S.reach;
+    const int c_lo = max(hc - R, 0), c_hi = min(hc + R, S.cols - 1);
+    const int r_lo = max(hr - R, 0), r_hi = min(hr + R, S.rows - 1);
+    if (uniform) ion_uniform(S, home, px, py, fx, fy);
+    const IonWeights iw = ion_weights(S, px, py);
+    float sx = 0.0f, sy = 0.0f;
+    const int ox = fix24(px), oy = fix24(py);
+    for (int cc = c_lo; cc <= c_hi; cc++) {
+        const int base = cc * S.rows;
+        if (!uniform) {
+            for (int rr = r_lo; rr <= r_hi; rr++) {
+                const float dens = S.density[base + rr];
+                if (dens > 0.0f) ion_cell(S, iw, hc - cc, hr - rr, dens, fx, fy);
+            }
+        }
+        const int jb = S.cstart[base + r_lo], je = S.cstart[base + r_hi + 1];
+        const int dx = (hc - cc) * (1 << 24) + ox;
+        for (int j = jb; j < je; j++) {
+            const int rr = S.scell[j] - base;
+            const float2 f = snap_pos(S, j);
+            pair_term(gt, S.pair_s2, dx - fix24(f.x), (hr - rr) * (1 << 24) + oy - fix24(f.y), sx, sy);
+        }
+    }
+    return make_float2(fx + S.k_pair_fx * sx, fy + S.k_pair_fx * sy);
+}
+
 // One electron's substep (physics.rs:139-158), one thread per electron: `slot`
 // is its place in the sorted snapshot.
-template <class NB>
-__device__ void push_core(const DevScene &S, const NB &nb, const lerp2 *gt, int slot, int k, float h, uint32_t t) {
+__device__ void push_core(const DevScene &S, const lerp2 *gt, int slot, int k, float h, uint32_t t) {
     const int id = S.order[slot];
     const int home = S.scell[slot];
     const int hc = home / S.rows, hr = home - (home / S.rows) * S.rows;
@@ -693,41 +709,9 @@ __device__ void push_core(const DevScene &S, const NB &nb, const lerp2 *gt, int 
     float ax, ay;
     field_accel(S, hc, hr, px, py, vx, vy, info.flags, k, h, t, ax, ay);
     if (S.close_on) {
-        // CloseRange::force (close_range.rs:314-376), every pair seen from both sides.
-        float fx = 0.0f, fy = 0.0f;
-        const int R = S.reach;
-        const int c_lo = max(hc - R, 0), c_hi = min(hc + R, S.cols - 1);
-        const int r_lo = max(hr - R, 0), r_hi = min(hr + R, S.rows - 1);
-        const bool uniform = (info.flags & F_ION_UNIFORM) != 0u;
-        if (uniform) ion_uniform(S, home, px, py, fx, fy);
-        const IonWeights iw = ion_weights(S, px, py);
-        // Neighbour cells entirely beyond the cutoff from this electron are skipped.
-        float sx = 0.0f, sy = 0.0f;
-        const float cut2 = S.cutoff2;
-        for (int cc = c_lo; cc <= c_hi; cc++) {
-            const float ox = (float)(hc - cc);
-            const float gapx = ox > 0.0f ? ox + px - 1.0f : (ox < 0.0f ? -ox - px : 0.0f);
-            for (int rr = r_lo; rr <= r_hi; rr++) {
-                const float oy = (float)(hr - rr);
-                if (!uniform) {
-                    const float dens = S.density[cc * S.rows + rr];
-                    if (dens > 0.0f) ion_cell(S, iw, hc - cc, hr - rr, dens, fx, fy);
-                }
-                const float gapy = oy > 0.0f ? oy + py - 1.0f : (oy < 0.0f ? -oy - py : 0.0f);
-                if (gapx * gapx + gapy * gapy >= cut2 * 1.0001f) continue;
-                const int2 o = nb.origin(hc, hr, px, py, cc, rr);
-                int j, e;
-                nb.range(cc, rr, j, e);
-                for (; j < e; j++) {
-                    const int2 q = nb.p(j);
-                    pair_term(gt, S.pair_s2, o.x - q.x, o.y - q.y, sx, sy);
-                }
-            }
-        }
-        fx += S.k_pair_fx * sx;
-        fy += S.k_pair_fx * sy;
-        ax += fx * S.inv_mass;
-        ay += fy * S.inv_mass;
+        const float2 f = sparse_close_force(S, gt, hc, hr, px, py, (info.flags & F_ION_UNIFORM) != 0u, home);
+        ax += f.x * S.inv_mass;
+        ay += f.y * S.inv_mass;
     }
     finish(S, id, hc, hr, px, py, vx, vy, info.damping, ax, ay, h);
 }
@@ -749,7 +733,7 @@ __device__ __forceinline__ void kick_core(const DevScene &S, int slot, int k, fl
 }
 
 __device__ void push_one(const DevScene &S, int slot, int k, float h, uint32_t t) {
-    push_core(S, GlobalNeighbours{S}, S.g_lerp, slot, k, h, t);
+    push_core(S, S.g_lerp, slot, k, h, t);
 }
 
 // Inclusive prefix sum over the 64 lanes of a wave.
@@ -935,41 +919,12 @@ constexpr int CELL_IMAX = 8;
 __device__ __forceinline__ int item_e(int n) { return n >= 2 ? 2 : 1; }
 
 // The close-range force alone on one electron (electron mode of FORCE).
-template <class NB>
-__device__ void force_core(const DevScene &S, const NB &nb, const lerp2 *gt, int slot, float2 *force_out) {
+__device__ void force_core(const DevScene &S, const lerp2 *gt, int slot, float2 *force_out) {
     const int home = S.scell[slot];
     const int hc = home / S.rows, hr = home - (home / S.rows) * S.rows;
     const float2 f = snap_pos(S, slot);
-    const float px = f.x, py = f.y;
-    const CellInfo info = S.info[home];
-    float fx = 0.0f, fy = 0.0f, sx = 0.0f, sy = 0.0f;
-    const int R = S.reach;
-    const int c_lo = max(hc - R, 0), c_hi = min(hc + R, S.cols - 1);
-    const int r_lo = max(hr - R, 0), r_hi = min(hr + R, S.rows - 1);
-    const bool uniform = (info.flags & F_ION_UNIFORM) != 0u;
-    if (uniform) ion_uniform(S, home, px, py, fx, fy);
-    const IonWeights iw = ion_weights(S, px, py);
-    for (int cc = c_lo; cc <= c_hi; cc++) {
-        const float ox = (float)(hc - cc);
-        const float gapx = ox > 0.0f ? ox + px - 1.0f : (ox < 0.0f ? -ox - px : 0.0f);
-        for (int rr = r_lo; rr <= r_hi; rr++) {
-            const float oy = (float)(hr - rr);
-            if (!uniform) {
-                const float dens = S.density[cc * S.rows + rr];
-                if (dens > 0.0f) ion_cell(S, iw, hc - cc, hr - rr, dens, fx, fy);
-            }
-            const float gapy = oy > 0.0f ? oy + py - 1.0f : (oy < 0.0f ? -oy - py : 0.0f);
-            if (gapx * gapx + gapy * gapy >= S.cutoff2 * 1.0001f) continue;
-            const int2 o = nb.origin(hc, hr, px, py, cc, rr);
-            int j, e;
-            nb.range(cc, rr, j, e);
-            for (; j < e; j++) {
-                const int2 q = nb.p(j);
-                pair_term(gt, S.pair_s2, o.x - q.x, o.y - q.y, sx, sy);
-            }
-        }
-    }
-    force_out[S.order[slot]] = make_float2(fx + S.k_pair_fx * sx, fy + S.k_pair_fx * sy);
+    const bool uniform = (S.info[home].flags & F_ION_UNIFORM) != 0u;
+    force_out[S.order[slot]] = sparse_close_force(S, gt, hc, hr, f.x, f.y, uniform, home);
 }
 
 // Substep k for the electrons of the crowded push tiles (tile_dense), per
@@ -1201,9 +1156,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
             continue;
         }
         if (FORCE)
-            force_core(S, GlobalNeighbours{S}, S.g_lerp, slot, force_out);
+            force_core(S, S.g_lerp, slot, force_out);
         else
-            push_core(S, GlobalNeighbours{S}, S.g_lerp, slot, k, h, t);
+            push_core(S, S.g_lerp, slot, k, h, t);
     }
 }
 
