@@ -104,7 +104,9 @@ def run_single(args):
 
 
 # Kernels that carry each tick phase (the rocprofv3 names without the namespace).
-PHASE_KERNELS = {"trim": ["k_trim", "k_trim_gain"], "push": ["k_push_tiled", "k_push_sparse<false>", "k_push_heavy<false>"], "rest": ["k_rest"],
+# The push phase holds the battery trim too: its blocks run in k_push_tiled's
+# launch and its gain kernel before the kick.
+PHASE_KERNELS = {"push": ["k_push_tiled", "k_trim_gain", "k_push_sparse<false>", "k_push_heavy<false>"], "rest": ["k_rest"],
                  "count": ["k_count"], "scan": ["k_scan", "k_scan_top"], "scatter": ["k_scatter"], "canon": ["k_canon"],
                  "deposit_smooth": ["k_deposit_smooth"], "fdtd": ["k_fdtd"], "correct": ["k_correct"],
                  "finish": ["k_finish"],

@@ -65,7 +65,7 @@ constexpr int PUSH_BLOCKS_MAX = 2048;   // tiled-push blocks per scene: four rou
 constexpr int DEP_TILE = 24;      // cells per side of a deposit + smoothing tile
 constexpr int CELL_MODE_MIN = 6;  // a push tile with a cell this crowded runs the cell mode
 constexpr int HEAVY_CELL = 64;    // a cell this crowded is pushed by k_push_heavy, spread over blocks
-constexpr int TRIM_BLOCKS = 8;    // blocks per battery stack in k_trim
+constexpr int TRIM_BLOCKS = 8;    // blocks per battery stack in the trim (trim_block)
 constexpr int REST_PHASES = 7;    // work phases of one extra electron substep in k_rest
 constexpr int REST_THREADS = 1024; // threads per k_rest block (the push tiles' layout)
 constexpr int FDTD_STRIP = 60;    // corner columns per FDTD wave (64 lanes: two halo lanes each side)
@@ -219,7 +219,7 @@ struct DevScene {
     float *trim_part;         // per stack and trim block: terminal potential sums (low, high)
     float *stack_read;        // per stack: the last trim's v_low, v_high, and 1 if it trimmed (else 0)
     int32_t *rest_sync;       // [rest_sync_len(max_substeps)]: k_rest's work ticket, then items done per
-                              // (substep, phase); zeroed by k_trim at the start of every tick
+                              // (substep, phase); zeroed by the trim at the start of every tick
     const float *phase_tab;   // [TICK_TABLE][n_ac]: frac(f t) per AC cell and tick
     const float *target_tab;  // [TICK_TABLE][n_stacks]: EMF target per tick
     int32_t n_ac;

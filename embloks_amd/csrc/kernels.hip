@@ -5,7 +5,8 @@
 // crates/embloks-core/src/simulation.rs:63-72, in the order of
 // GpuSim::encode_ticks, crates/embloks-gpu/src/sim.rs:607-661):
 //
-//   k_trim      battery gain feedback (battery.rs:260-281)
+//   trim        battery gain feedback (battery.rs:260-281): trim_block's
+//               blocks in the tiled push's launch, then k_trim_gain
 //   k_push      electron substep 0: gather, impressed field, P3M close force,
 //               implicit drag, step clamp, confinement (physics.rs:124-162)
 //   k_rest      electron substeps 1.. when the tick needs more than one
@@ -328,97 +329,99 @@ __device__ __forceinline__ float square_floor(const DevScene &S, int k, float u,
 // close-range potential of the ions and electrons near it (Simulation::potential,
 // simulation.rs:48-55; CloseRange::potential, close_range.rs:415-464).
 
-constexpr int TRIM_THREADS = 512;
-
-__global__ __launch_bounds__(TRIM_THREADS) void k_trim(Batch b) {
-    __shared__ float part[2][TRIM_THREADS / 64];
-    for (int blk = blockIdx.x; blk < b.total[KC_STACKS]; blk += gridDim.x) {
-        const int s = scene_of(b, KC_STACKS, blk);
-        const DevScene &S = b.scenes[s];
-        const int local = block_in_scene(b, KC_STACKS, s, blk);
-        const int st = local / TRIM_BLOCKS, tb = local - st * TRIM_BLOCKS;
-        const uint32_t t = S.ctr->tick;
-        if (local == 0 && threadIdx.x == 0) S.ctr->vmax_bits[(t + 1) & 1] = 0;
-        if (local == 0)   // k_rest's queue of this tick starts empty
-            for (int q = threadIdx.x; q < rest_sync_len(S.max_substeps); q += TRIM_THREADS) S.rest_sync[q] = 0;
-        if (st >= S.n_stacks) continue;
-        const StackInfo I = S.stacks[st];
-        if (!I.both_wired || I.amplitude == 0.0f) {
-            if (tb == 0 && threadIdx.x == 0) {
-                S.has_gain[st] = 0;
-                S.gains[st] = 1.0f;
-                S.stack_read[3 * st + 2] = 0.0f;   // the readings stay as they were (fields.wgsl:321-324)
-            }
-            continue;
+// One block's share of a battery stack's terminal potentials (block `blk` of
+// KC_STACKS: TRIM_BLOCKS blocks per stack, THREADS threads each), written to
+// trim_part for k_trim_gain. The trim only reads the potential and the sorted
+// snapshot of the tick before, so its blocks run in the tiled push's launch,
+// beside the push's own blocks; the gain follows in a kernel of its own, before
+// the kick reads it.
+template <int THREADS>
+__device__ __forceinline__ void trim_block(const Batch &b, int blk) {
+    __shared__ float part[2][THREADS / 64];
+    const int s = scene_of(b, KC_STACKS, blk);
+    const DevScene &S = b.scenes[s];
+    const int local = block_in_scene(b, KC_STACKS, s, blk);
+    const int st = local / TRIM_BLOCKS, tb = local - st * TRIM_BLOCKS;
+    const uint32_t t = S.ctr->tick;
+    if (local == 0 && threadIdx.x == 0) S.ctr->vmax_bits[(t + 1) & 1] = 0;
+    if (local == 0)   // k_rest's queue of this tick starts empty
+        for (int q = threadIdx.x; q < rest_sync_len(S.max_substeps); q += THREADS) S.rest_sync[q] = 0;
+    if (st >= S.n_stacks) return;
+    const StackInfo I = S.stacks[st];
+    if (!I.both_wired || I.amplitude == 0.0f) {
+        if (tb == 0 && threadIdx.x == 0) {
+            S.has_gain[st] = 0;
+            S.gains[st] = 1.0f;
+            S.stack_read[3 * st + 2] = 0.0f;   // the readings stay as they were (fields.wgsl:321-324)
         }
-        const int R = S.close_on ? S.reach : -1;
-        const int side = 2 * R + 1;
-        const int per = 1 + (S.close_on ? side * side : 0);
-        const int lists = I.low_n + I.high_n;
-        float acc_lo = 0.0f, acc_hi = 0.0f;
-        // Work item = (terminal cell, neighbour cell, one of TRIM_SPLIT lanes over
-        // that cell's electrons), so no thread walks a whole cell's electrons; the
-        // stack's TRIM_BLOCKS blocks take every TRIM_BLOCKS-th stride of items.
-        constexpr int TRIM_SPLIT = 4;
-        for (int it = tb * TRIM_THREADS + (int)threadIdx.x; it < lists * per * TRIM_SPLIT;
-             it += TRIM_THREADS * TRIM_BLOCKS) {
-            const int part = it % TRIM_SPLIT, item = it / TRIM_SPLIT;
-            const int li = item / per, sub = item - li * per;
-            const bool high = li >= I.low_n;
-            const int cell = S.stack_cells[high ? I.high_off + (li - I.low_n) : I.low_off + li];
-            const int c = cell / S.rows, r = cell - c * S.rows;
-            float v = 0.0f;
-            if (sub == 0) {
-                if (part == 0) {
-                    v = S.phi[(size_t)(c + S.margin) * S.pny + (r + S.margin)];
-                    if (S.close_on) v += S.ion_pot[cell];
-                }
-            } else {
-                const int cc = c + (sub - 1) / side - R, rr = r + (sub - 1) % side - R;
-                if (cc >= 0 && rr >= 0 && cc < S.cols && rr < S.rows) {
-                    const int nb = cc * S.rows + rr;
-                    // Offset of the electron from the centre of cell (c, r).
-                    const float ox = (float)(cc - c) - 0.5f + S.half, oy = (float)(rr - r) - 0.5f + S.half;
-                    for (int k = S.cstart[nb] + part; k < S.cstart[nb + 1]; k += TRIM_SPLIT) {
-                        const float2 f = snap_pos(S, k);
-                        const float u = (ox + f.x) * S.square_samples;
-                        const float w = (oy + f.y) * S.square_samples;
-                        v += square_floor(S, 0, u, w);
-                    }
-                    v *= S.k_q;
-                }
-            }
-            if (high) acc_hi += v;
-            else acc_lo += v;
-        }
-        for (int o = 32; o > 0; o >>= 1) {
-            acc_lo += __shfl_xor(acc_lo, o, 64);
-            acc_hi += __shfl_xor(acc_hi, o, 64);
-        }
-        if ((threadIdx.x & 63) == 0) {
-            part[0][threadIdx.x >> 6] = acc_lo;
-            part[1][threadIdx.x >> 6] = acc_hi;
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            float lo = 0.0f, hi = 0.0f;
-            for (int w = 0; w < TRIM_THREADS / 64; w++) {
-                lo += part[0][w];
-                hi += part[1][w];
-            }
-            float *mine = S.trim_part + 2 * ((size_t)st * TRIM_BLOCKS + tb);
-            mine[0] = lo;
-            mine[1] = hi;
-        }
-        __syncthreads();
+        return;
     }
+    const int R = S.close_on ? S.reach : -1;
+    const int side = 2 * R + 1;
+    const int per = 1 + (S.close_on ? side * side : 0);
+    const int lists = I.low_n + I.high_n;
+    float acc_lo = 0.0f, acc_hi = 0.0f;
+    // Work item = (terminal cell, neighbour cell, one of TRIM_SPLIT lanes over
+    // that cell's electrons), so no thread walks a whole cell's electrons; the
+    // stack's TRIM_BLOCKS blocks take every TRIM_BLOCKS-th stride of items.
+    constexpr int TRIM_SPLIT = 4;
+    for (int it = tb * THREADS + (int)threadIdx.x; it < lists * per * TRIM_SPLIT; it += THREADS * TRIM_BLOCKS) {
+        const int part_i = it % TRIM_SPLIT, item = it / TRIM_SPLIT;
+        const int li = item / per, sub = item - li * per;
+        const bool high = li >= I.low_n;
+        const int cell = S.stack_cells[high ? I.high_off + (li - I.low_n) : I.low_off + li];
+        const int c = cell / S.rows, r = cell - c * S.rows;
+        float v = 0.0f;
+        if (sub == 0) {
+            if (part_i == 0) {
+                v = S.phi[(size_t)(c + S.margin) * S.pny + (r + S.margin)];
+                if (S.close_on) v += S.ion_pot[cell];
+            }
+        } else {
+            const int cc = c + (sub - 1) / side - R, rr = r + (sub - 1) % side - R;
+            if (cc >= 0 && rr >= 0 && cc < S.cols && rr < S.rows) {
+                const int nb = cc * S.rows + rr;
+                // Offset of the electron from the centre of cell (c, r).
+                const float ox = (float)(cc - c) - 0.5f + S.half, oy = (float)(rr - r) - 0.5f + S.half;
+                for (int k = S.cstart[nb] + part_i; k < S.cstart[nb + 1]; k += TRIM_SPLIT) {
+                    const float2 f = snap_pos(S, k);
+                    const float u = (ox + f.x) * S.square_samples;
+                    const float w = (oy + f.y) * S.square_samples;
+                    v += square_floor(S, 0, u, w);
+                }
+                v *= S.k_q;
+            }
+        }
+        if (high) acc_hi += v;
+        else acc_lo += v;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        acc_lo += __shfl_xor(acc_lo, o, 64);
+        acc_hi += __shfl_xor(acc_hi, o, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        part[0][threadIdx.x >> 6] = acc_lo;
+        part[1][threadIdx.x >> 6] = acc_hi;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float lo = 0.0f, hi = 0.0f;
+        for (int w = 0; w < THREADS / 64; w++) {
+            lo += part[0][w];
+            hi += part[1][w];
+        }
+        float *mine = S.trim_part + 2 * ((size_t)st * TRIM_BLOCKS + tb);
+        mine[0] = lo;
+        mine[1] = hi;
+    }
+    __syncthreads();
 }
 
-// The stack's gain from its blocks' parts (k_trim, the kernel before), added in
-// block order, so the result does not depend on which block ended last. A
-// kernel of its own rather than a last-block handshake: the release/acquire of
-// a device-scope counter writes back and invalidates the L2 of the block's XCD,
-// once per block of k_trim.
+// The stack's gain from its blocks' parts (trim_block, in the kernel before),
+// added in block order, so the result does not depend on which block ended
+// last. A kernel of its own rather than a last-block handshake: the
+// release/acquire of a device-scope counter writes back and invalidates the L2
+// of the block's XCD, once per trim block.
 __global__ __launch_bounds__(64) void k_trim_gain(Batch b) {
     for (int blk = blockIdx.x; blk < b.total[KC_STACKS]; blk += gridDim.x) {
         const int s = scene_of(b, KC_STACKS, blk);
@@ -427,7 +430,7 @@ __global__ __launch_bounds__(64) void k_trim_gain(Batch b) {
         const int st = local / TRIM_BLOCKS, tb = local - st * TRIM_BLOCKS;
         if (tb != 0 || st >= S.n_stacks || threadIdx.x != 0) continue;
         const StackInfo I = S.stacks[st];
-        if (!I.both_wired || I.amplitude == 0.0f) continue;   // k_trim set the stack's readings
+        if (!I.both_wired || I.amplitude == 0.0f) continue;   // trim_block set the stack's readings
         const uint32_t t = S.ctr->tick;
         float lo = 0.0f, hi = 0.0f;
         const float *parts = S.trim_part + 2 * (size_t)st * TRIM_BLOCKS;
@@ -1089,11 +1092,11 @@ __device__ __forceinline__ void push_tile(const DevScene &S, PushTileLds &L, int
 }
 
 template <int MODE, int THREADS>
-__device__ __forceinline__ void push_tiled_body(Batch b, int k, float2 *force_out) {
+__device__ __forceinline__ void push_tiled_body(Batch b, int k, float2 *force_out, int first, int stride) {
     constexpr bool FORCE = MODE == ITEM_FORCE;
     __shared__ PushTileLds L;
     bool table_loaded = false;
-    for (int blk = blockIdx.x; blk < b.total[KC_PUSH]; blk += gridDim.x) {
+    for (int blk = first; blk < b.total[KC_PUSH]; blk += stride) {
         const int s = scene_of(b, KC_PUSH, blk);
         const DevScene &S = b.scenes[s];
         const uint32_t t = S.ctr->tick;
@@ -1269,12 +1272,22 @@ __global__ __launch_bounds__(HEAVY_THREADS) void k_push_heavy(Batch b, int k, fl
 template __global__ void k_push_heavy<false>(Batch, int, float2 *);
 template __global__ void k_push_heavy<true>(Batch, int, float2 *);
 
-__global__ __launch_bounds__(PUSH_THREADS) __attribute__((amdgpu_waves_per_eu(8))) void k_push_tiled(Batch b, int k) {
-    push_tiled_body<ITEM_STORE, PUSH_THREADS>(b, k, nullptr);
+// The tick's first kernel: blocks [0, trim_grid) compute the battery trim's
+// terminal potentials (trim_block), the others run the tiled push of substep k.
+// The trim reads only what the tick before left (the potential and the sorted
+// snapshot) and the push does not read the gains, so the trim's latency-bound
+// blocks overlap the push instead of running before it.
+__global__ __launch_bounds__(PUSH_THREADS) __attribute__((amdgpu_waves_per_eu(8))) void k_push_tiled(Batch b, int k,
+                                                                                                  int trim_grid) {
+    if ((int)blockIdx.x < trim_grid) {
+        for (int blk = blockIdx.x; blk < b.total[KC_STACKS]; blk += trim_grid) trim_block<PUSH_THREADS>(b, blk);
+        return;
+    }
+    push_tiled_body<ITEM_STORE, PUSH_THREADS>(b, k, nullptr, (int)blockIdx.x - trim_grid, (int)gridDim.x - trim_grid);
 }
 
 __global__ __launch_bounds__(FORCE_THREADS) void k_force_tiled(Batch b, float2 *force_out) {
-    push_tiled_body<ITEM_FORCE, FORCE_THREADS>(b, 0, force_out);
+    push_tiled_body<ITEM_FORCE, FORCE_THREADS>(b, 0, force_out, (int)blockIdx.x, (int)gridDim.x);
 }
 
 // Substeps 1.. of a tick that needs several (fast electrons; physics.rs:124-162:
@@ -2035,39 +2048,43 @@ hipError_t finish_sync_kernel(const Batch &b, hipStream_t stream) {
     } while (0)
 hipError_t tick_kernels(const Batch &b, int fdtd_substeps, hipStream_t stream, hipEvent_t *marks) {
     EMB_MARK(0);
-    EMB_LAUNCH(k_trim, KC_STACKS, TRIM_THREADS, b);
+    {   // the trim's blocks, then the tiled push's (KC_STACKS is never empty)
+        const int trim_grid = grid_of(b.total[KC_STACKS]);
+        const int push_grid = b.total[KC_PUSH] > 0 ? grid_of(b.total[KC_PUSH]) : 0;
+        k_push_tiled<<<trim_grid + push_grid, PUSH_THREADS, 0, stream>>>(b, 0, trim_grid);
+    }
     EMB_LAUNCH(k_trim_gain, KC_STACKS, 64, b);
-    EMB_MARK(1);
-    EMB_LAUNCH(k_push_tiled, KC_PUSH, PUSH_THREADS, b, 0);
     EMB_LAUNCH(k_push_sparse<false>, KC_ELECTRONS, 256, b, 0, nullptr);
     EMB_LAUNCH(k_push_heavy<false>, KC_HEAVY, HEAVY_THREADS, b, 0, nullptr);
-    EMB_MARK(2);
+    EMB_MARK(1);
     if (b.total[KC_ELECTRONS] > 0) EMB_LAUNCH(k_rest, KC_REST, REST_THREADS, b);
-    EMB_MARK(3);
+    EMB_MARK(2);
     EMB_LAUNCH(k_count, KC_ELECTRONS, 256, b);
-    EMB_MARK(4);
+    EMB_MARK(3);
     EMB_LAUNCH(k_scan, KC_SCAN, 256, b);
     EMB_LAUNCH(k_scan_top, KC_SCENE, 256, b);
-    EMB_MARK(5);
+    EMB_MARK(4);
     EMB_LAUNCH(k_scatter, KC_ELECTRONS, 256, b);
-    EMB_MARK(6);
+    EMB_MARK(5);
     EMB_LAUNCH(k_canon, KC_CELLS, 256, b);
-    EMB_MARK(7);
+    EMB_MARK(6);
     EMB_LAUNCH(k_deposit_smooth, KC_SMOOTH, DEP_THREADS, b, 0);
-    EMB_MARK(8);
+    EMB_MARK(7);
     for (int f = 0; f < fdtd_substeps; f++) EMB_LAUNCH(k_fdtd, KC_FDTD, 64 * FDTD_WAVES, b, f);
-    EMB_MARK(9);
+    EMB_MARK(8);
     hipError_t e = launch_dst_gemms(b, stream, DST_TICK);
     if (e != hipSuccess) return e;
-    EMB_MARK(10);
+    EMB_MARK(9);
     EMB_LAUNCH(k_correct, KC_NODES, 256, b, 1);
-    EMB_MARK(11);
+    EMB_MARK(10);
     EMB_LAUNCH(k_finish, KC_CELLS, 256, b, 0);
-    EMB_MARK(12);
+    EMB_MARK(11);
     return hipGetLastError();
 }
 
-extern const char *const TICK_PHASE_NAMES[] = {"trim", "push", "rest", "count", "scan", "scatter", "canon",
+// (the battery trim runs inside the push phase: its blocks beside the tiled
+// push's, its gain kernel before the kick)
+extern const char *const TICK_PHASE_NAMES[] = {"push", "rest", "count", "scan", "scatter", "canon",
                                         "deposit_smooth", "fdtd", "dst_gemm", "correct", "finish"};
 
 hipError_t deposit_kernel(const Batch &b, hipStream_t stream) {

@@ -137,7 +137,7 @@ void scene_blocks(const DevScene &d, int out[KC_COUNT]);
 // Kernel sequences over a batch (kernels.hip).
 hipError_t sort_kernels(const Batch &b, hipStream_t stream);
 hipError_t setup_kernels(const Batch &b, hipStream_t stream);   // sort, deposit, Gauss (no FDTD)
-constexpr int TICK_PHASES = 12;
+constexpr int TICK_PHASES = 11;
 extern const char *const TICK_PHASE_NAMES[];
 hipError_t tick_kernels(const Batch &b, int fdtd_substeps, hipStream_t stream, hipEvent_t *marks = nullptr);
 hipError_t deposit_kernel(const Batch &b, hipStream_t stream);
