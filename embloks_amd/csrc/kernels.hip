@@ -873,10 +873,35 @@ __device__ __forceinline__ void cell_item(const DevScene &S, const RUNS &runs, c
     if (!uniform) {
         const IonWeights iw = ion_weights(S, mpx, mpy);
         const int nr = r_hi - r_lo + 1, cells = (c_hi - c_lo + 1) * nr;
-        for (int w = sl / E; w < cells; w += SL / E) {
-            const int cc = c_lo + w / nr, rr = r_lo + w % nr;
-            const float dens = dens_w[(cc - wc0) * W + (rr - wr0)];
-            if (dens > 0.0f) ion_cell(S, iw, hc - cc, hr - rr, dens, fx, fy);
+        if (cells <= 64) {
+            // The window's cells that hold ions, compacted in cell order: lane w
+            // tests cell w, and the k-th cell with ions goes to lane k (a forward
+            // permute; the empty cells take the lanes after them). Lanes then
+            // walk the list, not the whole window: near a conductor's edge most
+            // window cells are empty.
+            const int cc = c_lo + lane / nr, rr = r_lo + lane % nr;
+            const float dens = lane < cells ? dens_w[(cc - wc0) * W + (rr - wr0)] : 0.0f;
+            const unsigned long long m = __ballot(dens > 0.0f);
+            const int K = __popcll(m);
+            const int below = __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+            const int dst = dens > 0.0f ? below : K + (lane - below);
+            const int packed = ((hc - cc + 8) << 4) | (hr - rr + 8);   // offsets in [-4, 4]
+            const int cpk = __builtin_amdgcn_ds_permute(dst * 4, packed);
+            const int cdn = __builtin_amdgcn_ds_permute(dst * 4, __float_as_int(dens));
+            // (the loop bound is the wave's: a lane reads its entry from another
+            // lane, which must still be active, so every lane runs every round)
+            for (int j0 = 0; j0 < K; j0 += SL / E) {
+                const int j = j0 + sl / E;
+                const int pk = __builtin_amdgcn_ds_bpermute(min(j, 63) * 4, cpk);
+                const float dn = __int_as_float(__builtin_amdgcn_ds_bpermute(min(j, 63) * 4, cdn));
+                if (j < K) ion_cell(S, iw, (pk >> 4) - 8, (pk & 15) - 8, dn, fx, fy);
+            }
+        } else {
+            for (int w = sl / E; w < cells; w += SL / E) {
+                const int cc = c_lo + w / nr, rr = r_lo + w % nr;
+                const float dens = dens_w[(cc - wc0) * W + (rr - wr0)];
+                if (dens > 0.0f) ion_cell(S, iw, hc - cc, hr - rr, dens, fx, fy);
+            }
         }
     }
     for (int off = I; off < 64; off <<= 1) {
