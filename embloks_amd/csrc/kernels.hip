@@ -1531,9 +1531,20 @@ __device__ __forceinline__ long long rint_ll(float x) {
     return SMALL ? (long long)__float2int_rn(x) : __float2ll_rn(x);
 }
 
-__global__ __launch_bounds__(DEP_THREADS) void k_deposit_smooth(Batch b, int bias) {
+// A tile whose source cells hold at most this many electrons takes them one
+// per thread (12 atomics each) instead of one source cell per thread: in a
+// sparse scene most source cells are empty, and a thread per cell walks them
+// all for a handful of electrons.
+constexpr int DEP_SPARSE_MAX = 2 * DEP_THREADS;
+constexpr int DEP_SRC = ST + 2 * MAX_PASSES + 2;   // source columns of a tile at most
+
+// (held to 128 VGPRs: two blocks per CU)
+__global__ __launch_bounds__(DEP_THREADS) __attribute__((amdgpu_waves_per_eu(4))) void k_deposit_smooth(Batch b,
+                                                                                                   int bias) {
     __shared__ long long dep[3][SW * SW];
     __shared__ long long tmp[ST * SW];
+    __shared__ long long lcur[2][ST * ST];   // the electron path's cell current of the tile's own cells
+    __shared__ int col_base[DEP_SRC + 1], col_g0[DEP_SRC];
     __shared__ float wmax[DEP_THREADS / 64];
     for (int blk = blockIdx.x; blk < b.total[KC_SMOOTH]; blk += gridDim.x) {
         const int s = scene_of(b, KC_SMOOTH, blk);
@@ -1556,103 +1567,180 @@ __global__ __launch_bounds__(DEP_THREADS) void k_deposit_smooth(Batch b, int bia
             dep[1][i * SW + j] = 0;
             dep[2][i * SW + j] = 0;
         }
-        __syncthreads();   // every entry initialised before any wave adds to it
-        // Electrons whose CIC corners reach the window live in cells [w0 - 1, w0 + W].
-        // One thread per such source cell sums what the cell's electrons put on
-        // the 3 x 3 nodes around it (CIC, fields.rs:27-40: the lower corner is
-        // floor(x - 1/2), 22-bit fixed-point weights per axis) in registers, in
-        // the cell's canonical electron order, then adds the nine sums to the
-        // window with LDS atomics: 27 atomics per cell, not 12 per electron.
-        // Charge: the exact int64 products of the weights. Current: the FP32 sum
-        // of v wx wy over the cell, rounded to int64 once per node. Integer atomics
-        // give the same bits in any order. A thread whose cell the tile owns also
-        // sums the cell current (physics.rs:34-43) and the speed maximum.
         const int sc0 = max(wc0 - 1, 0), sc1 = min(wc0 + W, S.cols - 1);
         const int sr0 = max(wr0 - 1, 0), sr1 = min(wr0 + W, S.rows - 1);
-        const int ncy = max(sr1 - sr0 + 1, 0), ncell = max(sc1 - sc0 + 1, 0) * ncy;
-        // The scene's arrays in registers: the LDS atomics would otherwise make the
-        // compiler reload them from the scene table every iteration.
-        const float4 *snap = S.snap;
-        const int32_t *cstart = S.cstart;
-        const int rows = S.rows, cols = S.cols;
+        const int ncol = max(sc1 - sc0 + 1, 0);
+        // The source columns' electron runs (rows sr0..sr1 of a column are
+        // contiguous in the sorted order); their total picks the path below.
+        if (threadIdx.x < 64) {
+            const int q = threadIdx.x;
+            int g0 = 0, m = 0;
+            if (q < ncol && sr0 <= sr1) {
+                g0 = S.cstart[(sc0 + q) * S.rows + sr0];
+                m = S.cstart[(sc0 + q) * S.rows + sr1 + 1] - g0;
+            }
+            const int incl = wave_scan_incl(m);
+            if (q < ncol) {
+                col_g0[q] = g0;
+                col_base[q + 1] = incl;
+            }
+            if (q == 0) col_base[0] = 0;
+        }
+        for (int q = threadIdx.x; q < ST * ST; q += DEP_THREADS) {
+            lcur[0][q] = 0;
+            lcur[1][q] = 0;
+        }
+        __syncthreads();   // every entry initialised before any wave adds to it
+        const int total = col_base[ncol];
         float vmax = 0.0f;
-        for (int q = threadIdx.x; q < ncell; q += DEP_THREADS) {
-            const int ce = sc0 + q / ncy, re = sr0 + q % ncy;
-            const int cell = ce * rows + re;
-            const int k0 = cstart[cell], k1 = cstart[cell + 1];
-            unsigned long long qn[9];
-            float jxn[9], jyn[9];
+        if (total <= DEP_SPARSE_MAX) {
+            // One electron per thread: its column by search over the runs, its
+            // row from the snapshot's cell; CIC weights as below, each nonzero
+            // node's charge product and current (rounded per electron) added to
+            // the window, the cell current to the tile's own cells.
+            for (int j = threadIdx.x; j < total; j += DEP_THREADS) {
+                int q = 0;
+                for (int step = 16; step > 0; step >>= 1)
+                    if (q + step < ncol && col_base[q + step] <= j) q += step;
+                const int g = col_g0[q] + (j - col_base[q]);
+                const int ce = sc0 + q, re = S.scell[g] - ce * S.rows;
+                const float4 e4 = S.snap[g];
+                const float x = e4.x, y = e4.y, vx = e4.z, vy = e4.w;
+                const bool lx = x < 0.5f, ly = y < 0.5f;
+                const int fu = (int)rintf((lx ? x + 0.5f : x - 0.5f) * 4194304.0f);
+                const int fv = (int)rintf((ly ? y + 0.5f : y - 0.5f) * 4194304.0f);
+                const int w0x = (1 << W_BITS) - fu, w0y = (1 << W_BITS) - fv;
+                const int wxn[2] = {w0x, fu}, wyn[2] = {w0y, fv};
+                const int nx0 = lx ? ce - 1 : ce, ny0 = ly ? re - 1 : re;   // lower corner node
 #pragma unroll
-            for (int n = 0; n < 9; n++) {
-                qn[n] = 0;
-                jxn[n] = 0.0f;
-                jyn[n] = 0.0f;
-            }
-            long long ix = 0, iy = 0;
-            // Four electrons per round, their loads all in flight before the sums;
-            // the padding slots past k1 add zero weights.
-            for (int kb = k0; kb < k1; kb += 4) {
-                float xs[4], ys[4], vxs[4], vys[4];
+                for (int i = 0; i < 2; i++) {
+                    const int li = clampi(nx0 + i, 0, S.cols - 1) - wc0;
+                    if (li < 0 || li >= W) continue;
 #pragma unroll
-                for (int u = 0; u < 4; u++) {
-                    const bool on = kb + u < k1;
-                    const float4 e4 = snap[on ? kb + u : k0];
-                    xs[u] = e4.x;
-                    ys[u] = e4.y;
-                    vxs[u] = on ? e4.z : 0.0f;
-                    vys[u] = on ? e4.w : 0.0f;
-                }
-#pragma unroll
-                for (int u = 0; u < 4; u++) {
-                    if (kb + u >= k1) break;
-                    const float x = xs[u], y = ys[u], vx = vxs[u], vy = vys[u];
-                    const bool lx = x < 0.5f, ly = y < 0.5f;   // lower corner at ce - 1 / re - 1
-                    const int fu = (int)rintf((lx ? x + 0.5f : x - 0.5f) * 4194304.0f);
-                    const int fv = (int)rintf((ly ? y + 0.5f : y - 0.5f) * 4194304.0f);
-                    const int w0x = (1 << W_BITS) - fu, w0y = (1 << W_BITS) - fv;
-                    // weights on node columns ce - 1, ce, ce + 1 and rows re - 1, re, re + 1
-                    const int wxn[3] = {lx ? w0x : 0, lx ? fu : w0x, lx ? 0 : fu};
-                    const int wyn[3] = {ly ? w0y : 0, ly ? fv : w0y, ly ? 0 : fv};
-#pragma unroll
-                    for (int i = 0; i < 3; i++)
-#pragma unroll
-                        for (int j = 0; j < 3; j++) {
-                            qn[3 * i + j] += (unsigned long long)(unsigned)wxn[i] * (unsigned)wyn[j];
-                            const float wf = __fmul_rn((float)wxn[i], (float)wyn[j]);
-                            jxn[3 * i + j] = __fmaf_rn(vx, wf, jxn[3 * i + j]);
-                            jyn[3 * i + j] = __fmaf_rn(vy, wf, jyn[3 * i + j]);
-                        }
-                    if (fabsf(vx) < 8.0f && fabsf(vy) < 8.0f) {   // |v| 2^28 < 2^31
-                        ix += rint_ll<true>(__fmul_rn(vx, (float)J_SCALE));
-                        iy += rint_ll<true>(__fmul_rn(vy, (float)J_SCALE));
-                    } else {
-                        ix += rint_ll<false>(__fmul_rn(vx, (float)J_SCALE));
-                        iy += rint_ll<false>(__fmul_rn(vy, (float)J_SCALE));
+                    for (int jj = 0; jj < 2; jj++) {
+                        const int lj = clampi(ny0 + jj, 0, S.rows - 1) - wr0;
+                        const unsigned long long qw = (unsigned long long)(unsigned)wxn[i] * (unsigned)wyn[jj];
+                        if (lj < 0 || lj >= W || qw == 0) continue;
+                        const int at = li * SW + lj;
+                        const float wf = __fmul_rn((float)wxn[i], (float)wyn[jj]);
+                        atomicAdd((unsigned long long *)&dep[0][at], as_ull(-(long long)qw));
+                        atomicAdd((unsigned long long *)&dep[1][at], as_ull(__float2ll_rn(__fmul_rn(__fmul_rn(vx, wf), J_FROM_W))));
+                        atomicAdd((unsigned long long *)&dep[2][at], as_ull(__float2ll_rn(__fmul_rn(__fmul_rn(vy, wf), J_FROM_W))));
                     }
-                    vmax = fmaxf(vmax, sqrtf(vx * vx + vy * vy));
                 }
+                const int oc = ce - (R0x - P), orow = re - (R0y - P);
+                if (oc >= 0 && oc < ST && orow >= 0 && orow < ST) {
+                    const long long ix = fabsf(vx) < 8.0f ? rint_ll<true>(__fmul_rn(vx, (float)J_SCALE))
+                                                          : rint_ll<false>(__fmul_rn(vx, (float)J_SCALE));
+                    const long long iy = fabsf(vy) < 8.0f ? rint_ll<true>(__fmul_rn(vy, (float)J_SCALE))
+                                                          : rint_ll<false>(__fmul_rn(vy, (float)J_SCALE));
+                    atomicAdd((unsigned long long *)&lcur[0][oc * ST + orow], as_ull(ix));
+                    atomicAdd((unsigned long long *)&lcur[1][oc * ST + orow], as_ull(iy));
+                }
+                vmax = fmaxf(vmax, sqrtf(vx * vx + vy * vy));
             }
-            // Cells owned by this tile: the cell current.
-            const int oc = ce - (R0x - P), orow = re - (R0y - P);
-            if (oc >= 0 && oc < ST && orow >= 0 && orow < ST) {
-                S.cur[cell] = ix;
-                S.cur[cells + cell] = iy;
+            __syncthreads();
+            for (int q = threadIdx.x; q < ST * ST; q += DEP_THREADS) {
+                const int c = R0x - P + q / ST, r = R0y - P + q % ST;
+                if (c < 0 || r < 0 || c >= S.cols || r >= S.rows) continue;
+                S.cur[c * S.rows + r] = lcur[0][q];
+                S.cur[cells + c * S.rows + r] = lcur[1][q];
             }
-            if (k1 == k0) continue;
-            // Nodes past the grid's edge fold onto the edge (corners clamped,
-            // fields.rs:28-37); nodes outside the window belong to other tiles.
+        } else {
+            // Electrons whose CIC corners reach the window live in cells [w0 - 1, w0 + W].
+            // One thread per such source cell sums what the cell's electrons put on
+            // the 3 x 3 nodes around it (CIC, fields.rs:27-40: the lower corner is
+            // floor(x - 1/2), 22-bit fixed-point weights per axis) in registers, in
+            // the cell's canonical electron order, then adds the nine sums to the
+            // window with LDS atomics: 27 atomics per cell, not 12 per electron.
+            // Charge: the exact int64 products of the weights. Current: the FP32 sum
+            // of v wx wy over the cell, rounded to int64 once per node. Integer atomics
+            // give the same bits in any order. A thread whose cell the tile owns also
+            // sums the cell current (physics.rs:34-43) and the speed maximum.
+            const int ncy = max(sr1 - sr0 + 1, 0), ncell = ncol * ncy;
+            // The scene's arrays in registers: the LDS atomics would otherwise make the
+            // compiler reload them from the scene table every iteration.
+            const float4 *snap = S.snap;
+            const int32_t *cstart = S.cstart;
+            const int rows = S.rows, cols = S.cols;
+            for (int q = threadIdx.x; q < ncell; q += DEP_THREADS) {
+                const int ce = sc0 + q / ncy, re = sr0 + q % ncy;
+                const int cell = ce * rows + re;
+                const int k0 = cstart[cell], k1 = cstart[cell + 1];
+                unsigned long long qn[9];
+                float jxn[9], jyn[9];
 #pragma unroll
-            for (int i = 0; i < 3; i++) {
-                const int li = clampi(ce - 1 + i, 0, cols - 1) - wc0;
-                if (li < 0 || li >= W) continue;
+                for (int n = 0; n < 9; n++) {
+                    qn[n] = 0;
+                    jxn[n] = 0.0f;
+                    jyn[n] = 0.0f;
+                }
+                long long ix = 0, iy = 0;
+                // Four electrons per round, their loads all in flight before the sums;
+                // the padding slots past k1 add zero weights.
+                for (int kb = k0; kb < k1; kb += 4) {
+                    float xs[4], ys[4], vxs[4], vys[4];
 #pragma unroll
-                for (int j = 0; j < 3; j++) {
-                    const int lj = clampi(re - 1 + j, 0, rows - 1) - wr0;
-                    if (lj < 0 || lj >= W || qn[3 * i + j] == 0) continue;
-                    const int at = li * SW + lj;
-                    atomicAdd((unsigned long long *)&dep[0][at], as_ull(-(long long)qn[3 * i + j]));
-                    atomicAdd((unsigned long long *)&dep[1][at], as_ull(__float2ll_rn(__fmul_rn(jxn[3 * i + j], J_FROM_W))));
-                    atomicAdd((unsigned long long *)&dep[2][at], as_ull(__float2ll_rn(__fmul_rn(jyn[3 * i + j], J_FROM_W))));
+                    for (int u = 0; u < 4; u++) {
+                        const bool on = kb + u < k1;
+                        const float4 e4 = snap[on ? kb + u : k0];
+                        xs[u] = e4.x;
+                        ys[u] = e4.y;
+                        vxs[u] = on ? e4.z : 0.0f;
+                        vys[u] = on ? e4.w : 0.0f;
+                    }
+#pragma unroll
+                    for (int u = 0; u < 4; u++) {
+                        if (kb + u >= k1) break;
+                        const float x = xs[u], y = ys[u], vx = vxs[u], vy = vys[u];
+                        const bool lx = x < 0.5f, ly = y < 0.5f;   // lower corner at ce - 1 / re - 1
+                        const int fu = (int)rintf((lx ? x + 0.5f : x - 0.5f) * 4194304.0f);
+                        const int fv = (int)rintf((ly ? y + 0.5f : y - 0.5f) * 4194304.0f);
+                        const int w0x = (1 << W_BITS) - fu, w0y = (1 << W_BITS) - fv;
+                        // weights on node columns ce - 1, ce, ce + 1 and rows re - 1, re, re + 1
+                        const int wxn[3] = {lx ? w0x : 0, lx ? fu : w0x, lx ? 0 : fu};
+                        const int wyn[3] = {ly ? w0y : 0, ly ? fv : w0y, ly ? 0 : fv};
+#pragma unroll
+                        for (int i = 0; i < 3; i++)
+#pragma unroll
+                            for (int j = 0; j < 3; j++) {
+                                qn[3 * i + j] += (unsigned long long)(unsigned)wxn[i] * (unsigned)wyn[j];
+                                const float wf = __fmul_rn((float)wxn[i], (float)wyn[j]);
+                                jxn[3 * i + j] = __fmaf_rn(vx, wf, jxn[3 * i + j]);
+                                jyn[3 * i + j] = __fmaf_rn(vy, wf, jyn[3 * i + j]);
+                            }
+                        if (fabsf(vx) < 8.0f && fabsf(vy) < 8.0f) {   // |v| 2^28 < 2^31
+                            ix += rint_ll<true>(__fmul_rn(vx, (float)J_SCALE));
+                            iy += rint_ll<true>(__fmul_rn(vy, (float)J_SCALE));
+                        } else {
+                            ix += rint_ll<false>(__fmul_rn(vx, (float)J_SCALE));
+                            iy += rint_ll<false>(__fmul_rn(vy, (float)J_SCALE));
+                        }
+                        vmax = fmaxf(vmax, sqrtf(vx * vx + vy * vy));
+                    }
+                }
+                // Cells owned by this tile: the cell current.
+                const int oc = ce - (R0x - P), orow = re - (R0y - P);
+                if (oc >= 0 && oc < ST && orow >= 0 && orow < ST) {
+                    S.cur[cell] = ix;
+                    S.cur[cells + cell] = iy;
+                }
+                if (k1 == k0) continue;
+                // Nodes past the grid's edge fold onto the edge (corners clamped,
+                // fields.rs:28-37); nodes outside the window belong to other tiles.
+#pragma unroll
+                for (int i = 0; i < 3; i++) {
+                    const int li = clampi(ce - 1 + i, 0, cols - 1) - wc0;
+                    if (li < 0 || li >= W) continue;
+#pragma unroll
+                    for (int j = 0; j < 3; j++) {
+                        const int lj = clampi(re - 1 + j, 0, rows - 1) - wr0;
+                        if (lj < 0 || lj >= W || qn[3 * i + j] == 0) continue;
+                        const int at = li * SW + lj;
+                        atomicAdd((unsigned long long *)&dep[0][at], as_ull(-(long long)qn[3 * i + j]));
+                        atomicAdd((unsigned long long *)&dep[1][at], as_ull(__float2ll_rn(__fmul_rn(jxn[3 * i + j], J_FROM_W))));
+                        atomicAdd((unsigned long long *)&dep[2][at], as_ull(__float2ll_rn(__fmul_rn(jyn[3 * i + j], J_FROM_W))));
+                    }
                 }
             }
         }
@@ -1662,7 +1750,13 @@ __global__ __launch_bounds__(DEP_THREADS) void k_deposit_smooth(Batch b, int bia
         if (threadIdx.x == 0) {
             float m = wmax[0];
             for (int w = 1; w < DEP_THREADS / 64; w++) m = fmaxf(m, wmax[w]);
-            atomicMax(&S.ctr->vmax_bits[(t + 1) & 1], __float_as_uint(m));
+            // Every block's maximum goes to one word: atomics on one address queue
+            // at its L2 channel, so a block adds its own only when it is larger
+            // than the word's value (read at device scope, past the L1). The word
+            // only grows within a tick, so a block that skips has nothing to add.
+            unsigned *slot = &S.ctr->vmax_bits[(t + 1) & 1];
+            if (__float_as_uint(m) > __hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+                atomicMax(slot, __float_as_uint(m));
         }
         if (S.dep_dump) {
             for (int q = threadIdx.x; q < ST * ST; q += DEP_THREADS) {
