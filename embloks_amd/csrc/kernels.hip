@@ -203,6 +203,7 @@ constexpr int CANON_CAP = 4096;
 __global__ __launch_bounds__(256) void k_canon(Batch b) {
     __shared__ int lorder[CANON_CAP], lsorted[CANON_CAP];
     __shared__ int lstart[257];
+    __shared__ uint8_t lcell[CANON_CAP];   // block-local cell of each slot
     for (int blk = blockIdx.x; blk < b.total[KC_CELLS]; blk += gridDim.x) {
         const int s = scene_of(b, KC_CELLS, blk);
         const DevScene &S = b.scenes[s];
@@ -254,18 +255,17 @@ __global__ __launch_bounds__(256) void k_canon(Batch b) {
         for (int q = S0 + (int)threadIdx.x; q < S1; q += 256) lorder[q - S0] = S.order[q];
         lstart[threadIdx.x] = (mine ? s0 : S1) - S0;
         if (threadIdx.x == 0) lstart[256] = S1 - S0;
+        // each cell marks its slots with its block-local index (a slot's cell
+        // without a search over the starts)
+        if (mine)
+            for (int a = s0 - S0; a < s1 - S0; a++) lcell[a] = (uint8_t)threadIdx.x;
         __syncthreads();
         // Rank sort within each cell, one thread per slot: an electron's place is
         // the number of smaller indices in its cell (the indices are distinct).
         // A crowded cell costs each of its lanes one pass over the cell, where a
         // thread's insertion sort would cost it the square.
         for (int a = threadIdx.x; a < S1 - S0; a += 256) {
-            int ci = 0;
-#pragma unroll
-            for (int step = 128; step > 0; step >>= 1) {   // the last cell starting at or before a
-                const int up = ci + step;
-                ci = lstart[up] <= a ? up : ci;
-            }
+            const int ci = lcell[a];
             const int b0 = lstart[ci], b1 = lstart[ci + 1];
             const int key = lorder[a];
             int rank = 0;
