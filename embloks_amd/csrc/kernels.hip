@@ -92,6 +92,35 @@ __device__ __forceinline__ int block_scan_256(int v, int *wsum, int &total) {
     return off + inc - v;
 }
 
+// Inclusive prefix sum over the 64 lanes of a wave.
+__device__ __forceinline__ int wave_scan_incl(int v) {
+    const int lane = threadIdx.x & 63;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int u = __shfl_up(v, o, 64);
+        if (lane >= o) v += u;
+    }
+    return v;
+}
+
+// Block-wide exclusive scan of one value per thread over THREADS threads
+// (every thread of the block calls it); returns the thread's prefix and sets
+// `total`.
+template <int THREADS>
+__device__ __forceinline__ int block_scan(int v, int *wsum, int &total) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int incl = wave_scan_incl(v);
+    if (lane == 63) wsum[w] = incl;
+    __syncthreads();
+    int base = 0, all = 0;
+    for (int q = 0; q < THREADS / 64; q++) {
+        if (q < w) base += wsum[q];
+        all += wsum[q];
+    }
+    __syncthreads();
+    total = all;
+    return base + incl - v;
+}
+
 __global__ __launch_bounds__(256) void k_scan(Batch b) {
     __shared__ int wsum[4];
     for (int blk = blockIdx.x; blk < b.total[KC_SCAN]; blk += gridDim.x) {
@@ -127,33 +156,41 @@ __global__ __launch_bounds__(256) void k_scan(Batch b) {
     }
 }
 
-// One block per scene: the scan tiles' totals become their offsets. Each
-// thread takes TOP_RUN consecutive totals per round (8192 per round), so a
-// 4096^2 grid's 16k tiles take two rounds of the block scan.
-constexpr int TOP_RUN = 32;
-__global__ __launch_bounds__(256) void k_scan_top(Batch b) {
-    __shared__ int wsum[4];
+// One block per scene: the scan tiles' totals become their offsets. The totals
+// are read and written back coalesced through LDS, TOP_CHUNK at a time, and each
+// of the 1024 threads scans TOP_RUN consecutive ones of a chunk (a 4096^2 grid's
+// 16k tiles take two chunks; a 512^2 grid's 256 tiles one, on 32 threads).
+constexpr int TOP_THREADS = 1024, TOP_RUN = 8, TOP_CHUNK = TOP_THREADS * TOP_RUN;
+__global__ __launch_bounds__(TOP_THREADS) void k_scan_top(Batch b) {
+    __shared__ int buf[TOP_CHUNK];
+    __shared__ int wsum[TOP_THREADS / 64];
     for (int blk = blockIdx.x; blk < b.total[KC_SCENE]; blk += gridDim.x) {
         const int s = scene_of(b, KC_SCENE, blk);
         const DevScene &S = b.scenes[s];
         const int ntiles = (S.cols * S.rows + SCAN_TILE - 1) / SCAN_TILE;
         int carry = 0;
-        for (int c0 = 0; c0 < ntiles; c0 += 256 * TOP_RUN) {
-            const int k0 = c0 + (int)threadIdx.x * TOP_RUN;
+        for (int c0 = 0; c0 < ntiles; c0 += TOP_CHUNK) {
+            const int m = min(TOP_CHUNK, ntiles - c0);
+            for (int i = threadIdx.x; i < m; i += TOP_THREADS) buf[i] = S.toff[c0 + i];
+            __syncthreads();
+            const int k0 = (int)threadIdx.x * TOP_RUN;
             int v[TOP_RUN], sum = 0;
 #pragma unroll
             for (int q = 0; q < TOP_RUN; q++) {
-                v[q] = k0 + q < ntiles ? S.toff[k0 + q] : 0;
+                v[q] = k0 + q < m ? buf[k0 + q] : 0;
                 sum += v[q];
             }
             int chunk_total;
-            int run = carry + block_scan_256(sum, wsum, chunk_total);
+            int run = carry + block_scan<TOP_THREADS>(sum, wsum, chunk_total);
 #pragma unroll
             for (int q = 0; q < TOP_RUN; q++) {
-                if (k0 + q < ntiles) S.toff[k0 + q] = run;
+                if (k0 + q < m) buf[k0 + q] = run;
                 run += v[q];
             }
+            __syncthreads();
+            for (int i = threadIdx.x; i < m; i += TOP_THREADS) S.toff[c0 + i] = buf[i];
             carry += chunk_total;
+            __syncthreads();
         }
         if (threadIdx.x == 0) {
             S.ctr->heavy_n = 0;
@@ -737,16 +774,6 @@ __device__ __forceinline__ void kick_core(const DevScene &S, int slot, int k, fl
 
 __device__ void push_one(const DevScene &S, int slot, int k, float h, uint32_t t) {
     push_core(S, S.g_lerp, slot, k, h, t);
-}
-
-// Inclusive prefix sum over the 64 lanes of a wave.
-__device__ __forceinline__ int wave_scan_incl(int v) {
-    const int lane = threadIdx.x & 63;
-    for (int o = 1; o < 64; o <<= 1) {
-        const int u = __shfl_up(v, o, 64);
-        if (lane >= o) v += u;
-    }
-    return v;
 }
 
 // Lanes per electron in the cell mode: the cell's count rounded up to a power of
@@ -1352,23 +1379,6 @@ __device__ __forceinline__ int rest_items(int p, int n, int cells, int ptiles) {
     }
 }
 
-// Block-wide exclusive scan of one value per thread (REST_THREADS threads);
-// returns the thread's prefix and sets `total`.
-__device__ __forceinline__ int rest_block_scan(int v, int *wsum, int &total) {
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int incl = wave_scan_incl(v);
-    if (lane == 63) wsum[w] = incl;
-    __syncthreads();
-    int base = 0, all = 0;
-    for (int q = 0; q < REST_THREADS / 64; q++) {
-        if (q < w) base += wsum[q];
-        all += wsum[q];
-    }
-    __syncthreads();
-    total = all;
-    return base + incl - v;
-}
-
 __global__ __launch_bounds__(REST_THREADS) void k_rest(Batch b) {
     __shared__ int sh_ticket, sh_fail;
     __shared__ int wsum[REST_THREADS / 64];
@@ -1455,7 +1465,7 @@ __global__ __launch_bounds__(REST_THREADS) void k_rest(Batch b) {
                     sum += v[q];
                 }
                 int tile_total;
-                int run = rest_block_scan(sum, wsum, tile_total);
+                int run = block_scan<REST_THREADS>(sum, wsum, tile_total);
                 if (p == 1) {
                     if (tid == 0) S.toff[item] = tile_total;
                 } else {
@@ -1477,7 +1487,7 @@ __global__ __launch_bounds__(REST_THREADS) void k_rest(Batch b) {
                     const int q = q0 + tid;
                     const int v = q < tiles ? S.toff[q] : 0;
                     int chunk;
-                    const int pre = rest_block_scan(v, wsum, chunk);
+                    const int pre = block_scan<REST_THREADS>(v, wsum, chunk);
                     if (q < tiles) S.toff[q] = carry + pre;
                     carry += chunk;
                 }
@@ -2095,7 +2105,7 @@ static inline int grid_of(int total) { return total < 8192 ? total : 8192; }
 hipError_t sort_kernels(const Batch &b, hipStream_t stream) {
     EMB_LAUNCH(k_count, KC_ELECTRONS, 256, b);
     EMB_LAUNCH(k_scan, KC_SCAN, 256, b);
-    EMB_LAUNCH(k_scan_top, KC_SCENE, 256, b);
+    EMB_LAUNCH(k_scan_top, KC_SCENE, TOP_THREADS, b);
     EMB_LAUNCH(k_scatter, KC_ELECTRONS, 256, b);
     EMB_LAUNCH(k_canon, KC_CELLS, 256, b);
     return hipGetLastError();
@@ -2181,7 +2191,7 @@ hipError_t tick_kernels(const Batch &b, int fdtd_substeps, hipStream_t stream, h
     EMB_LAUNCH(k_count, KC_ELECTRONS, 256, b);
     EMB_MARK(3);
     EMB_LAUNCH(k_scan, KC_SCAN, 256, b);
-    EMB_LAUNCH(k_scan_top, KC_SCENE, 256, b);
+    EMB_LAUNCH(k_scan_top, KC_SCENE, TOP_THREADS, b);
     EMB_MARK(4);
     EMB_LAUNCH(k_scatter, KC_ELECTRONS, 256, b);
     EMB_MARK(5);
