@@ -65,7 +65,7 @@ constexpr int PUSH_BLOCKS_MAX = 2048;   // tiled-push blocks per scene: four rou
 constexpr int DEP_TILE = 24;      // cells per side of a deposit + smoothing tile
 constexpr int CELL_MODE_MIN = 6;  // a push tile with a cell this crowded runs the cell mode
 constexpr int HEAVY_CELL = 64;    // a cell this crowded is pushed by k_push_heavy, spread over blocks
-constexpr int TRIM_BLOCKS = 8;    // blocks per battery stack in the trim (trim_block)
+constexpr int TRIM_BLOCKS = 4;    // blocks per battery stack in the trim (trim_block, 1024 threads each)
 constexpr int REST_PHASES = 7;    // work phases of one extra electron substep in k_rest
 constexpr int REST_THREADS = 1024; // threads per k_rest block (the push tiles' layout)
 constexpr int FDTD_STRIP = 60;    // corner columns per FDTD wave (64 lanes: two halo lanes each side)
@@ -235,7 +235,7 @@ enum KernelClass {
     KC_SMOOTH,          // 32 x 32 tiles of the smoothing region
     KC_FDTD,            // FDTD_WAVES strips x fdtd_rows() rows of the (nx+1) x (ny+1) corner space
     KC_GEMM,            // 2 channels x 64 x 64 output tiles of nx x ny
-    KC_GEMM_SMALL,      // the same with small tiles, for scenes with few 64 x 64 tiles (dst_small)
+    KC_GEMM_SMALL,      // the same with small tiles (a batch with few 64 x 64 tiles runs these)
     KC_NODES,           // 256 corners of (nx+1) x (ny+1) per block
     KC_STACKS,          // TRIM_BLOCKS blocks per battery stack (at least one stack per scene)
     KC_SCENE,           // one block per scene

@@ -29,8 +29,9 @@
 //     transposed where the contraction runs over rows) by an O(n^2) pass before
 //     each product, so the O(n^3) kernel only loads bf16 and multiplies.
 // Both stage operands through LDS in K slices with the next slices prefetched
-// into registers; the tick's products of small scenes run 32 x 32 tiles on two
-// waves, the others 64 x 64 tiles (or larger) on a 2 x 2 grid of waves.
+// into registers; the tick's products run 32 x 32 tiles on two waves when the
+// batch has few tiles, else 64 x 64 tiles (the isolated solve's are larger) on
+// a 2 x 2 grid of waves.
 
 #include "device.hpp"
 #include "gemm.hpp"
@@ -335,20 +336,22 @@ __host__ __device__ inline int dst_blocks(int nx, int ny, int bm, int bn) {
     return 2 * (le + lo > re + ro ? le + lo : re + ro);
 }
 
-// Scenes with at most DST_SPLIT_TILES output tiles of 64 x 64 per stage (both
-// channels) do not fill the chip's 256 CUs with them: their products run
-// SMALL_BM x SMALL_BN tiles on blocks of SMALL_GM x SMALL_GN waves instead
-// (KC_GEMM_KS). At 544 nodes the four products took 51 us against 54.5 us with
-// 64 x 64 tiles; 32 x 64 tiles (69 us) and 64 x 64 tiles with K split over the
-// block's four waves and the partial tiles summed in LDS (72 us) were slower.
-constexpr int DST_SPLIT_TILES = 512;
+// A batch whose stages have at most DST_SMALL_TILES output tiles of 64 x 64 (both
+// channels, over all its scenes) does not fill the chip's 256 CUs with them:
+// its products run SMALL_BM x SMALL_BN tiles on blocks of SMALL_GM x SMALL_GN
+// waves instead (KC_GEMM_SMALL). Every element's sum over K runs in the same
+// order either way, so the results have the same bits. At 544 nodes (one
+// scene) the four products took 51 us against 54.5 us with 64 x 64 tiles;
+// 32 x 64 tiles (69 us) and 64 x 64 tiles with K split over the block's four
+// waves and the partial tiles summed in LDS (72 us) were slower. A full batch
+// (the 256-request serving stream) keeps the 64 x 64 tiles: small tiles for its
+// small scenes, in a launch of their own, took its throughput from 2.87 to 2.52
+// G cell-steps/s.
+constexpr int DST_SMALL_TILES = 512;
 constexpr int SMALL_BM = 32, SMALL_BN = 32, SMALL_GM = 2, SMALL_GN = 1;
 
-bool dst_small(int nx, int ny) { return dst_blocks(nx, ny, TICK_BM, TICK_BN) <= DST_SPLIT_TILES; }
-
-int dst_gemm_blocks(int nx, int ny) {
-    return dst_small(nx, ny) ? dst_blocks(nx, ny, SMALL_BM, SMALL_BN) : dst_blocks(nx, ny, TICK_BM, TICK_BN);
-}
+int dst_gemm_blocks(int nx, int ny) { return dst_blocks(nx, ny, TICK_BM, TICK_BN); }
+int dst_gemm_small_blocks(int nx, int ny) { return dst_blocks(nx, ny, SMALL_BM, SMALL_BN); }
 
 // Block `local` of a scene's stage STAGE: its product's arguments and output
 // tile origin, or false when the block has nothing to do.
@@ -426,7 +429,7 @@ __global__ __launch_bounds__(256) void k_dst_gemm(Batch b, int chans) {
     }
 }
 
-// The scenes of dst_small: SMALL_BM x SMALL_BN tiles.
+// A batch with few 64 x 64 tiles: SMALL_BM x SMALL_BN tiles.
 constexpr int SMALL_THREADS = 64 * SMALL_GM * SMALL_GN;
 template <int STAGE>
 __global__ __launch_bounds__(SMALL_THREADS) void k_dst_gemm_small(Batch b, int chans) {
@@ -457,12 +460,14 @@ template __global__ void k_dst_gemm_small<1>(Batch, int);
 template __global__ void k_dst_gemm_small<2>(Batch, int);
 template __global__ void k_dst_gemm_small<3>(Batch, int);
 
-// The four products of one Poisson solve for every scene of the batch: each
-// stage's 64 x 64 tiles and small tiles in a launch of their own (either may be
-// empty), in stage order.
+// The four products of one Poisson solve for every scene of the batch, one
+// launch per stage, in stage order: small tiles when the batch's 64 x 64 tiles
+// would not fill the chip.
 hipError_t launch_dst_gemms(const Batch &b, hipStream_t stream, int chans) {
-    const int whole = std::min(b.total[KC_GEMM], 8192), split = std::min(b.total[KC_GEMM_SMALL], 8192);
-    if (whole + split == 0) return hipSuccess;
+    if (b.total[KC_GEMM] == 0) return hipSuccess;
+    const bool small = b.total[KC_GEMM] <= DST_SMALL_TILES;
+    const int whole = small ? 0 : std::min(b.total[KC_GEMM], 8192);
+    const int split = small ? std::min(b.total[KC_GEMM_SMALL], 8192) : 0;
 #define EMB_DST_STAGE(st)                                                                         \
     if (whole) k_dst_gemm<st><<<whole, 256, 0, stream>>>(b, chans);                               \
     if (split) k_dst_gemm_small<st><<<split, SMALL_THREADS, 0, stream>>>(b, chans);

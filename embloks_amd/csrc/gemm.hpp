@@ -63,9 +63,9 @@ hipError_t launch_gemm_nt_bf16x3(const __bf16 *ah, const __bf16 *al, int lda, co
 // potential alone (read on demand).
 enum { DST_BOTH = 0, DST_TICK = 1, DST_POT = 2 };
 hipError_t launch_dst_gemms(const Batch &b, hipStream_t stream, int chans);
+// (KC_GEMM: 64 x 64 tiles; KC_GEMM_SMALL: the small tiles a batch with few
+// 64 x 64 tiles runs instead)
 int dst_gemm_blocks(int nx, int ny);
-// Whether a scene's products run small tiles (KC_GEMM_SMALL) rather than
-// 64 x 64 ones (KC_GEMM).
-bool dst_small(int nx, int ny);
+int dst_gemm_small_blocks(int nx, int ny);
 
 }  // namespace emb

@@ -1324,6 +1324,13 @@ __global__ __launch_bounds__(HEAVY_THREADS) void k_push_heavy(Batch b, int k, fl
 template __global__ void k_push_heavy<false>(Batch, int, float2 *);
 template __global__ void k_push_heavy<true>(Batch, int, float2 *);
 
+// The trim alone (a batch with more than TRIM_FUSE_MAX trim blocks): the same
+// blocks of PUSH_THREADS threads, so a scene's gains have the same bits either way.
+constexpr int TRIM_FUSE_MAX = 256;
+__global__ __launch_bounds__(PUSH_THREADS) void k_trim(Batch b) {
+    for (int blk = blockIdx.x; blk < b.total[KC_STACKS]; blk += gridDim.x) trim_block<PUSH_THREADS>(b, blk);
+}
+
 // The tick's first kernel: blocks [0, trim_grid) compute the battery trim's
 // terminal potentials (trim_block), the others run the tiled push of substep k.
 // The trim reads only what the tick before left (the potential and the sorted
@@ -2177,10 +2184,14 @@ hipError_t finish_sync_kernel(const Batch &b, hipStream_t stream) {
     } while (0)
 hipError_t tick_kernels(const Batch &b, int fdtd_substeps, hipStream_t stream, hipEvent_t *marks) {
     EMB_MARK(0);
-    {   // the trim's blocks, then the tiled push's (KC_STACKS is never empty)
-        const int trim_grid = grid_of(b.total[KC_STACKS]);
+    {   // the trim's blocks, then the tiled push's (KC_STACKS is never empty);
+        // a batch of many stacks runs its trim alone first (beside the push its
+        // blocks would each hold a push block's LDS)
+        const bool fuse = b.total[KC_STACKS] <= TRIM_FUSE_MAX;
+        if (!fuse) EMB_LAUNCH(k_trim, KC_STACKS, PUSH_THREADS, b);
+        const int trim_grid = fuse ? grid_of(b.total[KC_STACKS]) : 0;
         const int push_grid = b.total[KC_PUSH] > 0 ? grid_of(b.total[KC_PUSH]) : 0;
-        k_push_tiled<<<trim_grid + push_grid, PUSH_THREADS, 0, stream>>>(b, 0, trim_grid);
+        if (trim_grid + push_grid > 0) k_push_tiled<<<trim_grid + push_grid, PUSH_THREADS, 0, stream>>>(b, 0, trim_grid);
     }
     EMB_LAUNCH(k_trim_gain, KC_STACKS, 64, b);
     EMB_LAUNCH(k_push_sparse<false>, KC_ELECTRONS, 256, b, 0, nullptr);

@@ -326,9 +326,8 @@ void scene_blocks(const DevScene &d, int out[KC_COUNT]) {
     out[KC_CELLS] = cdiv(cells, 256);
     out[KC_SMOOTH] = cdiv(d.cols + 2 * d.passes, DEP_TILE) * cdiv(d.rows + 2 * d.passes, DEP_TILE);
     out[KC_FDTD] = cdiv(cdiv(d.ny + 1, FDTD_STRIP), FDTD_WAVES) * cdiv(d.nx + 1, fdtd_rows(d.nx, d.ny));
-    const bool small = dst_small(d.nx, d.ny);
-    out[KC_GEMM] = small ? 0 : dst_gemm_blocks(d.nx, d.ny);
-    out[KC_GEMM_SMALL] = small ? dst_gemm_blocks(d.nx, d.ny) : 0;
+    out[KC_GEMM] = dst_gemm_blocks(d.nx, d.ny);
+    out[KC_GEMM_SMALL] = dst_gemm_small_blocks(d.nx, d.ny);
     out[KC_NODES] = cdiv((long long)(d.nx + 1) * (d.ny + 1), 256);
     out[KC_STACKS] = std::max(1, d.n_stacks) * TRIM_BLOCKS;
     out[KC_SCENE] = 1;
