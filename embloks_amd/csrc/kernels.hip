@@ -241,6 +241,7 @@ __global__ __launch_bounds__(256) void k_canon(Batch b) {
     __shared__ int lorder[CANON_CAP], lsorted[CANON_CAP];
     __shared__ int lstart[257];
     __shared__ uint8_t lcell[CANON_CAP];   // block-local cell of each slot
+    __shared__ int wnew[5];                // new dense tiles per wave, then the block's list base
     for (int blk = blockIdx.x; blk < b.total[KC_CELLS]; blk += gridDim.x) {
         const int s = scene_of(b, KC_CELLS, blk);
         const DevScene &S = b.scenes[s];
@@ -270,12 +271,23 @@ __global__ __launch_bounds__(256) void k_canon(Batch b) {
                 const unsigned bit = 1u << (8 * (tile & 3));
                 first = (atomicOr(reinterpret_cast<unsigned *>(S.tile_dense + (tile & ~3)), bit) & bit) == 0;
             }
-            // the wave's new tiles take consecutive list places: one counter atomic per wave
+            // the block's new tiles take consecutive list places: one counter
+            // atomic per block (atomics on one word queue at its L2 channel)
             const unsigned long long m = __ballot(first);
-            int base = 0;
-            if (lane == 0 && m) base = atomicAdd(&S.ctr->dense_n, __popcll(m));
-            base = __shfl(base, 0, 64);
-            if (first) S.dense_tiles[base + __popcll(m & ((1ull << lane) - 1ull))] = tile;
+            const int w = threadIdx.x >> 6;
+            if (lane == 0) wnew[w] = __popcll(m);
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                int sum = 0;
+                for (int q = 0; q < 4; q++) {
+                    const int c4 = wnew[q];
+                    wnew[q] = sum;
+                    sum += c4;
+                }
+                wnew[4] = sum ? atomicAdd(&S.ctr->dense_n, sum) : 0;
+            }
+            __syncthreads();
+            if (first) S.dense_tiles[wnew[4] + wnew[w] + __popcll(m & ((1ull << lane) - 1ull))] = tile;
         }
         if (mine) {
             S.cstart[c] = s0;
