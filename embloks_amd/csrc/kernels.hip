@@ -749,8 +749,10 @@ __device__ __forceinline__ float2 sparse_close_force(const DevScene &S, const le
 }
 
 // One electron's substep (physics.rs:139-158), one thread per electron: `slot`
-// is its place in the sorted snapshot.
-__device__ void push_core(const DevScene &S, const lerp2 *gt, int slot, int k, float h, uint32_t t) {
+// is its place in the sorted snapshot. Inlined into k_push_sparse: as a call,
+// each electron saved and restored the caller's registers through scratch
+// (420 MB written and read back per launch at 4096^2, 5M electrons).
+__device__ __forceinline__ void push_core(const DevScene &S, const lerp2 *gt, int slot, int k, float h, uint32_t t) {
     const int id = S.order[slot];
     const int home = S.scell[slot];
     const int hc = home / S.rows, hr = home - (home / S.rows) * S.rows;
@@ -784,7 +786,8 @@ __device__ __forceinline__ void kick_core(const DevScene &S, int slot, int k, fl
     finish(S, id, hc, hr, px, py, vx, vy, info.damping, ax, ay, h);
 }
 
-__device__ void push_one(const DevScene &S, int slot, int k, float h, uint32_t t) {
+// (a call from k_rest's queue, whose tile items hold many registers already)
+__device__ __noinline__ void push_one(const DevScene &S, int slot, int k, float h, uint32_t t) {
     push_core(S, S.g_lerp, slot, k, h, t);
 }
 
