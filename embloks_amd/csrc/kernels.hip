@@ -1041,6 +1041,7 @@ struct PushTileLds {
     int cell_e0[PUSH_CELLS], cell_n[PUSH_CELLS];     // tile cells: first sorted slot, electrons
     int col_base[PUSH_WIN + 1], col_g0[PUSH_WIN];   // LDS base and first global slot per column
     int item_pref[PUSH_CELLS + 1];                   // cell mode: prefix of work items per tile cell
+    int item_next;                                   // the pass's next work item, drawn by the waves
 };
 
 // One push tile in cell mode (every thread of the block; the pair table must be
@@ -1111,6 +1112,7 @@ __device__ __forceinline__ void push_tile(const DevScene &S, PushTileLds &L, int
         const int la = a * PUSH_TILE, lz = b * PUSH_TILE;
         const int base = col_base[ca], staged_n = col_base[cb] - base;
         const bool staged = staged_n <= PUSH_STAGE;
+        if (threadIdx.x == 0) L.item_next = item_pref[la];
         if (staged) {
             for (int q = threadIdx.x; q < (cb - ca) * W; q += THREADS) {
                 const int ci = ca + q / W, ri = q - (q / W) * W;
@@ -1138,7 +1140,15 @@ __device__ __forceinline__ void push_tile(const DevScene &S, PushTileLds &L, int
             }
         }
         __syncthreads();
-        for (int it = item_pref[la] + wave; it < item_pref[lz]; it += WAVES) {
+        // Each wave draws its next item from an LDS counter as it finishes the
+        // last, so the waves end the pass together although items differ in
+        // cost (a cell's last item is short); any wave computes an item alike.
+        const int it_end = item_pref[lz];
+        for (;;) {
+            int it = 0;
+            if (lane == 0) it = atomicAdd(&L.item_next, 1);
+            it = __builtin_amdgcn_readfirstlane(it);
+            if (it >= it_end) break;
             int lo = la, hi = lz;
             while (hi - lo > 1) {
                 const int mid = (lo + hi) >> 1;
