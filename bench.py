@@ -108,7 +108,7 @@ def run_single(args):
 # launch and its gain kernel before the kick.
 PHASE_KERNELS = {"push": ["k_push_tiled", "k_trim_gain", "k_push_sparse<false>", "k_push_heavy<false>"], "rest": ["k_rest"],
                  "count": ["k_count"], "scan": ["k_scan", "k_scan_top"], "scatter": ["k_scatter"], "canon": ["k_canon"],
-                 "deposit_smooth": ["k_deposit_smooth"], "fdtd": ["k_fdtd"], "correct": ["k_correct"],
+                 "deposit_smooth": ["k_deposit_smooth<512>"], "fdtd": ["k_fdtd"], "correct": ["k_correct"],
                  "finish": ["k_finish"],
                  "dst_gemm": [f"k_dst_gemm<{i}>" for i in range(4)]}
 

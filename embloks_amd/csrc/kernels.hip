@@ -1584,7 +1584,12 @@ __global__ __launch_bounds__(REST_THREADS) void k_rest(Batch b) {
 constexpr int ST = DEP_TILE;    // output tile
 constexpr int MAX_PASSES = 4;
 constexpr int SW = ST + 2 * MAX_PASSES;   // LDS row pitch: window of the tile plus a halo of `passes`
-constexpr int DEP_THREADS = 512;    // two blocks per CU at up to 128 VGPRs: 289 tiles at 512^2 all resident at once
+// Threads per deposit block: 512 (two blocks per CU at up to 128 VGPRs) where
+// most tiles hold many electrons, 256 (four per CU) where most take the
+// electron path below and a block's latency chain is short: the 4096^2 wire
+// scene's deposit 0.88 -> 0.68 ms, while the 512^2 scene's took 46 -> 67 us.
+// The results do not depend on the choice.
+constexpr int DEP_THREADS_DENSE = 512, DEP_THREADS_SPARSE = 256;
 
 // x rounded to the nearest integer (ties to even) as int64: __float2ll_rn, or,
 // when the caller knows |x| < 2^31 (SMALL), the same value through the 32-bit
@@ -1598,10 +1603,11 @@ __device__ __forceinline__ long long rint_ll(float x) {
 // per thread (12 atomics each) instead of one source cell per thread: in a
 // sparse scene most source cells are empty, and a thread per cell walks them
 // all for a handful of electrons.
-constexpr int DEP_SPARSE_MAX = 2 * DEP_THREADS;
+constexpr int DEP_SPARSE_MAX = 1024;
 constexpr int DEP_SRC = ST + 2 * MAX_PASSES + 2;   // source columns of a tile at most
 
-// (held to 128 VGPRs: two blocks per CU)
+// (held to 128 VGPRs)
+template <int DEP_THREADS>
 __global__ __launch_bounds__(DEP_THREADS) __attribute__((amdgpu_waves_per_eu(4))) void k_deposit_smooth(Batch b,
                                                                                                    int bias) {
     __shared__ long long dep[3][SW * SW];
@@ -2154,6 +2160,15 @@ static inline int grid_of(int total) { return total < 8192 ? total : 8192; }
         if (b.total[kc] > 0) kernel<<<grid_of(b.total[kc]), threads, 0, stream>>>(__VA_ARGS__); \
     } while (0)
 
+// The deposit with the block size that suits the batch: the wide blocks when
+// its electrons average at least 512 per tile (KC_ELECTRONS counts blocks of 256).
+static void launch_deposit(const Batch &b, hipStream_t stream, int bias) {
+    if (b.total[KC_ELECTRONS] >= 2 * b.total[KC_SMOOTH])
+        EMB_LAUNCH(k_deposit_smooth<DEP_THREADS_DENSE>, KC_SMOOTH, DEP_THREADS_DENSE, b, bias);
+    else
+        EMB_LAUNCH(k_deposit_smooth<DEP_THREADS_SPARSE>, KC_SMOOTH, DEP_THREADS_SPARSE, b, bias);
+}
+
 // Counting sort of the electrons by cell: counts, scan, scatter, canonical order.
 hipError_t sort_kernels(const Batch &b, hipStream_t stream) {
     EMB_LAUNCH(k_count, KC_ELECTRONS, 256, b);
@@ -2169,7 +2184,7 @@ hipError_t sort_kernels(const Batch &b, hipStream_t stream) {
 hipError_t setup_kernels(const Batch &b, hipStream_t stream) {
     hipError_t e = sort_kernels(b, stream);
     if (e != hipSuccess) return e;
-    EMB_LAUNCH(k_deposit_smooth, KC_SMOOTH, DEP_THREADS, b, -1);
+    launch_deposit(b, stream, -1);
     // Gauss's law enforced twice (Maxwell::enforce_gauss, maxwell.rs:176-193).
     // From fields that are far from it (a new scene starts at E = 0) the first
     // pass solves a full potential, whose FP32 rounding stays in E; the second
@@ -2254,7 +2269,7 @@ hipError_t tick_kernels(const Batch &b, int fdtd_substeps, hipStream_t stream, h
     EMB_MARK(5);
     EMB_LAUNCH(k_canon, KC_CELLS, 256, b);
     EMB_MARK(6);
-    EMB_LAUNCH(k_deposit_smooth, KC_SMOOTH, DEP_THREADS, b, 0);
+    launch_deposit(b, stream, 0);
     EMB_MARK(7);
     for (int f = 0; f < fdtd_substeps; f++) EMB_LAUNCH(k_fdtd, KC_FDTD, 64 * FDTD_WAVES, b, f);
     EMB_MARK(8);
@@ -2274,7 +2289,7 @@ extern const char *const TICK_PHASE_NAMES[] = {"push", "rest", "count", "scan", 
                                         "deposit_smooth", "fdtd", "dst_gemm", "correct", "finish"};
 
 hipError_t deposit_kernel(const Batch &b, hipStream_t stream) {
-    EMB_LAUNCH(k_deposit_smooth, KC_SMOOTH, DEP_THREADS, b, -1);
+    launch_deposit(b, stream, -1);
     return hipGetLastError();
 }
 
