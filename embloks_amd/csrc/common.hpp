@@ -251,6 +251,7 @@ struct Batch {
     const int32_t *prefix;      // device: KC_COUNT * (n_scenes + 1)
     int32_t n_scenes;
     int32_t total[KC_COUNT];    // blocks per class over the batch
+    int32_t even_fdtd;          // some scene takes an even number of FDTD substeps (its tick ends in k_finish)
 };
 
 }  // namespace emb

@@ -2077,6 +2077,34 @@ __global__ __launch_bounds__(256) void k_rhs(Batch b) {
 // solve carries only one rounding of its right-hand side. The first channel,
 // P(-4 pi k rho), is the stored potential. With `tick` set, the scene's tick
 // counter advances here, after everything that reads it.
+// End of a tick for one cell (finish_tick, fields.wgsl:270-281): the displayed
+// E and B at the cell centre and the smoothed (ammeter) cell current move
+// towards their instantaneous values by alpha. sync = 1 (finish_sync,
+// fields.wgsl:284-290, after a scene is set up or edited): the display jumps
+// to the exact fields and the current average is left alone.
+__device__ __forceinline__ void finish_cell(const DevScene &S, int cell, float ex, float ey, float bz, int sync) {
+    const int cells = S.cols * S.rows;
+    const float a = sync ? 1.0f : S.alpha_display;
+    float *d = S.display;
+    d[cell] += a * (ex - d[cell]);
+    d[cells + cell] += a * (ey - d[cells + cell]);
+    d[2 * cells + cell] += a * (bz - d[2 * cells + cell]);
+    if (!sync) {
+        for (int k = 0; k < 2; k++) {
+            const float now = (float)S.cur[k * cells + cell] * S.i_from_fixed;
+            S.cur_smooth[k * cells + cell] += S.alpha_current * (now - S.cur_smooth[k * cells + cell]);
+        }
+    }
+}
+
+//
+// In a tick, a scene whose fields came from the second buffer (an odd number
+// of FDTD substeps, so the correction's sources are not overwritten by it)
+// also takes the end of its tick here (finish_cell): a node thread inside the
+// visible cells forms the corrected E of its cell's far edges, (i + 1, j) and
+// (i, j + 1), as the threads owning them do, and reads Bz from the buffer it
+// is copied from, so the averages are those k_finish would form after this
+// kernel, without a launch and a second read of the fields.
 __global__ __launch_bounds__(256) void k_correct(Batch b, int tick) {
     for (int blk = blockIdx.x; blk < b.total[KC_NODES]; blk += gridDim.x) {
         const int s = scene_of(b, KC_NODES, blk);
@@ -2092,15 +2120,31 @@ __global__ __launch_bounds__(256) void k_correct(Batch b, int tick) {
         const int i = q / (ny + 1), j = q - (q / (ny + 1)) * (ny + 1);
         const size_t pnodes = (size_t)nx * S.pny;
         auto p = [&](int a, int c) { return S.phi[pnodes + (size_t)a * S.pny + c]; };
-        if (j < ny) {
-            const float high = i < nx ? p(i, j) : 0.0f, low = i > 0 ? p(i - 1, j) : 0.0f;
-            S.ex[(size_t)i * ny + j] = ex[(size_t)i * ny + j] - (high - low);
+        // Corrected Ex at (a, c), c < ny, and Ey at (a, c), a < nx.
+        auto cex = [&](int a, int c) {
+            const float high = a < nx ? p(a, c) : 0.0f, low = a > 0 ? p(a - 1, c) : 0.0f;
+            return ex[(size_t)a * ny + c] - (high - low);
+        };
+        auto cey = [&](int a, int c) {
+            const float high = c < ny ? p(a, c) : 0.0f, low = c > 0 ? p(a, c - 1) : 0.0f;
+            return ey[(size_t)a * (ny + 1) + c] - (high - low);
+        };
+        float exn = 0.0f, eyn = 0.0f;
+        if (j < ny) S.ex[(size_t)i * ny + j] = exn = cex(i, j);
+        if (i < nx) S.ey[(size_t)i * (ny + 1) + j] = eyn = cey(i, j);
+        if (alt) {
+            S.bz[q] = S.bz2[q];
+            const int c = i - S.margin, r = j - S.margin;
+            if (c >= 0 && r >= 0 && c < S.cols && r < S.rows) {
+                const float *bz = S.bz2;
+                const size_t np = (size_t)ny + 1;
+                const float exc = 0.5f * (exn + cex(i + 1, j));
+                const float eyc = 0.5f * (eyn + cey(i, j + 1));
+                const float bzc = 0.25f * (bz[(size_t)i * np + j] + bz[(size_t)(i + 1) * np + j] + bz[(size_t)i * np + j + 1] +
+                                           bz[(size_t)(i + 1) * np + j + 1]);
+                finish_cell(S, c * S.rows + r, exc, eyc, bzc, 0);
+            }
         }
-        if (i < nx) {
-            const float high = j < ny ? p(i, j) : 0.0f, low = j > 0 ? p(i, j - 1) : 0.0f;
-            S.ey[(size_t)i * (ny + 1) + j] = ey[(size_t)i * (ny + 1) + j] - (high - low);
-        }
-        if (alt) S.bz[q] = S.bz2[q];
     }
 }
 
@@ -2199,15 +2243,14 @@ hipError_t setup_kernels(const Batch &b, hipStream_t stream) {
     return hipGetLastError();
 }
 
-// End of a tick (finish_tick, fields.wgsl:270-281): the displayed E and B at
-// the cell centres and the smoothed (ammeter) cell current move towards their
-// instantaneous values by alpha. sync = 1 (finish_sync, fields.wgsl:284-290,
-// after a scene is set up or edited): the display jumps to the exact fields
-// and the current average is left alone.
+// The end of a tick (finish_cell) at every cell of the scenes whose k_correct
+// did not take it (sync, or an even number of FDTD substeps).
 __global__ __launch_bounds__(256) void k_finish(Batch b, int sync) {
     for (int blk = blockIdx.x; blk < b.total[KC_CELLS]; blk += gridDim.x) {
         const int s = scene_of(b, KC_CELLS, blk);
         const DevScene &S = b.scenes[s];
+        // (a tick's k_correct did this for a scene after an odd number of FDTD substeps)
+        if (!sync && (S.fdtd_substeps & 1)) continue;
         const int cells = S.cols * S.rows;
         const int cell = block_in_scene(b, KC_CELLS, s, blk) * 256 + (int)threadIdx.x;
         if (cell >= cells) continue;
@@ -2218,17 +2261,7 @@ __global__ __launch_bounds__(256) void k_finish(Batch b, int sync) {
         const float ey = 0.5f * (S.ey[(size_t)i * (ny + 1) + j] + S.ey[(size_t)i * (ny + 1) + j + 1]);
         const float bz = 0.25f * (S.bz[(size_t)i * (ny + 1) + j] + S.bz[(size_t)(i + 1) * (ny + 1) + j] +
                                   S.bz[(size_t)i * (ny + 1) + j + 1] + S.bz[(size_t)(i + 1) * (ny + 1) + j + 1]);
-        const float a = sync ? 1.0f : S.alpha_display;
-        float *d = S.display;
-        d[cell] += a * (ex - d[cell]);
-        d[cells + cell] += a * (ey - d[cells + cell]);
-        d[2 * cells + cell] += a * (bz - d[2 * cells + cell]);
-        if (!sync) {
-            for (int k = 0; k < 2; k++) {
-                const float now = (float)S.cur[k * cells + cell] * S.i_from_fixed;
-                S.cur_smooth[k * cells + cell] += S.alpha_current * (now - S.cur_smooth[k * cells + cell]);
-            }
-        }
+        finish_cell(S, cell, ex, ey, bz, sync);
     }
 }
 
@@ -2278,7 +2311,7 @@ hipError_t tick_kernels(const Batch &b, int fdtd_substeps, hipStream_t stream, h
     EMB_MARK(9);
     EMB_LAUNCH(k_correct, KC_NODES, 256, b, 1);
     EMB_MARK(10);
-    EMB_LAUNCH(k_finish, KC_CELLS, 256, b, 0);
+    if (b.even_fdtd) EMB_LAUNCH(k_finish, KC_CELLS, 256, b, 0);
     EMB_MARK(11);
     return hipGetLastError();
 }

@@ -451,6 +451,7 @@ int make_resident(EmbDevice *dev, const EmbSettings &s, HostScene &&hs_in, hipSt
     r.batch.scenes = r.d_scene;
     r.batch.prefix = r.d_prefix;
     r.batch.n_scenes = 1;
+    r.batch.even_fdtd = r.ds.fdtd_substeps % 2 == 0;
     r.tab_floats = (size_t)TICK_TABLE * (hs.ac_freq.size() + hs.stacks.size());
     if (r.tab_floats > 0) {
         EMB_HIP(hipHostMalloc((void **)&r.pin_tab, r.tab_floats * sizeof(float), hipHostMallocDefault));
