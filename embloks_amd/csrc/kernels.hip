@@ -252,6 +252,14 @@ __global__ __launch_bounds__(256) void k_canon(Batch b) {
         const int c = c0 + (int)threadIdx.x;
         const bool mine = c < c1;
         const int s0 = mine ? scan_start(S, c) : 0, s1 = mine ? scan_start(S, c + 1) : 0;
+        const bool in_lds = S1 - S0 <= CANON_CAP;
+        // The block's slots into LDS first: their loads are in flight during the
+        // dense-tile atomics below, whose results each wait for.
+        if (in_lds) {
+            for (int q = S0 + (int)threadIdx.x; q < S1; q += 256) lorder[q - S0] = S.order[q];
+            lstart[threadIdx.x] = (mine ? s0 : S1) - S0;
+            if (threadIdx.x == 0) lstart[256] = S1 - S0;
+        }
         // A push tile with a crowded cell runs the cell mode (k_push_tiled);
         // the others go electron by electron (k_push_sparse). The first cell to
         // flag its tile lists it (the flag's byte set in its word by an atomic
@@ -296,14 +304,11 @@ __global__ __launch_bounds__(256) void k_canon(Batch b) {
             // does not matter, each cell's result depends on the cell alone.
             if (s1 - s0 >= HEAVY_CELL) S.heavy_cells[atomicAdd(&S.ctr->heavy_n, 1)] = c;
         }
-        if (S1 - S0 > CANON_CAP) {
+        if (!in_lds) {
             if (mine) canon_cell(S, c, s0, s1);
             __syncthreads();
             continue;
         }
-        for (int q = S0 + (int)threadIdx.x; q < S1; q += 256) lorder[q - S0] = S.order[q];
-        lstart[threadIdx.x] = (mine ? s0 : S1) - S0;
-        if (threadIdx.x == 0) lstart[256] = S1 - S0;
         // each cell marks its slots with its block-local index (a slot's cell
         // without a search over the starts)
         if (mine)
