@@ -107,7 +107,7 @@ def run_single(args):
 # The push phase holds the battery trim too: its blocks run in k_push_tiled's
 # launch and its gain kernel before the kick.
 PHASE_KERNELS = {"push": ["k_push_tiled", "k_trim_gain", "k_push_sparse<false>", "k_push_heavy<false>"], "rest": ["k_rest"],
-                 "count": ["k_count"], "scan": ["k_scan", "k_scan_top"], "scatter": ["k_scatter"], "canon": ["k_canon"],
+                 "count": ["k_count"], "scan": ["k_scan"], "scatter": ["k_scatter<true>"], "canon": ["k_canon"],
                  "deposit_smooth": ["k_deposit_smooth<512>"], "fdtd": ["k_fdtd"], "correct": ["k_correct"],
                  "finish": ["k_finish"],
                  "dst_gemm": [f"k_dst_gemm<{i}>" for i in range(4)]}

@@ -189,7 +189,8 @@ struct DevScene {
     int32_t *ecell, *erank;   // per electron: cell and rank within it
     int32_t *count;           // per cell (zero between sorts)
     int32_t *lstart;          // per cell: start within its scan tile
-    int32_t *toff;            // per scan tile: offset of the tile
+    int32_t *toff;            // per scan tile: its electron count (k_scan), then its offset (k_rest's queue)
+    int32_t *toff_pre;        // per scan tile: offset of the tile (the main sort's top-level scan)
     int32_t *cstart;          // per cell + 1: absolute start in the sorted order
     uint8_t *tile_dense;      // per push tile: a cell holds >= CELL_MODE_MIN electrons
     int32_t *dense_tiles;     // the tiles with tile_dense set (ctr->dense_n of them), in no particular order
@@ -252,6 +253,7 @@ struct Batch {
     int32_t n_scenes;
     int32_t total[KC_COUNT];    // blocks per class over the batch
     int32_t even_fdtd;          // some scene takes an even number of FDTD substeps (its tick ends in k_finish)
+    int32_t top_small;          // every scene has at most 256 scan tiles (k_scatter scans their totals itself)
 };
 
 }  // namespace emb

@@ -40,7 +40,7 @@ __device__ __forceinline__ int cell_of(const DevScene &S, int32_t cxy) {
 __device__ __forceinline__ int scan_start(const DevScene &S, int cell) {
     const int cells = S.cols * S.rows;
     if (cell >= cells) return S.n;
-    return S.lstart[cell] + S.toff[cell / SCAN_TILE];
+    return S.lstart[cell] + S.toff_pre[cell / SCAN_TILE];
 }
 
 __device__ __forceinline__ unsigned long long as_ull(long long v) { return (unsigned long long)v; }

@@ -188,7 +188,7 @@ __global__ __launch_bounds__(TOP_THREADS) void k_scan_top(Batch b) {
                 run += v[q];
             }
             __syncthreads();
-            for (int i = threadIdx.x; i < m; i += TOP_THREADS) S.toff[c0 + i] = buf[i];
+            for (int i = threadIdx.x; i < m; i += TOP_THREADS) S.toff_pre[c0 + i] = buf[i];
             carry += chunk_total;
             __syncthreads();
         }
@@ -200,12 +200,41 @@ __global__ __launch_bounds__(TOP_THREADS) void k_scan_top(Batch b) {
     }
 }
 
+// Each electron to its slot: its cell's start plus its rank there. With TOP
+// (every scene of the batch has at most 256 scan tiles), each block first scans
+// the tile totals itself, in place of k_scan_top's launch, and the scene's
+// first block stores the offsets for k_canon and clears the list counters.
+template <bool TOP>
 __global__ __launch_bounds__(256) void k_scatter(Batch b) {
+    __shared__ int ltoff[256];
+    __shared__ int wsum[4];
     for (int blk = blockIdx.x; blk < b.total[KC_ELECTRONS]; blk += gridDim.x) {
         const int s = scene_of(b, KC_ELECTRONS, blk);
         const DevScene &S = b.scenes[s];
-        const int i = block_in_scene(b, KC_ELECTRONS, s, blk) * 256 + (int)threadIdx.x;
-        if (i < S.n) S.order[scan_start(S, S.ecell[i]) + S.erank[i]] = i;
+        const int local = block_in_scene(b, KC_ELECTRONS, s, blk);
+        const int i = local * 256 + (int)threadIdx.x;
+        if (TOP) {
+            const int cells = S.cols * S.rows, ntiles = (cells + SCAN_TILE - 1) / SCAN_TILE;
+            const int v = (int)threadIdx.x < ntiles ? S.toff[threadIdx.x] : 0;
+            int total;
+            const int pre = block_scan_256(v, wsum, total);
+            ltoff[threadIdx.x] = pre;
+            if (local == 0) {
+                if ((int)threadIdx.x < ntiles) S.toff_pre[threadIdx.x] = pre;
+                if (threadIdx.x == 0) {
+                    S.ctr->heavy_n = 0;
+                    S.ctr->dense_n = 0;
+                }
+            }
+            __syncthreads();
+            if (i < S.n) {
+                const int cell = S.ecell[i];
+                S.order[S.lstart[cell] + ltoff[cell / SCAN_TILE] + S.erank[i]] = i;
+            }
+            __syncthreads();   // the next block's scan reuses ltoff
+        } else if (i < S.n) {
+            S.order[scan_start(S, S.ecell[i]) + S.erank[i]] = i;
+        }
     }
 }
 
@@ -2222,8 +2251,12 @@ static void launch_deposit(const Batch &b, hipStream_t stream, int bias) {
 hipError_t sort_kernels(const Batch &b, hipStream_t stream) {
     EMB_LAUNCH(k_count, KC_ELECTRONS, 256, b);
     EMB_LAUNCH(k_scan, KC_SCAN, 256, b);
-    EMB_LAUNCH(k_scan_top, KC_SCENE, TOP_THREADS, b);
-    EMB_LAUNCH(k_scatter, KC_ELECTRONS, 256, b);
+    if (b.top_small) {
+        EMB_LAUNCH(k_scatter<true>, KC_ELECTRONS, 256, b);
+    } else {
+        EMB_LAUNCH(k_scan_top, KC_SCENE, TOP_THREADS, b);
+        EMB_LAUNCH(k_scatter<false>, KC_ELECTRONS, 256, b);
+    }
     EMB_LAUNCH(k_canon, KC_CELLS, 256, b);
     return hipGetLastError();
 }
@@ -2301,9 +2334,12 @@ hipError_t tick_kernels(const Batch &b, int fdtd_substeps, hipStream_t stream, h
     EMB_LAUNCH(k_count, KC_ELECTRONS, 256, b);
     EMB_MARK(3);
     EMB_LAUNCH(k_scan, KC_SCAN, 256, b);
-    EMB_LAUNCH(k_scan_top, KC_SCENE, TOP_THREADS, b);
+    if (!b.top_small) EMB_LAUNCH(k_scan_top, KC_SCENE, TOP_THREADS, b);
     EMB_MARK(4);
-    EMB_LAUNCH(k_scatter, KC_ELECTRONS, 256, b);
+    if (b.top_small)
+        EMB_LAUNCH(k_scatter<true>, KC_ELECTRONS, 256, b);
+    else
+        EMB_LAUNCH(k_scatter<false>, KC_ELECTRONS, 256, b);
     EMB_MARK(5);
     EMB_LAUNCH(k_canon, KC_CELLS, 256, b);
     EMB_MARK(6);

@@ -101,6 +101,7 @@ void layout(Bump &a, const Counts &c, DevScene &d, DevScene **d_scene, int32_t *
     d.count = a.take<int32_t>(cells);
     d.lstart = a.take<int32_t>(cells);
     d.toff = a.take<int32_t>((cells + SCAN_TILE - 1) / SCAN_TILE);
+    d.toff_pre = a.take<int32_t>((cells + SCAN_TILE - 1) / SCAN_TILE);
     d.cstart = a.take<int32_t>(cells + 1);
     const size_t ptiles = (size_t)((c.cols + PUSH_TILE - 1) / PUSH_TILE) * ((c.rows + PUSH_TILE - 1) / PUSH_TILE);
     d.tile_dense = a.take<uint8_t>(ptiles);
@@ -321,7 +322,7 @@ size_t scene_bytes(const EmbSettings &s, int cols, int rows, int64_t n, int n_st
 void scene_blocks(const DevScene &d, int out[KC_COUNT]) {
     const long long cells = (long long)d.cols * d.rows;
     auto cdiv = [](long long a, long long b) { return (int)((a + b - 1) / b); };
-    out[KC_ELECTRONS] = cdiv(d.n, 256);
+    out[KC_ELECTRONS] = std::max(1, cdiv(d.n, 256));   // (k_scatter<true> runs the scene's top-level scan even with no electrons)
     out[KC_SCAN] = cdiv(cells, SCAN_TILE);
     out[KC_CELLS] = cdiv(cells, 256);
     out[KC_SMOOTH] = cdiv(d.cols + 2 * d.passes, DEP_TILE) * cdiv(d.rows + 2 * d.passes, DEP_TILE);
@@ -452,6 +453,7 @@ int make_resident(EmbDevice *dev, const EmbSettings &s, HostScene &&hs_in, hipSt
     r.batch.prefix = r.d_prefix;
     r.batch.n_scenes = 1;
     r.batch.even_fdtd = r.ds.fdtd_substeps % 2 == 0;
+    r.batch.top_small = (r.ds.cols * r.ds.rows + SCAN_TILE - 1) / SCAN_TILE <= 256;
     r.tab_floats = (size_t)TICK_TABLE * (hs.ac_freq.size() + hs.stacks.size());
     if (r.tab_floats > 0) {
         EMB_HIP(hipHostMalloc((void **)&r.pin_tab, r.tab_floats * sizeof(float), hipHostMallocDefault));

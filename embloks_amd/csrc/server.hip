@@ -106,6 +106,7 @@ int rebuild_batch(EmbServer *sv) {
     prefix.assign((size_t)KC_COUNT * (n + 1), 0);
     sv->fdtd_max = 1;
     sv->batch.even_fdtd = 0;
+    sv->batch.top_small = 1;
     for (int k = 0; k < KC_COUNT; k++) sv->batch.total[k] = 0;
     for (int i = 0; i < n; i++) {
         const DevScene &d = sv->active[(size_t)i]->r.ds;
@@ -115,6 +116,7 @@ int rebuild_batch(EmbServer *sv) {
         for (int k = 0; k < KC_COUNT; k++) prefix[(size_t)k * (n + 1) + i + 1] = prefix[(size_t)k * (n + 1) + i] + blocks[k];
         sv->fdtd_max = std::max(sv->fdtd_max, d.fdtd_substeps);
         if (d.fdtd_substeps % 2 == 0) sv->batch.even_fdtd = 1;
+        if ((d.cols * d.rows + SCAN_TILE - 1) / SCAN_TILE > 256) sv->batch.top_small = 0;
     }
     for (int k = 0; k < KC_COUNT; k++) sv->batch.total[k] = prefix[(size_t)k * (n + 1) + n];
     if (n > 0) {
