@@ -413,12 +413,17 @@ def run_serving(stream, batched: bool):
     t0 = time.perf_counter()
     pending = list(stream)
     active = 0
+    shown = 0.0
     while pending or active:
         now = time.perf_counter() - t0
         while pending and pending[0][0] <= now:
             _, steps, w = pending.pop(0)
             srv.submit(w, steps)
         active = srv.poll()
+        if now - shown >= 30.0:   # a progress line on stderr for long streams
+            shown = now
+            print(f"serving ({'batched' if batched else 'plain'}): {now:.0f} s, {active} active, {len(pending)} to arrive",
+                  file=sys.stderr, flush=True)
         if not active and pending:
             time.sleep(max(0.0, min(0.001, pending[0][0] - (time.perf_counter() - t0))))
     st = srv.stats()
