@@ -1418,11 +1418,12 @@ __global__ __launch_bounds__(PUSH_THREADS) void k_trim(Batch b) {
 // blocks overlap the push instead of running before it.
 __global__ __launch_bounds__(PUSH_THREADS) __attribute__((amdgpu_waves_per_eu(8))) void k_push_tiled(Batch b, int k,
                                                                                                   int trim_grid) {
-    if ((int)blockIdx.x < trim_grid) {
-        for (int blk = blockIdx.x; blk < b.total[KC_STACKS]; blk += trim_grid) trim_block<PUSH_THREADS>(b, blk);
+    const int push_grid = (int)gridDim.x - trim_grid;
+    if ((int)blockIdx.x >= push_grid) {
+        for (int blk = (int)blockIdx.x - push_grid; blk < b.total[KC_STACKS]; blk += trim_grid) trim_block<PUSH_THREADS>(b, blk);
         return;
     }
-    push_tiled_body<ITEM_STORE, PUSH_THREADS>(b, k, nullptr, (int)blockIdx.x - trim_grid, (int)gridDim.x - trim_grid);
+    push_tiled_body<ITEM_STORE, PUSH_THREADS>(b, k, nullptr, (int)blockIdx.x, push_grid);
 }
 
 __global__ __launch_bounds__(FORCE_THREADS) void k_force_tiled(Batch b, float2 *force_out) {
@@ -2316,9 +2317,10 @@ hipError_t finish_sync_kernel(const Batch &b, hipStream_t stream) {
     } while (0)
 hipError_t tick_kernels(const Batch &b, int fdtd_substeps, hipStream_t stream, hipEvent_t *marks) {
     EMB_MARK(0);
-    {   // the trim's blocks, then the tiled push's (KC_STACKS is never empty);
-        // a batch of many stacks runs its trim alone first (beside the push its
-        // blocks would each hold a push block's LDS)
+    {   // the tiled push's blocks, then the trim's (KC_STACKS is never empty),
+        // which start as the push's last blocks leave CUs idle; a batch of many
+        // stacks runs its trim alone first (beside the push its blocks would
+        // each hold a push block's LDS)
         const bool fuse = b.total[KC_STACKS] <= TRIM_FUSE_MAX;
         if (!fuse) EMB_LAUNCH(k_trim, KC_STACKS, PUSH_THREADS, b);
         const int trim_grid = fuse ? grid_of(b.total[KC_STACKS]) : 0;
